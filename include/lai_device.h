@@ -1,0 +1,159 @@
+/*
+ * lai_device.h -- C ABI of the MI355X (gfx950) Rys ERI / Fock-build backend.
+ *
+ * This is the boundary the C++ host layer (Engine, GpuFockBuilder, FockBuilder)
+ * calls.  Plain pointers, sizes and status codes only; no C++ or HIP types.
+ * Every function returns LAI_OK (0) or a negative status; the message of the
+ * last failure on the calling thread is available from lai_dev_last_error().
+ * No exception crosses this boundary.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository brycewestheimer/libaccint-public).
+ *
+ * Pointer conventions: "d_" arguments are device pointers (allocated with
+ * lai_dev_malloc or any HIP allocator), everything else is host memory.
+ * "stream" is a hipStream_t passed as void* (NULL = the default stream).
+ * Matrices are row-major nbf x nbf doubles.  Shell order, function offsets
+ * and Cartesian component order follow the reference BasisSet
+ * (src/shared/basis/basis_set.cpp:161-185,
+ *  include/libaccint/math/cartesian_indices.hpp:26-35).
+ */
+#ifndef LAI_DEVICE_H
+#define LAI_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LAI_OK 0
+#define LAI_ERR_INVALID_ARGUMENT (-1) /* -> InvalidArgumentException */
+#define LAI_ERR_INVALID_STATE (-2)    /* -> InvalidStateException */
+#define LAI_ERR_MEMORY (-3)           /* -> MemoryException / std::runtime_error */
+#define LAI_ERR_BACKEND (-4)          /* -> BackendError (HIP runtime failure) */
+#define LAI_ERR_UNSUPPORTED (-5)      /* -> std::invalid_argument (angular momentum) */
+
+/* J/K build flags */
+#define LAI_JK_FP64 0  /* every quartet in FP64 */
+#define LAI_JK_MIXED 1 /* quartets marked FP32 by the worklist run the FP32 recursion */
+
+#define LAI_MAX_L 3 /* highest shell angular momentum of the energy kernels (f) */
+
+typedef struct lai_ctx lai_ctx;
+typedef struct lai_basis lai_basis;
+
+/* ---- context --------------------------------------------------------------
+ * Replaces CudaEngine construction (src/device/cuda/engine/cuda_engine.cu:183-187:
+ * BackendError on init failure) and the Boys/Rys constant uploads of
+ * src/device/cuda/math/boys_device.cu. */
+int lai_dev_init(int device, lai_ctx** out);
+int lai_dev_fini(lai_ctx* ctx);
+const char* lai_dev_last_error(void);
+int lai_dev_device_count(int* n);
+
+/* ---- memory helpers (replace include/libaccint/memory/device_memory.hpp) -- */
+int lai_dev_malloc(lai_ctx* ctx, size_t bytes, void** d_ptr);
+int lai_dev_free(lai_ctx* ctx, void* d_ptr);
+int lai_dev_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream);
+int lai_dev_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream);
+/* zero n doubles with a fill kernel (graph-capture safe; replaces
+ * matrix_zero_kernel, src/device/cuda/consumers/fock_builder_gpu.cu:99-105) */
+int lai_dev_zero(void* d_ptr, size_t n_doubles, void* stream);
+int lai_dev_stream_sync(void* stream);
+
+/* ---- basis ------------------------------------------------------------------
+ * Uploads shells (normalised coefficients, src/shared/basis/shell.cpp:32-113)
+ * and builds the primitive-pair tables of every shell pair.  Replaces
+ * ShellSetDeviceCache::get_or_upload (src/device/cuda/basis/device_data.cu:42-90)
+ * and ShellSetPair::build_pair_data_impl (src/shared/basis/shell_set_pair.cpp:128-181).
+ * Zero-coefficient primitives are dropped from the pair tables (exact). */
+int lai_dev_upload_basis(lai_ctx* ctx, int n_shells, const int* L, const double* centers,
+                         const int* nprim, const double* exps, const double* coefs,
+                         lai_basis** out);
+int lai_dev_basis_free(lai_basis* b);
+int lai_dev_basis_nbf(const lai_basis* b);
+
+/* ---- raw ERIs ---------------------------------------------------------------
+ * (ab|cd) blocks for n arbitrary shell quartets (shells4 = 4n shell indices),
+ * block t written row-major [na][nb][nc][nd] at out + offsets[t] (host
+ * buffers; offsets has n+1 entries and is filled by the call).
+ * Replaces kernels::cuda::dispatch_eri_kernel + the D2H of
+ * CudaEngine::compute_eri_batch_device (src/device/cuda/kernels/eri_kernel.cu:635-689,
+ * src/device/cuda/engine/cuda_engine.cu:1233-1277) and, on the CPU side,
+ * kernels::compute_eri (src/host/kernels/eri_kernel.cpp:168-381). */
+int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* shells4, double* out,
+                         int64_t* offsets);
+
+/* ---- Schwarz screening -------------------------------------------------------
+ * Q_ij = sqrt(max_ab |(ab|ab)|) for every shell pair i <= j, packed at
+ * j*(j+1)/2 + i (src/shared/screening/schwarz_bounds.cpp:20-59,
+ * include/libaccint/screening/schwarz_bounds.hpp:110-113).  Computed on the
+ * device and kept there; copied to Q_packed when it is not NULL. */
+int lai_dev_schwarz(lai_basis* b, double* Q_packed);
+/* use a caller-provided Q table instead (same packing) */
+int lai_dev_set_schwarz(lai_basis* b, const double* Q_packed);
+
+/* ---- worklist ---------------------------------------------------------------
+ * Canonical quartets i<=j, k<=l, pair(ij) <= pair(kl) that pass
+ * Q_ij*Q_kl >= thr (ScreeningOptions::passes_screening,
+ * include/libaccint/screening/screening_options.hpp:147-149; loop
+ * include/libaccint/engine/engine.hpp:1815-1819), sorted by angular-momentum
+ * class.  thr <= 0 keeps every quartet (no Schwarz table needed).
+ * Quartets with Q_ij*Q_kl*dmax < tau_fp32 are marked for the FP32 recursion
+ * (used by LAI_JK_MIXED; tau_fp32 <= 0 marks none).  Returns the quartet
+ * count, the FP32 count and an order-independent checksum of the (i,j,k,l)
+ * list: sum over quartets of mix64(i,j,k,l) modulo 2^64 (mix64 in
+ * lai_quartet_mix64).  Replaces the QuartetTask list of
+ * compute_and_consume_screened_parallel_impl (engine.hpp:1877-1923). */
+int lai_dev_build_worklist(lai_basis* b, double thr, double tau_fp32, double dmax,
+                           int64_t* n_quartets, int64_t* n_fp32, uint64_t* checksum);
+uint64_t lai_quartet_mix64(int i, int j, int k, int l);
+/* Per-class quartet counts and algorithmic work of the current worklist.
+ * Classes: pair class p(La,Lb) = La*(La+1)/2 + Lb for La >= Lb; quartet class
+ * pb*(pb+1)/2 + pk for bra pair class pb >= ket pair class pk (55 classes up
+ * to f).  flops_fp64/flops_fp32 sum K_q * F_prim over the FP64 / FP32 parts,
+ * with K_q the product of non-zero-coefficient primitive counts and
+ * F_prim = 30 + n_r (50 + 3 F_dir + 3 N_cart) in the reference orientation
+ * of each canonical quartet; n_eri sums na*nb*nc*nd. */
+int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, double* flops_fp64,
+                           double* flops_fp32, int64_t* n_eri);
+
+/* ---- J/K build ---------------------------------------------------------------
+ * d_J += J[D], d_K += K[D] over the current worklist (D symmetric, device).
+ * J_mn = sum (mn|ls) D_ls, K_ml = sum (mn|ls) D_ns, i.e. the result of
+ * FockBuilder over the full basis (src/shared/consumers/fock_builder.cpp:64-222).
+ * Replaces GpuFockBuilder::accumulate_device_eri_batch +
+ * fock_accumulate_batch_kernel (src/device/cuda/consumers/fock_builder_gpu.cu:116-180,600-621)
+ * with digestion fused into the ERI kernels.  Asynchronous on stream. */
+int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream);
+
+/* Capture the same J/K build into a hipGraph bound to these pointers, and
+ * replay it (the per-SCF-iteration Fock build). */
+int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags,
+                             void* stream);
+int lai_dev_jk_graph_launch(lai_basis* b, void* stream);
+
+/* ---- one-electron integrals ---------------------------------------------------
+ * kind 0 = overlap S, 1 = kinetic T, 2 = nuclear attraction V over point
+ * charges (Z, xyz[3*n_charges]).  Writes the full nbf x nbf matrix to d_out.
+ * Replaces overlap/kinetic/nuclear/fused_1e kernels
+ * (src/device/cuda/kernels/{overlap,kinetic,nuclear,fused_1e}_kernel.cu) and
+ * CpuEngine::compute_1e_parallel_impl (src/host/engine/cpu_engine.cpp:482-593). */
+int lai_dev_onee(lai_basis* b, int kind, int n_charges, const double* Z, const double* xyz,
+                 double* d_out, void* stream);
+
+/* ---- math probes (parity tests) ----------------------------------------------
+ * Rys roots/weights for n_roots at n points T (device arrays), as used by
+ * every kernel.  Replaces rys_quadrature_impl (src/device/cuda/math/rys_device.cuh:194-234). */
+int lai_dev_rys(int n_roots, int n, const double* d_T, double* d_roots, double* d_weights,
+                void* stream);
+/* Boys F_0..F_nmax at n points (device arrays, out [n][nmax+1]); replaces
+ * boys_evaluate_array_device (src/device/cuda/math/boys_device.cuh:223-300). */
+int lai_dev_boys(int nmax, int n, const double* d_T, double* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,841 @@
+// api.hip -- implementation of the C ABI in include/lai_device.h: context,
+// device memory helpers, basis upload with primitive-pair tables, Schwarz
+// bounds, the screened AM-sorted worklist, the fused J/K build (plain launch
+// and hipGraph), raw ERI blocks, one-electron integrals and math probes.
+#include "lai_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "aux_kernels.hpp"
+#include "eri_kernels.hpp"
+#include "launch.hpp"
+
+using namespace lai;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define LAI_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(LAI_ERR_BACKEND, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Launchers {
+  ClassLauncher f64[kNumClasses] = {};
+  ClassLauncher f32[kNumClasses] = {};
+  ClassInfo info[kNumClasses] = {};
+  Launchers() {
+    fill_launchers_part_0(f64, f32, info);
+    fill_launchers_part_1(f64, f32, info);
+    fill_launchers_part_2(f64, f32, info);
+    fill_launchers_part_3(f64, f32, info);
+    fill_launchers_part_4(f64, f32, info);
+    fill_launchers_part_5(f64, f32, info);
+    fill_launchers_part_6(f64, f32, info);
+    fill_launchers_part_7(f64, f32, info);
+  }
+};
+const Launchers& launchers() {
+  static Launchers L;
+  return L;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t assign(const T* host, size_t count) {
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, sizeof(T) * count);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = count;
+    return hipMemcpy(p, host, sizeof(T) * count, hipMemcpyHostToDevice);
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, sizeof(T) * count);
+    if (e != hipSuccess) p = nullptr;
+    else n = count;
+    return e;
+  }
+};
+
+inline int pair_index(int ns, int i, int j) {  // i <= j, reference loop order
+  return i * ns - i * (i - 1) / 2 + (j - i);
+}
+
+uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// F_dir of the reference recursion (src/host/kernels/eri_kernel.cpp:119-163):
+// VRR terms 2t-1, two flops per horizontal-transfer entry (bra transfer on
+// the d = 0 slice only).
+double fdir(int La, int Lb, int Lc, int Ld) {
+  const int LAB = La + Lb, LCD = Lc + Ld;
+  double f = 0;
+  for (int a = 0; a < LAB; ++a) f += 2 * (1 + (a > 0)) - 1;
+  for (int c = 0; c < LCD; ++c)
+    for (int a = 0; a <= LAB; ++a) f += 2 * (1 + (c > 0) + (a > 0)) - 1;
+  for (int b = 0; b < Lb; ++b) f += 2.0 * (LAB - b) * (LCD + 1);
+  for (int d = 0; d < Ld; ++d) f += 2.0 * (La + 1) * (Lb + 1) * (LCD - d);
+  return f;
+}
+double fprim(int La, int Lb, int Lc, int Ld) {
+  const int nr = (La + Lb + Lc + Ld) / 2 + 1;
+  const double ncart4 = double(ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld));
+  return 30.0 + nr * (30.0 + 20.0 + 3.0 * fdir(La, Lb, Lc, Ld) + 3.0 * ncart4);
+}
+
+int hw_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 4;
+  return int(std::min(n, 32u));
+}
+
+template <typename F>
+void parallel_for(int n, F&& f) {
+  const int nt = std::min(hw_threads(), std::max(1, n));
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t]() {
+      for (;;) {
+        const int i = next.fetch_add(16);
+        if (i >= n) break;
+        for (int k = i; k < std::min(n, i + 16); ++k) f(k, t);
+      }
+    });
+  for (auto& x : th) x.join();
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// opaque types
+// ---------------------------------------------------------------------------
+struct lai_ctx {
+  int device = 0;
+  int n_cu = 256;
+  DevBuf<double> rys;
+};
+
+struct ClassList {
+  DevBuf<Quartet> q64, q32;
+  int64_t n64 = 0, n32 = 0;
+};
+
+struct lai_basis {
+  lai_ctx* ctx = nullptr;
+  int ns = 0, nbf = 0;
+  std::vector<int> L, nprim, prim_off, func_off;
+  std::vector<double> center, exps, coefs;
+  // pairs in reference order (i <= j)
+  std::vector<PairRec> pairs_h;
+  std::vector<int> pair_cls;  // pair class of each pair
+  std::vector<int> pi, pj;    // shells of each pair
+  DevBuf<PairRec> pairs;
+  DevBuf<PrimPair> pp;
+  // device copies of shell data for the one-electron kernel
+  DevBuf<int> dL, dnprim, dprim_off, dfunc_off;
+  DevBuf<double> dcenter, dexps, dcoefs;
+  // Schwarz
+  std::vector<double> Q;  // packed j*(j+1)/2+i
+  bool have_Q = false;
+  // worklist
+  ClassList cls[kNumClasses];
+  bool have_worklist = false;
+  int64_t counts[kNumClasses] = {};
+  double flops64 = 0, flops32 = 0;
+  int64_t n_eri = 0;
+  // J/K scratch (unsymmetrised)
+  DevBuf<double> Jt, Kt;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// context and memory
+// ---------------------------------------------------------------------------
+extern "C" const char* lai_dev_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int lai_dev_device_count(int* n) {
+  if (!n) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_device_count: null output");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *n = c;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_init(int device, lai_ctx** out) {
+  if (!out) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_init: null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(LAI_ERR_BACKEND, "lai_dev_init: no HIP device available");
+  if (device < 0 || device >= count)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_init: device index out of range");
+  LAI_HIP(hipSetDevice(device));
+  auto* ctx = new lai_ctx;
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+  std::vector<double> tab(LAI_RYS_TABLE_SIZE);
+  std::memcpy(tab.data(), lai_rys_cheb_h, sizeof(double) * LAI_RYS_CHEB_SIZE);
+  for (int i = 0; i < int(sizeof(lai_rys_hx2) / sizeof(double)); ++i) {
+    tab[LAI_RYS_HX2_OFF + i] = lai_rys_hx2[i];
+    tab[LAI_RYS_HW_OFF + i] = lai_rys_hw[i];
+  }
+  hipError_t e = ctx->rys.assign(tab.data(), tab.size());
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(LAI_ERR_BACKEND, std::string("lai_dev_init: table upload: ") + hipGetErrorString(e));
+  }
+  (void)launchers();
+  *out = ctx;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_fini(lai_ctx* ctx) {
+  if (!ctx) return LAI_OK;
+  ctx->rys.release();
+  delete ctx;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_malloc(lai_ctx*, size_t bytes, void** p) {
+  if (!p) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_malloc: null output");
+  *p = nullptr;
+  if (bytes == 0) return LAI_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(LAI_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_free(lai_ctx*, void* p) {
+  if (p) LAI_HIP(hipFree(p));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return LAI_OK;
+  LAI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, hipStream_t(stream)));
+  LAI_HIP(hipStreamSynchronize(hipStream_t(stream)));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return LAI_OK;
+  LAI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, hipStream_t(stream)));
+  LAI_HIP(hipStreamSynchronize(hipStream_t(stream)));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_zero(void* p, size_t n, void* stream) {
+  if (n == 0) return LAI_OK;
+  if (!p) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_zero: null pointer");
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(blocks)), dim3(256), 0, hipStream_t(stream),
+                     static_cast<double*>(p), n);
+  LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_stream_sync(void* stream) {
+  LAI_HIP(hipStreamSynchronize(hipStream_t(stream)));
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// basis
+// ---------------------------------------------------------------------------
+extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const double* centers,
+                                    const int* nprim, const double* exps, const double* coefs,
+                                    lai_basis** out) {
+  if (!ctx || !out || ns < 0 || (ns > 0 && (!L || !centers || !nprim || !exps || !coefs)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_basis: invalid arguments");
+  if (ns > 65535) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_basis: more than 65535 shells");
+  *out = nullptr;
+  for (int s = 0; s < ns; ++s) {
+    if (L[s] < 0 || L[s] > kMaxL)
+      return fail(LAI_ERR_UNSUPPORTED, "lai_dev_upload_basis: angular momentum " + std::to_string(L[s]) +
+                                           " of shell " + std::to_string(s) + " outside [0, 3]");
+    if (nprim[s] <= 0) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_basis: shell without primitives");
+  }
+  auto* b = new lai_basis;
+  b->ctx = ctx;
+  b->ns = ns;
+  b->L.assign(L, L + ns);
+  b->nprim.assign(nprim, nprim + ns);
+  b->center.assign(centers, centers + 3 * ns);
+  int np = 0, nbf = 0;
+  for (int s = 0; s < ns; ++s) {
+    b->prim_off.push_back(np);
+    b->func_off.push_back(nbf);
+    np += nprim[s];
+    nbf += ncart(L[s]);
+  }
+  b->nbf = nbf;
+  b->exps.assign(exps, exps + np);
+  b->coefs.assign(coefs, coefs + np);
+
+  // pair records and primitive-pair tables
+  const double sq2pi52 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+  const int npairs = ns * (ns + 1) / 2;
+  b->pairs_h.resize(npairs);
+  b->pair_cls.resize(npairs);
+  b->pi.resize(npairs);
+  b->pj.resize(npairs);
+  std::vector<PrimPair> pph;
+  int p = 0;
+  for (int i = 0; i < ns; ++i)
+    for (int j = i; j < ns; ++j, ++p) {
+      const int sa = (L[i] >= L[j]) ? i : j, sb = (sa == i) ? j : i;
+      PairRec r{};
+      r.sa = sa;
+      r.sb = sb;
+      r.fa = b->func_off[sa];
+      r.fb = b->func_off[sb];
+      r.same = (i == j);
+      const double* A = &centers[3 * sa];
+      const double* B = &centers[3 * sb];
+      r.ABx = A[0] - B[0];
+      r.ABy = A[1] - B[1];
+      r.ABz = A[2] - B[2];
+      const double AB2 = r.ABx * r.ABx + r.ABy * r.ABy + r.ABz * r.ABz;
+      r.pp_off = int(pph.size());
+      for (int x = 0; x < nprim[sa]; ++x) {
+        const double al = exps[b->prim_off[sa] + x], ca = coefs[b->prim_off[sa] + x];
+        if (ca == 0.0) continue;
+        for (int y = 0; y < nprim[sb]; ++y) {
+          const double be = exps[b->prim_off[sb] + y], cb = coefs[b->prim_off[sb] + y];
+          if (cb == 0.0) continue;
+          const double zeta = al + be, iz = 1.0 / zeta, mu = al * be * iz;
+          PrimPair q;
+          q.zeta = zeta;
+          q.Px = (al * A[0] + be * B[0]) * iz;
+          q.Py = (al * A[1] + be * B[1]) * iz;
+          q.Pz = (al * A[2] + be * B[2]) * iz;
+          q.PAx = q.Px - A[0];
+          q.PAy = q.Py - A[1];
+          q.PAz = q.Pz - A[2];
+          q.Kc = ca * cb * std::exp(-mu * AB2) * sq2pi52 * iz;
+          pph.push_back(q);
+        }
+      }
+      r.npp = int(pph.size()) - r.pp_off;
+      b->pairs_h[p] = r;
+      b->pair_cls[p] = pair_class(L[sa], L[sb]);
+      b->pi[p] = i;
+      b->pj[p] = j;
+    }
+  hipError_t e;
+  if ((e = b->pairs.assign(b->pairs_h.data(), b->pairs_h.size())) != hipSuccess ||
+      (e = b->pp.assign(pph.data(), pph.size())) != hipSuccess ||
+      (e = b->dL.assign(b->L.data(), ns)) != hipSuccess ||
+      (e = b->dnprim.assign(b->nprim.data(), ns)) != hipSuccess ||
+      (e = b->dprim_off.assign(b->prim_off.data(), ns)) != hipSuccess ||
+      (e = b->dfunc_off.assign(b->func_off.data(), ns)) != hipSuccess ||
+      (e = b->dcenter.assign(b->center.data(), b->center.size())) != hipSuccess ||
+      (e = b->dexps.assign(b->exps.data(), b->exps.size())) != hipSuccess ||
+      (e = b->dcoefs.assign(b->coefs.data(), b->coefs.size())) != hipSuccess ||
+      (e = b->Jt.alloc(size_t(nbf) * nbf)) != hipSuccess ||
+      (e = b->Kt.alloc(size_t(nbf) * nbf)) != hipSuccess) {
+    lai_dev_basis_free(b);
+    return fail(LAI_ERR_MEMORY, std::string("lai_dev_upload_basis: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_basis_free(lai_basis* b) {
+  if (!b) return LAI_OK;
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->pairs.release();
+  b->pp.release();
+  b->dL.release();
+  b->dnprim.release();
+  b->dprim_off.release();
+  b->dfunc_off.release();
+  b->dcenter.release();
+  b->dexps.release();
+  b->dcoefs.release();
+  b->Jt.release();
+  b->Kt.release();
+  for (auto& c : b->cls) {
+    c.q64.release();
+    c.q32.release();
+  }
+  delete b;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_basis_nbf(const lai_basis* b) { return b ? b->nbf : -1; }
+
+// ---------------------------------------------------------------------------
+// raw ERI blocks
+// ---------------------------------------------------------------------------
+namespace {
+// Orders a shell quartet into class order and returns the class id, the
+// (bra, ket) pair indices and the permutation code (class position p ->
+// original position, 2 bits each).
+int classify(const lai_basis* b, const int s[4], Quartet& q, int& perm) {
+  const int ns = b->ns;
+  auto pidx = [&](int x, int y) { return x <= y ? pair_index(ns, x, y) : pair_index(ns, y, x); };
+  const int p0 = pidx(s[0], s[1]), p1 = pidx(s[2], s[3]);
+  const int c0 = b->pair_cls[p0], c1 = b->pair_cls[p1];
+  const bool swap_bk = c0 < c1;
+  const int pb = swap_bk ? p1 : p0, pk = swap_bk ? p0 : p1;
+  const int ob = swap_bk ? 2 : 0, ok = swap_bk ? 0 : 2;  // original positions of bra/ket pairs
+  const PairRec& rb = b->pairs_h[pb];
+  const PairRec& rk = b->pairs_h[pk];
+  // class position 0 = rb.sa: original position ob if s[ob] == rb.sa else ob+1
+  const int o0 = (s[ob] == rb.sa) ? ob : ob + 1;
+  const int o1 = (o0 == ob) ? ob + 1 : ob;
+  const int o2 = (s[ok] == rk.sa) ? ok : ok + 1;
+  const int o3 = (o2 == ok) ? ok + 1 : ok;
+  perm = o0 | (o1 << 2) | (o2 << 4) | (o3 << 6);
+  q.bra = pb;
+  q.ket = pk;
+  const int cb = std::max(c0, c1), ck = std::min(c0, c1);
+  return cb * (cb + 1) / 2 + ck;
+}
+
+struct EriBatch {
+  std::vector<Quartet> q[kNumClasses];
+  std::vector<int> perm[kNumClasses];
+  std::vector<long long> obase[kNumClasses];
+};
+
+// Runs mode-1 kernels for a classified batch writing into d_out (device).
+int run_eri_batch(lai_basis* b, EriBatch& eb, double* d_out, hipStream_t st) {
+  const auto& Ls = launchers();
+  for (int c = 0; c < kNumClasses; ++c) {
+    const int nq = int(eb.q[c].size());
+    if (nq == 0) continue;
+    DevBuf<Quartet> dq;
+    DevBuf<int> dp;
+    DevBuf<long long> dob;
+    hipError_t e;
+    if ((e = dq.assign(eb.q[c].data(), nq)) != hipSuccess || (e = dp.assign(eb.perm[c].data(), nq)) != hipSuccess ||
+        (e = dob.assign(eb.obase[c].data(), nq)) != hipSuccess)
+      return fail(LAI_ERR_MEMORY, std::string("eri batch: ") + hipGetErrorString(e));
+    EriArgs a{};
+    a.q = dq.p;
+    a.nq = nq;
+    a.pairs = b->pairs.p;
+    a.pp = b->pp.p;
+    a.rys = b->ctx->rys.p;
+    a.mode = 1;
+    a.out = d_out;
+    a.perm = dp.p;
+    a.obase = dob.p;
+    a.nbf = b->nbf;
+    Ls.f64[c](a, st, b->ctx->n_cu * 16);
+    LAI_HIP(hipGetLastError());
+    LAI_HIP(hipStreamSynchronize(st));
+  }
+  return LAI_OK;
+}
+}  // namespace
+
+extern "C" int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* s4, double* out, int64_t* offsets) {
+  if (!b || n < 0 || (n > 0 && (!s4 || !out || !offsets)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: invalid arguments");
+  EriBatch eb;
+  int64_t off = 0;
+  for (int64_t t = 0; t < n; ++t) {
+    const int* s = s4 + 4 * t;
+    for (int k = 0; k < 4; ++k)
+      if (s[k] < 0 || s[k] >= b->ns) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: shell index out of range");
+    Quartet q;
+    int perm;
+    const int c = classify(b, s, q, perm);
+    offsets[t] = off;
+    eb.q[c].push_back(q);
+    eb.perm[c].push_back(perm);
+    eb.obase[c].push_back(off);
+    off += int64_t(ncart(b->L[s[0]])) * ncart(b->L[s[1]]) * ncart(b->L[s[2]]) * ncart(b->L[s[3]]);
+  }
+  offsets[n] = off;
+  DevBuf<double> d;
+  hipError_t e = d.alloc(size_t(off));
+  if (e != hipSuccess) return fail(LAI_ERR_MEMORY, "lai_dev_eri_quartets: output allocation");
+  int rc = run_eri_batch(b, eb, d.p, nullptr);
+  if (rc != LAI_OK) return rc;
+  LAI_HIP(hipMemcpy(out, d.p, sizeof(double) * off, hipMemcpyDeviceToHost));
+  d.release();
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz
+// ---------------------------------------------------------------------------
+extern "C" int lai_dev_schwarz(lai_basis* b, double* Qout) {
+  if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_schwarz: null basis");
+  const int ns = b->ns, npairs = ns * (ns + 1) / 2;
+  EriBatch eb;
+  std::vector<long long> offs(npairs);
+  std::vector<int> ni(npairs), nj(npairs);
+  long long off = 0;
+  for (int p = 0; p < npairs; ++p) {
+    const int i = b->pi[p], j = b->pj[p];
+    const int s[4] = {i, j, i, j};
+    Quartet q;
+    int perm;
+    const int c = classify(b, s, q, perm);
+    eb.q[c].push_back(q);
+    eb.perm[c].push_back(perm);
+    eb.obase[c].push_back(off);
+    offs[p] = off;
+    ni[p] = ncart(b->L[i]);
+    nj[p] = ncart(b->L[j]);
+    off += (long long)(ni[p] * nj[p]) * (ni[p] * nj[p]);
+  }
+  DevBuf<double> d, dQ;
+  DevBuf<long long> doff;
+  DevBuf<int> dni, dnj;
+  hipError_t e;
+  if ((e = d.alloc(size_t(off))) != hipSuccess || (e = dQ.alloc(npairs)) != hipSuccess ||
+      (e = doff.assign(offs.data(), npairs)) != hipSuccess || (e = dni.assign(ni.data(), npairs)) != hipSuccess ||
+      (e = dnj.assign(nj.data(), npairs)) != hipSuccess)
+    return fail(LAI_ERR_MEMORY, std::string("lai_dev_schwarz: ") + hipGetErrorString(e));
+  int rc = run_eri_batch(b, eb, d.p, nullptr);
+  if (rc != LAI_OK) return rc;
+  hipLaunchKernelGGL(schwarz_reduce_kernel, dim3((npairs + 127) / 128), dim3(128), 0, nullptr, d.p, doff.p,
+                     dni.p, dnj.p, dQ.p, npairs);
+  LAI_HIP(hipGetLastError());
+  std::vector<double> Qp(npairs);
+  LAI_HIP(hipMemcpy(Qp.data(), dQ.p, sizeof(double) * npairs, hipMemcpyDeviceToHost));
+  // reference packing j*(j+1)/2 + i
+  b->Q.assign(npairs, 0.0);
+  for (int p = 0; p < npairs; ++p) {
+    const int i = b->pi[p], j = b->pj[p];
+    b->Q[size_t(j) * (j + 1) / 2 + i] = Qp[p];
+  }
+  b->have_Q = true;
+  if (Qout) std::memcpy(Qout, b->Q.data(), sizeof(double) * npairs);
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_set_schwarz(lai_basis* b, const double* Q) {
+  if (!b || !Q) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_schwarz: invalid arguments");
+  b->Q.assign(Q, Q + size_t(b->ns) * (b->ns + 1) / 2);
+  b->have_Q = true;
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// worklist
+// ---------------------------------------------------------------------------
+extern "C" uint64_t lai_quartet_mix64(int i, int j, int k, int l) {
+  const uint64_t key = (uint64_t(uint16_t(i)) << 48) | (uint64_t(uint16_t(j)) << 32) |
+                       (uint64_t(uint16_t(k)) << 16) | uint64_t(uint16_t(l));
+  return splitmix64(key);
+}
+
+extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, double tau, double dmax, int64_t* n_out,
+                                      int64_t* n32_out, uint64_t* checksum) {
+  if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_build_worklist: null basis");
+  const bool screen = thr > 0.0;
+  if (screen && !b->have_Q)
+    return fail(LAI_ERR_INVALID_STATE, "lai_dev_build_worklist: Schwarz bounds not computed");
+  const int ns = b->ns, npairs = ns * (ns + 1) / 2;
+  std::vector<double> Qp(npairs, 1.0);
+  if (b->have_Q)
+    for (int p = 0; p < npairs; ++p) {
+      const int i = b->pi[p], j = b->pj[p];
+      Qp[p] = b->Q[size_t(j) * (j + 1) / 2 + i];
+    }
+  const bool mixed = tau > 0.0 && b->have_Q;
+  const int nth = hw_threads();
+  // per-thread per-class buckets, per bra pair order kept by pass structure
+  struct Bucket {
+    std::vector<Quartet> q64[kNumClasses], q32[kNumClasses];
+    uint64_t sum = 0;
+    int64_t counts[kNumClasses] = {};
+    double f64 = 0, f32 = 0;
+    int64_t neri = 0;
+  };
+  std::vector<Bucket> bk(nth);
+  // class F_prim table in the reference orientation (computed per quartet)
+  const auto& Ls = launchers();
+  (void)Ls;
+  double fp_tab[kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
+  for (int x = 0; x <= kMaxL; ++x)
+    for (int y = 0; y <= kMaxL; ++y)
+      for (int z = 0; z <= kMaxL; ++z)
+        for (int w = 0; w <= kMaxL; ++w) fp_tab[x][y][z][w] = fprim(x, y, z, w);
+  std::vector<int> npp(npairs);
+  for (int p = 0; p < npairs; ++p) npp[p] = b->pairs_h[p].npp;
+  parallel_for(npairs, [&](int p, int t) {
+    Bucket& B = bk[t];
+    const double qp = Qp[p];
+    const int i = b->pi[p], j = b->pj[p];
+    const int cp = b->pair_cls[p];
+    for (int r = p; r < npairs; ++r) {
+      const double prod = qp * Qp[r];
+      if (screen && !(prod >= thr)) continue;
+      const int cr = b->pair_cls[r];
+      Quartet q;
+      int id;
+      if (cp >= cr) {
+        q.bra = p;
+        q.ket = r;
+        id = cp * (cp + 1) / 2 + cr;
+      } else {
+        q.bra = r;
+        q.ket = p;
+        id = cr * (cr + 1) / 2 + cp;
+      }
+      const bool f32 = mixed && prod * dmax < tau;
+      (f32 ? B.q32[id] : B.q64[id]).push_back(q);
+      const int k = b->pi[r], l = b->pj[r];
+      B.sum += lai_quartet_mix64(i, j, k, l);
+      B.counts[id]++;
+      const double fl = double(npp[p]) * npp[r] * fp_tab[b->L[i]][b->L[j]][b->L[k]][b->L[l]];
+      (f32 ? B.f32 : B.f64) += fl;
+      B.neri += int64_t(ncart(b->L[i])) * ncart(b->L[j]) * ncart(b->L[k]) * ncart(b->L[l]);
+    }
+  });
+  uint64_t sum = 0;
+  int64_t total = 0, total32 = 0;
+  for (int c = 0; c < kNumClasses; ++c) b->counts[c] = 0;
+  b->flops64 = b->flops32 = 0;
+  b->n_eri = 0;
+  for (auto& B : bk) {
+    sum += B.sum;
+    b->flops64 += B.f64;
+    b->flops32 += B.f32;
+    b->n_eri += B.neri;
+    for (int c = 0; c < kNumClasses; ++c) b->counts[c] += B.counts[c];
+  }
+  for (int c = 0; c < kNumClasses; ++c) {
+    std::vector<Quartet> a64, a32;
+    size_t n64 = 0, n32 = 0;
+    for (auto& B : bk) {
+      n64 += B.q64[c].size();
+      n32 += B.q32[c].size();
+    }
+    a64.reserve(n64);
+    a32.reserve(n32);
+    for (auto& B : bk) {
+      a64.insert(a64.end(), B.q64[c].begin(), B.q64[c].end());
+      a32.insert(a32.end(), B.q32[c].begin(), B.q32[c].end());
+      std::vector<Quartet>().swap(B.q64[c]);
+      std::vector<Quartet>().swap(B.q32[c]);
+    }
+    hipError_t e;
+    if ((e = b->cls[c].q64.assign(a64.data(), a64.size())) != hipSuccess ||
+        (e = b->cls[c].q32.assign(a32.data(), a32.size())) != hipSuccess)
+      return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
+    b->cls[c].n64 = int64_t(a64.size());
+    b->cls[c].n32 = int64_t(a32.size());
+    total += b->cls[c].n64 + b->cls[c].n32;
+    total32 += b->cls[c].n32;
+  }
+  b->have_worklist = true;
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (n_out) *n_out = total;
+  if (n32_out) *n32_out = total32;
+  if (checksum) *checksum = sum;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, double* f64, double* f32,
+                                      int64_t* neri) {
+  if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_worklist_stats: null basis");
+  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_worklist_stats: no worklist");
+  if (counts) std::memcpy(counts, b->counts, sizeof(b->counts));
+  if (f64) *f64 = b->flops64;
+  if (f32) *f32 = b->flops32;
+  if (neri) *neri = b->n_eri;
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// J/K build
+// ---------------------------------------------------------------------------
+extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
+  if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: invalid arguments");
+  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk: worklist not built");
+  if (flags != LAI_JK_FP64 && flags != LAI_JK_MIXED)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: unknown flags");
+  hipStream_t st = hipStream_t(stream);
+  const size_t nn = size_t(b->nbf) * b->nbf;
+  int rc;
+  if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
+  if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
+  const auto& Ls = launchers();
+  EriArgs a{};
+  a.pairs = b->pairs.p;
+  a.pp = b->pp.p;
+  a.rys = b->ctx->rys.p;
+  a.D = d_D;
+  a.Jt = b->Jt.p;
+  a.Kt = b->Kt.p;
+  a.nbf = b->nbf;
+  a.mode = 0;
+  const int max_blocks = b->ctx->n_cu * 16;
+  for (int c = kNumClasses - 1; c >= 0; --c) {  // heavy classes first
+    if (b->cls[c].n64 > 0) {
+      a.q = b->cls[c].q64.p;
+      a.nq = int(b->cls[c].n64);
+      Ls.f64[c](a, st, max_blocks);
+    }
+    if (b->cls[c].n32 > 0) {
+      a.q = b->cls[c].q32.p;
+      a.nq = int(b->cls[c].n32);
+      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
+    }
+  }
+  LAI_HIP(hipGetLastError());
+  const unsigned blocks = unsigned(std::min<size_t>((nn + 255) / 256, 8192));
+  hipLaunchKernelGGL(symmetrize_add_kernel, dim3(blocks), dim3(256), 0, st, b->Jt.p, b->Kt.p, d_J, d_K, b->nbf);
+  LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags,
+                                        void* stream) {
+  if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_graph_capture: null basis");
+  if (!stream) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_graph_capture: capture needs a non-default stream");
+  hipStream_t st = hipStream_t(stream);
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  LAI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = lai_dev_jk(b, d_D, d_J, d_K, flags, stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != LAI_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(LAI_ERR_BACKEND, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  LAI_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_jk_graph_launch(lai_basis* b, void* stream) {
+  if (!b || !b->graph_exec) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_graph_launch: no captured graph");
+  LAI_HIP(hipGraphLaunch(b->graph_exec, hipStream_t(stream)));
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one-electron integrals
+// ---------------------------------------------------------------------------
+extern "C" int lai_dev_onee(lai_basis* b, int kind, int nch, const double* Z, const double* xyz, double* d_out,
+                            void* stream) {
+  if (!b || !d_out || kind < 0 || kind > 2 || (kind == 2 && nch > 0 && (!Z || !xyz)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_onee: invalid arguments");
+  DevBuf<double> dZ, dxyz;
+  hipError_t e;
+  if (kind == 2 && nch > 0) {
+    if ((e = dZ.assign(Z, nch)) != hipSuccess || (e = dxyz.assign(xyz, size_t(3) * nch)) != hipSuccess)
+      return fail(LAI_ERR_MEMORY, "lai_dev_onee: charge upload");
+  }
+  OneEArgs a{};
+  a.n_shells = b->ns;
+  a.L = b->dL.p;
+  a.prim_off = b->dprim_off.p;
+  a.nprim = b->dnprim.p;
+  a.func_off = b->dfunc_off.p;
+  a.center = b->dcenter.p;
+  a.exps = b->dexps.p;
+  a.coefs = b->dcoefs.p;
+  a.kind = kind;
+  a.n_charges = kind == 2 ? nch : 0;
+  a.Z = dZ.p;
+  a.cxyz = dxyz.p;
+  a.rys = b->ctx->rys.p;
+  a.out = d_out;
+  a.nbf = b->nbf;
+  const int npairs = b->ns * (b->ns + 1) / 2;
+  hipLaunchKernelGGL(onee_kernel, dim3((npairs + 63) / 64), dim3(64), 0, hipStream_t(stream), a);
+  LAI_HIP(hipGetLastError());
+  LAI_HIP(hipStreamSynchronize(hipStream_t(stream)));  // charge buffers are released on return
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// probes
+// ---------------------------------------------------------------------------
+static lai_ctx* probe_ctx() {
+  static lai_ctx* c = nullptr;
+  if (!c) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (lai_dev_init(dev, &c) != LAI_OK) c = nullptr;
+  }
+  return c;
+}
+
+extern "C" int lai_dev_rys(int nroots, int n, const double* d_T, double* d_r, double* d_w, void* stream) {
+  if (nroots < 1 || nroots > LAI_RYS_NMAX || n < 0 || (n > 0 && (!d_T || !d_r || !d_w)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_rys: invalid arguments");
+  if (n == 0) return LAI_OK;
+  lai_ctx* c = probe_ctx();
+  if (!c) return fail(LAI_ERR_BACKEND, "lai_dev_rys: no device context");
+  hipLaunchKernelGGL(rys_probe_kernel, dim3((n + 127) / 128), dim3(128), 0, hipStream_t(stream), nroots, n, d_T,
+                     c->rys.p, d_r, d_w);
+  LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_boys(int nmax, int n, const double* d_T, double* d_out, void* stream) {
+  if (nmax < 0 || nmax > 40 || n < 0 || (n > 0 && (!d_T || !d_out)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_boys: invalid arguments");
+  if (n == 0) return LAI_OK;
+  hipLaunchKernelGGL(boys_probe_kernel, dim3((n + 127) / 128), dim3(128), 0, hipStream_t(stream), nmax, n, d_T,
+                     d_out);
+  LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}

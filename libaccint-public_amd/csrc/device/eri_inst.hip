@@ -1,0 +1,59 @@
+// eri_inst.hip -- instantiates the per-class ERI/J-K kernels.  Compiled
+// LAI_NPARTS times with -DLAI_PART=k so the 55 classes x 2 precisions build in
+// parallel; part k registers the launchers of the classes with id % LAI_NPARTS == k.
+#include "eri_kernels.hpp"
+#include "launch.hpp"
+
+#ifndef LAI_PART
+#error "compile with -DLAI_PART=<k> -DLAI_NPARTS=<n>"
+#endif
+
+namespace lai {
+namespace {
+
+template <int ID, typename R>
+void launch_class(const EriArgs& a, hipStream_t s, int max_blocks) {
+  constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
+  constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
+  using P = ClassPolicy<La, Lb, Lc, Ld>;
+  if (a.nq <= 0) return;
+  if constexpr (P::regs) {
+    int blocks = (a.nq + 127) / 128;
+    if (blocks > max_blocks) blocks = max_blocks;
+    dim3 grid(blocks, P::NCH);
+    hipLaunchKernelGGL((eri_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(128), 0, s, a);
+  } else {
+    int blocks = a.nq < 8 * max_blocks ? a.nq : 8 * max_blocks;
+    hipLaunchKernelGGL((eri_kernel_lds<La, Lb, Lc, Ld, R, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
+  }
+}
+
+template <int ID>
+void fill(ClassLauncher* f64, ClassLauncher* f32, ClassInfo* info) {
+  if constexpr (ID % LAI_NPARTS == LAI_PART) {
+    constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
+    constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
+    using P = ClassPolicy<La, Lb, Lc, Ld>;
+    f64[ID] = &launch_class<ID, double>;
+    f32[ID] = &launch_class<ID, float>;
+    info[ID].La = La;
+    info[ID].Lb = Lb;
+    info[ID].Lc = Lc;
+    info[ID].Ld = Ld;
+    info[ID].regs = P::regs ? 1 : 0;
+    info[ID].nchunk = P::regs ? P::NCH : 0;
+    info[ID].nt = P::regs ? 128 : P::NT;
+  }
+  if constexpr (ID + 1 < kNumClasses) fill<ID + 1>(f64, f32, info);
+}
+
+}  // namespace
+
+#define LAI_CAT2(a, b) a##b
+#define LAI_CAT(a, b) LAI_CAT2(a, b)
+void LAI_CAT(fill_launchers_part_, LAI_PART)(ClassLauncher* f64, ClassLauncher* f32,
+                                              ClassInfo* info) {
+  fill<0>(f64, f32, info);
+}
+
+}  // namespace lai

@@ -1,0 +1,592 @@
+// eri_kernels.hpp -- Rys-quadrature ERI kernels with fused J/K digestion for
+// gfx950.  Two execution shapes, both templated on the angular-momentum class
+// (La >= Lb, Lc >= Ld, bra pair class >= ket pair class) and on the compute
+// type R (double, or float for the mixed-precision path):
+//
+//  * Kernel A ("register" shape): one lane per contracted shell quartet.  The
+//    2D Rys integrals, the horizontal transfer and the contraction live in
+//    VGPRs; all loops over Cartesian components are unrolled at compile
+//    time.  Large classes split the bra index a into chunks (gridDim.y) so the
+//    per-lane accumulator stays small.
+//  * Kernel B ("LDS" shape): one workgroup of NT lanes per shell quartet.  For
+//    every primitive quartet, 3*NR lanes build the (root, axis) 2D tables into
+//    LDS; every lane then forms its strided subset of the Cartesian products.
+//
+// The algorithm per primitive quartet follows the reference's Rys recursion
+// (src/host/kernels/eri_kernel.cpp:104-164,247-368): roots u_r and weights
+// w_r for T = rho |PQ|^2, B10/B01/B00 and the effective displacements, the
+// vertical recursion in (a,0|c,0), horizontal transfer to b then d, and
+// (ab|cd) = sum_r Ix Iy Iz with the weight and prefactor folded into Iz.
+// Roots come from the piecewise-Chebyshev table (common.hpp), not from a
+// per-quartet moment/QL solve.
+//
+// J/K digestion (mode 0) accumulates the unsymmetrised matrices Jt, Kt from
+// which J = (Jt + Jt^T)/2 and K = (Kt + Kt^T)/2.  For a canonical quartet
+// with degeneracy flags ij, kl, bk (bra shells equal, ket shells equal, bra
+// pair equal ket pair) and symmetric D this reproduces the 8-permutation
+// scatter of FockBuilder::accumulate_symmetric
+// (src/shared/consumers/fock_builder.cpp:123-222):
+//   Jt[ab] += s_ij s_kl g D[cd]          Jt[cd] += s_ij s_kl g D[ab]   (!bk)
+//   Kt[ac] += s_bk g D[bd]               Kt[ad] += s_bk g D[bc]        (!kl)
+//   Kt[bc] += s_bk g D[ad]   (!ij)       Kt[bd] += s_bk g D[ac]        (!ij && !kl)
+// with s_x = 1 if the flag is set, else 2.
+// Mode 1 writes the contracted block to an ERI buffer in the caller's
+// original shell order (parity tests, Schwarz diagonal).
+#pragma once
+
+#include "common.hpp"
+
+namespace lai {
+
+struct EriArgs {
+  const Quartet* __restrict__ q;
+  int nq;
+  const PairRec* __restrict__ pairs;
+  const PrimPair* __restrict__ pp;
+  const double* __restrict__ rys;
+  // mode 0: digestion
+  const double* __restrict__ D;
+  double* Jt;
+  double* Kt;
+  int nbf;
+  int mode;
+  // mode 1: raw ERI output, [q][block] in the original order given by perm
+  double* out;
+  const int* __restrict__ perm;        // per quartet: packed original position of class positions
+  const long long* __restrict__ obase;  // per quartet: offset of its block in out
+};
+
+// --------------------------------------------------------------------------
+// 2D recursion into registers (Kernel A)
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld, typename R>
+__device__ __forceinline__ void build_2d(R (&I)[La + 1][Lb + 1][Lc + 1][Ld + 1], R PA, R QC, R AB,
+                                         R CD, R B10, R B01, R B00, R s) {
+  constexpr int LAB = La + Lb, LCD = Lc + Ld;
+  R G[LAB + 1][LCD + 1];
+  G[0][0] = s;
+#pragma unroll
+  for (int a = 0; a < LAB; ++a) {
+    R v = PA * G[a][0];
+    if (a > 0) v += R(a) * B10 * G[a - 1][0];
+    G[a + 1][0] = v;
+  }
+#pragma unroll
+  for (int c = 0; c < LCD; ++c) {
+#pragma unroll
+    for (int a = 0; a <= LAB; ++a) {
+      R v = QC * G[a][c];
+      if (c > 0) v += R(c) * B01 * G[a][c - 1];
+      if (a > 0) v += R(a) * B00 * G[a - 1][c];
+      G[a][c + 1] = v;
+    }
+  }
+  R H[Lb + 1][LAB + 1][LCD + 1];
+#pragma unroll
+  for (int a = 0; a <= LAB; ++a)
+#pragma unroll
+    for (int k = 0; k <= LCD; ++k) H[0][a][k] = G[a][k];
+#pragma unroll
+  for (int b = 0; b < Lb; ++b)
+#pragma unroll
+    for (int a = 0; a < LAB - b; ++a)
+#pragma unroll
+      for (int k = 0; k <= LCD; ++k) H[b + 1][a][k] = H[b][a + 1][k] + AB * H[b][a][k];
+#pragma unroll
+  for (int a = 0; a <= La; ++a)
+#pragma unroll
+    for (int b = 0; b <= Lb; ++b) {
+      R X[Ld + 1][LCD + 1];
+#pragma unroll
+      for (int k = 0; k <= LCD; ++k) X[0][k] = H[b][a][k];
+#pragma unroll
+      for (int d = 0; d < Ld; ++d)
+#pragma unroll
+        for (int c = 0; c < LCD - d; ++c) X[d + 1][c] = X[d][c + 1] + CD * X[d][c];
+#pragma unroll
+      for (int c = 0; c <= Lc; ++c)
+#pragma unroll
+        for (int d = 0; d <= Ld; ++d) I[a][b][c][d] = X[d][c];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Digestion helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int orig_offset(int perm, const int (&idx)[4], const int (&n)[4]) {
+  // perm packs, for class position p (0..3), the original position in bits
+  // [2p, 2p+1].  Original dims are the class dims permuted the same way.
+  int oi[4], on[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int o = (perm >> (2 * p)) & 3;
+    oi[o] = idx[p];
+    on[o] = n[p];
+  }
+  return ((oi[0] * on[1] + oi[1]) * on[2] + oi[2]) * on[3] + oi[3];
+}
+
+// --------------------------------------------------------------------------
+// Kernel A: one lane per shell quartet, bra index a in [A0, A1)
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1>
+__device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
+  constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NA = A1 - A0;
+  constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
+  const Quartet q = g.q[qi];
+  const PairRec bra = g.pairs[q.bra];
+  const PairRec ket = g.pairs[q.ket];
+  R acc[NA][nb][nc][nd];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b)
+#pragma unroll
+      for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) acc[a][b][c][d] = R(0);
+
+  const R ABx = R(bra.ABx), ABy = R(bra.ABy), ABz = R(bra.ABz);
+  const R CDx = R(ket.ABx), CDy = R(ket.ABy), CDz = R(ket.ABz);
+  for (int ib = 0; ib < bra.npp; ++ib) {
+    const PrimPair P = g.pp[bra.pp_off + ib];
+    const double zeta = P.zeta;
+    const double h_z = 0.5 / zeta;
+    for (int ik = 0; ik < ket.npp; ++ik) {
+      const PrimPair Q = g.pp[ket.pp_off + ik];
+      const double eta = Q.zeta;
+      const double izpe = 1.0 / (zeta + eta);
+      const double rz = eta * izpe;   // rho / zeta
+      const double re = zeta * izpe;  // rho / eta
+      const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+      const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = P.Kc * Q.Kc * sqrt(izpe);
+      double u[NR], w[NR];
+      rys_roots<NR>(T, g.rys, u, w);
+      const double h_e = 0.5 / eta;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double ur = u[r];
+        const R B00 = R(0.5 * ur * izpe);
+        const R B10 = R(h_z - h_z * rz * ur);
+        const R B01 = R(h_e - h_e * re * ur);
+        const double sz = rz * ur, se = re * ur;
+        R Ix[La + 1][Lb + 1][Lc + 1][Ld + 1], Iy[La + 1][Lb + 1][Lc + 1][Ld + 1],
+            Iz[La + 1][Lb + 1][Lc + 1][Ld + 1];
+        build_2d<La, Lb, Lc, Ld, R>(Ix, R(P.PAx - sz * PQx), R(Q.PAx + se * PQx), ABx, CDx, B10, B01,
+                                    B00, R(1));
+        build_2d<La, Lb, Lc, Ld, R>(Iy, R(P.PAy - sz * PQy), R(Q.PAy + se * PQy), ABy, CDy, B10, B01,
+                                    B00, R(1));
+        build_2d<La, Lb, Lc, Ld, R>(Iz, R(P.PAz - sz * PQz), R(Q.PAz + se * PQz), ABz, CDz, B10, B01,
+                                    B00, R(pref * w[r]));
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+              for (int d = 0; d < nd; ++d) {
+                const int ai = A0 + a;
+                acc[a][b][c][d] += Ix[cart_comp(La, ai, 0)][cart_comp(Lb, b, 0)][cart_comp(Lc, c, 0)]
+                                     [cart_comp(Ld, d, 0)] *
+                                   Iy[cart_comp(La, ai, 1)][cart_comp(Lb, b, 1)][cart_comp(Lc, c, 1)]
+                                     [cart_comp(Ld, d, 1)] *
+                                   Iz[cart_comp(La, ai, 2)][cart_comp(Lb, b, 2)][cart_comp(Lc, c, 2)]
+                                     [cart_comp(Ld, d, 2)];
+              }
+      }
+    }
+  }
+
+  // contracted integrals with per-component normalisation
+  double gv[NA][nb][nc][nd];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b)
+#pragma unroll
+      for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d)
+          gv[a][b][c][d] = double(acc[a][b][c][d]) * (norm_corr(La, A0 + a) * norm_corr(Lb, b) *
+                                                        norm_corr(Lc, c) * norm_corr(Ld, d));
+
+  const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
+  if (g.mode == 1) {
+    constexpr int na = ncart(La);
+    const int perm = g.perm[qi];
+    const int n4[4] = {na, nb, nc, nd};
+    double* out = g.out + g.obase[qi];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < nb; ++b)
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) {
+            const int idx[4] = {A0 + a, b, c, d};
+            out[orig_offset(perm, idx, n4)] = gv[a][b][c][d];
+          }
+    return;
+  }
+
+  const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
+  const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+  const double fK = bk ? 1.0 : 2.0;
+  const size_t n = size_t(g.nbf);
+  const double* __restrict__ D = g.D;
+  // Jt[ab] += fJ sum_cd g D[cd]
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[(fc + c) * n + fd + d];
+      atomic_add(&g.Jt[(fa + A0 + a) * n + fb + b], fJ * s);
+    }
+  if (!bk) {
+#pragma unroll
+    for (int c = 0; c < nc; ++c)
+#pragma unroll
+      for (int d = 0; d < nd; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < nb; ++b) s += gv[a][b][c][d] * D[(fa + A0 + a) * n + fb + b];
+        atomic_add(&g.Jt[(fc + c) * n + fd + d], fJ * s);
+      }
+  }
+  // Kt[ac] += fK sum_bd g D[bd]
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int c = 0; c < nc; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < nb; ++b)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[(fb + b) * n + fd + d];
+      atomic_add(&g.Kt[(fa + A0 + a) * n + fc + c], fK * s);
+    }
+  if (!kl) {  // Kt[ad] += fK sum_bc g D[bc]
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int d = 0; d < nd; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < nb; ++b)
+#pragma unroll
+          for (int c = 0; c < nc; ++c) s += gv[a][b][c][d] * D[(fb + b) * n + fc + c];
+        atomic_add(&g.Kt[(fa + A0 + a) * n + fd + d], fK * s);
+      }
+  }
+  if (!ij) {  // Kt[bc] += fK sum_ad g D[ad]
+#pragma unroll
+    for (int b = 0; b < nb; ++b)
+#pragma unroll
+      for (int c = 0; c < nc; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[(fa + A0 + a) * n + fd + d];
+        atomic_add(&g.Kt[(fb + b) * n + fc + c], fK * s);
+      }
+  }
+  if (!ij && !kl) {  // Kt[bd] += fK sum_ac g D[ac]
+#pragma unroll
+    for (int b = 0; b < nb; ++b)
+#pragma unroll
+      for (int d = 0; d < nd; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int c = 0; c < nc; ++c) s += gv[a][b][c][d] * D[(fa + A0 + a) * n + fc + c];
+        atomic_add(&g.Kt[(fb + b) * n + fd + d], fK * s);
+      }
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld, typename R, int CHA, int C>
+__device__ __forceinline__ void dispatch_chunk(const EriArgs& g, int qi, int chunk) {
+  constexpr int na = ncart(La);
+  constexpr int A0 = C * CHA;
+  constexpr int A1 = (A0 + CHA < na) ? A0 + CHA : na;
+  if (chunk == C) {
+    quartet_regs<La, Lb, Lc, Ld, R, A0, A1>(g, qi);
+  } else if constexpr (A1 < na) {
+    dispatch_chunk<La, Lb, Lc, Ld, R, CHA, C + 1>(g, qi, chunk);
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld, typename R, int CHA>
+__global__ __launch_bounds__(128) void eri_kernel_regs(EriArgs g) {
+  const int chunk = blockIdx.y;
+  for (int qi = blockIdx.x * blockDim.x + threadIdx.x; qi < g.nq; qi += gridDim.x * blockDim.x)
+    dispatch_chunk<La, Lb, Lc, Ld, R, CHA, 0>(g, qi, chunk);
+}
+
+// --------------------------------------------------------------------------
+// Kernel B: one workgroup of NT lanes per shell quartet, 2D tables in LDS
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld, typename R>
+struct LdsShape {
+  static constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
+  static constexpr int S = (La + 1) * (Lb + 1) * (Lc + 1) * (Ld + 1);
+  static constexpr int LCD = Lc + Ld;
+  static constexpr int HS = (La + 1) * (Lb + 1) * (LCD + 1);  // bra-transferred scratch
+  static constexpr int NOUT = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld);
+};
+
+// Builds one (root, axis) table I[a][b][c][d] into tab (stride layout
+// ((a*(Lb+1)+b)*(Lc+1)+c)*(Ld+1)+d), using hs as private LDS scratch.
+template <int La, int Lb, int Lc, int Ld, typename R>
+__device__ __forceinline__ void build_2d_lds(R* __restrict__ tab, R* __restrict__ hs, R PA, R QC,
+                                             R AB, R CD, R B10, R B01, R B00, R s) {
+  constexpr int LAB = La + Lb, LCD = Lc + Ld;
+  R cm1[LAB + 1], c0[LAB + 1];
+#pragma unroll
+  for (int k = 0; k <= LCD; ++k) {
+    R c1[LAB + 1];
+    if (k == 0) {
+      c1[0] = s;
+#pragma unroll
+      for (int a = 0; a < LAB; ++a) {
+        R v = PA * c1[a];
+        if (a > 0) v += R(a) * B10 * c1[a - 1];
+        c1[a + 1] = v;
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a <= LAB; ++a) {
+        R v = QC * c0[a];
+        if (k > 1) v += R(k - 1) * B01 * cm1[a];
+        if (a > 0) v += R(a) * B00 * c0[a - 1];
+        c1[a] = v;
+      }
+    }
+    // bra transfer on this column
+    R h[Lb + 1][LAB + 1];
+#pragma unroll
+    for (int a = 0; a <= LAB; ++a) h[0][a] = c1[a];
+#pragma unroll
+    for (int b = 0; b < Lb; ++b)
+#pragma unroll
+      for (int a = 0; a < LAB - b; ++a) h[b + 1][a] = h[b][a + 1] + AB * h[b][a];
+#pragma unroll
+    for (int a = 0; a <= La; ++a)
+#pragma unroll
+      for (int b = 0; b <= Lb; ++b) hs[(a * (Lb + 1) + b) * (LCD + 1) + k] = h[b][a];
+#pragma unroll
+    for (int a = 0; a <= LAB; ++a) {
+      cm1[a] = c0[a];
+      c0[a] = c1[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a <= La; ++a)
+#pragma unroll
+    for (int b = 0; b <= Lb; ++b) {
+      R X[Ld + 1][LCD + 1];
+#pragma unroll
+      for (int k = 0; k <= LCD; ++k) X[0][k] = hs[(a * (Lb + 1) + b) * (LCD + 1) + k];
+#pragma unroll
+      for (int d = 0; d < Ld; ++d)
+#pragma unroll
+        for (int c = 0; c < LCD - d; ++c) X[d + 1][c] = X[d][c + 1] + CD * X[d][c];
+#pragma unroll
+      for (int c = 0; c <= Lc; ++c)
+#pragma unroll
+        for (int d = 0; d <= Ld; ++d) tab[((a * (Lb + 1) + b) * (Lc + 1) + c) * (Ld + 1) + d] = X[d][c];
+    }
+}
+
+// One Rys root/weight pair (index r of N) from the table.
+template <int N>
+__device__ __forceinline__ void rys_root_one(double T, const double* __restrict__ tab, int r,
+                                             double& u, double& w) {
+  if (T < double(lai_rys_tcut[N])) {
+    const int k = int(T);
+    const double x = 2.0 * (T - double(k)) - 1.0;
+    const double x2 = 2.0 * x;
+    const double* c = tab + lai_rys_off[N] + k * (2 * N * LAI_RYS_NTERMS);
+    double v[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const double* cf = c + (f * N + r) * LAI_RYS_NTERMS;
+      double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+      for (int j = LAI_RYS_NTERMS - 1; j >= 1; --j) {
+        const double dk = fma(x2, d1, cf[j] - d2);
+        d2 = d1;
+        d1 = dk;
+      }
+      v[f] = fma(x, d1, cf[0] - d2);
+    }
+    u = v[0];
+    w = v[1];
+  } else {
+    const double it = 1.0 / T;
+    u = tab[LAI_RYS_HX2_OFF + lai_rys_hoff[N] + r] * it;
+    w = tab[LAI_RYS_HW_OFF + lai_rys_hoff[N] + r] * sqrt(it);
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld, typename R, int NT>
+__global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
+  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
+  constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NJ = (NOUT + NT - 1) / NT;
+  __shared__ R tab[NR * 3 * S];
+  __shared__ R hs[NR * 3 * Sh::HS];
+  const int tid = threadIdx.x;
+
+  // this lane's outputs o = tid + NT*j and their table offsets
+  int ofs[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int o = tid + NT * j;
+    if (o < NOUT) {
+      const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
+      int ax[3], bx[3], cx[3], dx[3];
+      // component exponents by a runtime walk (once per launch)
+      int cnt = 0;
+      for (int lx = La; lx >= 0; --lx)
+        for (int ly = La - lx; ly >= 0; --ly) {
+          if (cnt == a) { ax[0] = lx; ax[1] = ly; ax[2] = La - lx - ly; }
+          ++cnt;
+        }
+      cnt = 0;
+      for (int lx = Lb; lx >= 0; --lx)
+        for (int ly = Lb - lx; ly >= 0; --ly) {
+          if (cnt == b) { bx[0] = lx; bx[1] = ly; bx[2] = Lb - lx - ly; }
+          ++cnt;
+        }
+      cnt = 0;
+      for (int lx = Lc; lx >= 0; --lx)
+        for (int ly = Lc - lx; ly >= 0; --ly) {
+          if (cnt == c) { cx[0] = lx; cx[1] = ly; cx[2] = Lc - lx - ly; }
+          ++cnt;
+        }
+      cnt = 0;
+      for (int lx = Ld; lx >= 0; --lx)
+        for (int ly = Ld - lx; ly >= 0; --ly) {
+          if (cnt == d) { dx[0] = lx; dx[1] = ly; dx[2] = Ld - lx - ly; }
+          ++cnt;
+        }
+      int p = 0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const int off = ((ax[x] * (Lb + 1) + bx[x]) * (Lc + 1) + cx[x]) * (Ld + 1) + dx[x];
+        p |= off << (10 * x);
+      }
+      ofs[j] = p;
+    } else {
+      ofs[j] = 0;
+    }
+  }
+
+  for (int qi = blockIdx.x; qi < g.nq; qi += gridDim.x) {
+    const Quartet q = g.q[qi];
+    const PairRec bra = g.pairs[q.bra];
+    const PairRec ket = g.pairs[q.ket];
+    R acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[j] = R(0);
+    const int npp_b = bra.npp, npp_k = ket.npp;
+    for (int t = 0; t < npp_b * npp_k; ++t) {
+      const int ib = t / npp_k, ik = t - ib * npp_k;
+      __syncthreads();  // previous products finished with tab
+      if (tid < 3 * NR) {
+        const int r = tid / 3, ax = tid - 3 * (tid / 3);
+        const PrimPair P = g.pp[bra.pp_off + ib];
+        const PrimPair Q = g.pp[ket.pp_off + ik];
+        const double zeta = P.zeta, eta = Q.zeta;
+        const double izpe = 1.0 / (zeta + eta);
+        const double rz = eta * izpe, re = zeta * izpe;
+        const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+        const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
+        double ur, wr;
+        rys_root_one<NR>(T, g.rys, r, ur, wr);
+        const double h_z = 0.5 / zeta, h_e = 0.5 / eta;
+        const R B00 = R(0.5 * ur * izpe);
+        const R B10 = R(h_z - h_z * rz * ur);
+        const R B01 = R(h_e - h_e * re * ur);
+        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
+        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
+        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
+        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
+        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+        const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
+        build_2d_lds<La, Lb, Lc, Ld, R>(tab + (r * 3 + ax) * S, hs + (r * 3 + ax) * Sh::HS,
+                                        R(PA - rz * ur * PQ), R(QC + re * ur * PQ), R(AB), R(CD), B10,
+                                        B01, B00, R(sc));
+      }
+      __syncthreads();
+#pragma unroll 1
+      for (int r = 0; r < NR; ++r) {
+        const R* tx = tab + (r * 3) * S;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int p = ofs[j];
+          acc[j] += tx[p & 1023] * tx[S + ((p >> 10) & 1023)] * tx[2 * S + (p >> 20)];
+        }
+      }
+    }
+
+    const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
+    const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
+    const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+    const double fK = bk ? 1.0 : 2.0;
+    const size_t n = size_t(g.nbf);
+    const int perm = g.mode == 1 ? g.perm[qi] : 0;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int o = tid + NT * j;
+      if (o >= NOUT) continue;
+      const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
+      const double v = double(acc[j]) * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) *
+                       norm_corr(Ld, d);
+      if (g.mode == 1) {
+        const int idx[4] = {a, b, c, d};
+        const int n4[4] = {na, nb, nc, nd};
+        g.out[g.obase[qi] + orig_offset(perm, idx, n4)] = v;
+        continue;
+      }
+      const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+      const double* __restrict__ D = g.D;
+      atomic_add(&g.Jt[mu * n + nu], fJ * v * D[la * n + si]);
+      if (!bk) atomic_add(&g.Jt[la * n + si], fJ * v * D[mu * n + nu]);
+      atomic_add(&g.Kt[mu * n + la], fK * v * D[nu * n + si]);
+      if (!kl) atomic_add(&g.Kt[mu * n + si], fK * v * D[nu * n + la]);
+      if (!ij) atomic_add(&g.Kt[nu * n + la], fK * v * D[mu * n + si]);
+      if (!ij && !kl) atomic_add(&g.Kt[nu * n + si], fK * v * D[mu * n + la]);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Per-class execution policy
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld>
+struct ClassPolicy {
+  static constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  static constexpr int rest = nb * nc * nd;
+  static constexpr int CHA = rest >= 27 ? 1 : (27 / rest < na ? 27 / rest : na);
+  static constexpr int NCH = (na + CHA - 1) / CHA;
+  static constexpr bool regs = rest <= 27 && NCH <= 6;
+  static constexpr int NOUT = na * nb * nc * nd;
+  static constexpr int NT = NOUT <= 64 * 24 ? 64 : (NOUT <= 128 * 24 ? 128 : 256);
+};
+
+}  // namespace lai

@@ -1,0 +1,30 @@
+// launch.hpp -- per-class launcher table shared by the instantiation units and
+// the C-ABI implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lai {
+
+struct EriArgs;
+
+using ClassLauncher = void (*)(const EriArgs&, hipStream_t, int max_blocks);
+
+struct ClassInfo {
+  int La, Lb, Lc, Ld;
+  int regs;    // 1: register shape (Kernel A), 0: LDS shape (Kernel B)
+  int nchunk;  // Kernel A chunks along the bra index
+  int nt;      // lanes per workgroup
+};
+
+#define LAI_NPARTS_FIXED 8
+void fill_launchers_part_0(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_1(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_2(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_3(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_4(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_5(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_6(ClassLauncher*, ClassLauncher*, ClassInfo*);
+void fill_launchers_part_7(ClassLauncher*, ClassLauncher*, ClassInfo*);
+
+}  // namespace lai

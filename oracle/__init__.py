@@ -1,0 +1,289 @@
+"""CPU oracle for the Rys ERI + J/K path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker: ``tests/``, the smoke check and the bench's
+CPU-baseline leg load it to produce reference answers.  No product code may
+import it.  It wraps two shared libraries through ctypes:
+
+* ``oracle/liboracle.so`` -- the C restatement of the reference CPU path
+  (oracle/oracle.c, each function cites the reference file:line it follows);
+* ``oracle/_ref/libref.so`` -- the reference's own CPU sources compiled in
+  place plus a thin driver (oracle/ref_harness.cpp).  Optional: present only
+  where it was built.
+
+The basis parsing here restates ``src/shared/data/bse_json_parser.cpp:27-87``
+(one shell per coefficient column, all-zero columns skipped, zero
+coefficients inside a column kept) and ``src/shared/data/builtin_basis.cpp``
+(STO-3G, H/C/N/O/F).
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_REPO = os.path.dirname(_HERE)
+BASIS_DIR = os.path.join(_REPO, "libaccint-public_amd", "share", "basis_sets")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+# --------------------------------------------------------------------------
+# basis data
+# --------------------------------------------------------------------------
+
+# STO-3G (Hehre, Stewart, Pople 1969) as published by the Basis Set Exchange.
+_STO3G_1S = [0.15432897, 0.53532814, 0.44463454]
+_STO3G_2S = [-0.09996723, 0.39951283, 0.70011547]
+_STO3G_2P = [0.15591627, 0.60768372, 0.39195739]
+_STO3G = {
+    1: [(0, [3.42525091, 0.62391373, 0.16885540], _STO3G_1S)],
+    6: [(0, [71.6168370, 13.0450960, 3.5305122], _STO3G_1S),
+        (0, [2.9412494, 0.6834831, 0.2222899], _STO3G_2S),
+        (1, [2.9412494, 0.6834831, 0.2222899], _STO3G_2P)],
+    7: [(0, [99.1061690, 18.0523120, 4.8856602], _STO3G_1S),
+        (0, [3.7804559, 0.8784966, 0.2857144], _STO3G_2S),
+        (1, [3.7804559, 0.8784966, 0.2857144], _STO3G_2P)],
+    8: [(0, [130.7093200, 23.8088610, 6.4436083], _STO3G_1S),
+        (0, [5.0331513, 1.1695961, 0.3803890], _STO3G_2S),
+        (1, [5.0331513, 1.1695961, 0.3803890], _STO3G_2P)],
+    9: [(0, [166.6791300, 30.3608120, 8.2168207], _STO3G_1S),
+        (0, [6.4648032, 1.5022812, 0.4885885], _STO3G_2S),
+        (1, [6.4648032, 1.5022812, 0.4885885], _STO3G_2P)],
+}
+
+
+def _element_shells(name, z):
+    name = name.lower()
+    if name in ("sto-3g", "sto3g"):
+        return _STO3G[z]
+    with open(os.path.join(BASIS_DIR, name + ".json")) as f:
+        data = json.load(f)
+    out = []
+    for sh in data["elements"][str(z)]["electron_shells"]:
+        exps = [float(x) for x in sh["exponents"]]
+        ams = sh["angular_momentum"]
+        cols = sh["coefficients"]
+        if len(ams) == 1:
+            pairs = [(ams[0], c) for c in cols]
+        else:
+            pairs = list(zip(ams, cols))
+        for am, col in pairs:
+            c = [float(x) for x in col]
+            if all(v == 0.0 for v in c):
+                continue
+            out.append((am, exps, c))
+    return out
+
+
+class RawBasis:
+    """Un-normalised shells in reference order: atoms in order, shells of an
+    atom in file order."""
+
+    def __init__(self, name, atoms):
+        L, ctr, npr, ex, cf, atom_of = [], [], [], [], [], []
+        for ia, (z, xyz) in enumerate(atoms):
+            for am, exps, coefs in _element_shells(name, int(z)):
+                L.append(am)
+                ctr.extend(float(v) for v in xyz)
+                npr.append(len(exps))
+                ex.extend(exps)
+                cf.extend(coefs)
+                atom_of.append(ia)
+        self.L = np.array(L, dtype=np.int32)
+        self.centers = np.array(ctr, dtype=np.float64).reshape(-1, 3)
+        self.nprim = np.array(npr, dtype=np.int32)
+        self.exps = np.array(ex, dtype=np.float64)
+        self.raw_coefs = np.array(cf, dtype=np.float64)
+        self.atom = np.array(atom_of, dtype=np.int32)
+        self.n_shells = len(L)
+        self.nbf = int(sum((l + 1) * (l + 2) // 2 for l in L))
+        self.charges = np.array([float(z) for z, _ in atoms])
+        self.charge_xyz = np.array([list(map(float, xyz)) for _, xyz in atoms]).reshape(-1, 3)
+
+
+# --------------------------------------------------------------------------
+# shared-library wrappers
+# --------------------------------------------------------------------------
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    return ctypes.CDLL(path)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load(os.path.join(_HERE, "liboracle.so"))
+        if _lib is None:
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _lib.oracle_basis_create.restype = ctypes.c_void_p
+        _lib.oracle_basis_free.argtypes = [ctypes.c_void_p]
+        _lib.oracle_basis_nbf.argtypes = [ctypes.c_void_p]
+        _lib.oracle_screened_list.restype = ctypes.c_int64
+        for fn in ("oracle_eri_shell_quartet", "oracle_eri_tensor", "oracle_overlap",
+                   "oracle_kinetic", "oracle_nuclear", "oracle_schwarz", "oracle_jk_canonical",
+                   "oracle_jk_full", "oracle_basis_export", "oracle_screened_list"):
+            getattr(_lib, fn).restype = None if fn != "oracle_screened_list" else ctypes.c_int64
+        _lib.oracle_boys_array.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        _lib.oracle_rys.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+    return _lib
+
+
+def ref_lib():
+    """The compiled reference sources, or None where they were not built."""
+    global _ref
+    if _ref is None:
+        _ref = _load(os.path.join(_HERE, "_ref", "libref.so"))
+        if _ref is not None:
+            _ref.ref_basis_create.restype = ctypes.c_void_p
+            _ref.ref_basis_free.argtypes = [ctypes.c_void_p]
+            _ref.ref_basis_nbf.argtypes = [ctypes.c_void_p]
+            _ref.ref_boys.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+            _ref.ref_rys.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+    return _ref
+
+
+def boys(nmax, T, use_ref=False):
+    out = np.zeros(nmax + 1)
+    if use_ref:
+        ref_lib().ref_boys(nmax, float(T), _d(out))
+    else:
+        lib().oracle_boys_array(nmax, float(T), _d(out))
+    return out
+
+
+def rys(n, T, use_ref=False):
+    r = np.zeros(n)
+    w = np.zeros(n)
+    if use_ref:
+        ref_lib().ref_rys(n, float(T), _d(r), _d(w))
+    else:
+        lib().oracle_rys(n, float(T), _d(r), _d(w))
+    return r, w
+
+
+class Oracle:
+    """The C restatement (``use_ref=False``) or the compiled reference
+    (``use_ref=True``) over one basis."""
+
+    def __init__(self, raw: RawBasis, use_ref=False):
+        self.raw = raw
+        self.use_ref = use_ref
+        if use_ref:
+            self._l = ref_lib()
+            if self._l is None:
+                raise RuntimeError("oracle/_ref/libref.so is not built")
+            mk, self._free, self._pfx = self._l.ref_basis_create, self._l.ref_basis_free, "ref_"
+        else:
+            self._l = lib()
+            mk, self._free, self._pfx = self._l.oracle_basis_create, self._l.oracle_basis_free, "oracle_"
+        self._keep = [np.ascontiguousarray(raw.L), np.ascontiguousarray(raw.centers.ravel()),
+                      np.ascontiguousarray(raw.nprim), np.ascontiguousarray(raw.exps),
+                      np.ascontiguousarray(raw.raw_coefs)]
+        k = self._keep
+        self.h = ctypes.c_void_p(mk(raw.n_shells, _i(k[0]), _d(k[1]), _i(k[2]), _d(k[3]), _d(k[4])))
+        self.nbf = raw.nbf
+
+    def __del__(self):
+        try:
+            self._free(self.h)
+        except Exception:
+            pass
+
+    def _f(self, name):
+        return getattr(self._l, self._pfx + name)
+
+    def normalized(self):
+        c = np.zeros(len(self.raw.exps))
+        fo = np.zeros(self.raw.n_shells, dtype=np.int32)
+        self._f("basis_export")(self.h, _d(c), _i(fo))
+        return c, fo
+
+    def eri_quartet(self, i, j, k, l):
+        L = self.raw.L
+        n = [(L[s] + 1) * (L[s] + 2) // 2 for s in (i, j, k, l)]
+        out = np.zeros(n[0] * n[1] * n[2] * n[3])
+        self._f("eri_shell_quartet")(self.h, int(i), int(j), int(k), int(l), _d(out))
+        return out.reshape(n)
+
+    def eri_tensor(self):
+        n = self.nbf
+        if self.use_ref:
+            out = np.zeros((n, n, n, n))
+            fo = self.normalized()[1]
+            for i in range(self.raw.n_shells):
+                for j in range(self.raw.n_shells):
+                    for k in range(self.raw.n_shells):
+                        for l in range(self.raw.n_shells):
+                            blk = self.eri_quartet(i, j, k, l)
+                            a, b, c, d = blk.shape
+                            out[fo[i]:fo[i] + a, fo[j]:fo[j] + b, fo[k]:fo[k] + c, fo[l]:fo[l] + d] = blk
+            return out
+        out = np.zeros(n ** 4)
+        self._l.oracle_eri_tensor(self.h, _d(out))
+        return out.reshape(n, n, n, n)
+
+    def overlap(self):
+        S = np.zeros((self.nbf, self.nbf))
+        self._f("overlap")(self.h, _d(S))
+        return S
+
+    def kinetic(self):
+        T = np.zeros((self.nbf, self.nbf))
+        self._f("kinetic")(self.h, _d(T))
+        return T
+
+    def nuclear(self, Z=None, xyz=None):
+        Z = np.ascontiguousarray(self.raw.charges if Z is None else Z, dtype=np.float64)
+        xyz = np.ascontiguousarray(self.raw.charge_xyz if xyz is None else xyz, dtype=np.float64)
+        V = np.zeros((self.nbf, self.nbf))
+        self._f("nuclear")(self.h, len(Z), _d(Z), _d(xyz.ravel()), _d(V))
+        return V
+
+    def schwarz(self, n_threads=0):
+        ns = self.raw.n_shells
+        Q = np.zeros(ns * (ns + 1) // 2)
+        if self.use_ref:
+            self._l.ref_schwarz(self.h, _d(Q))
+        else:
+            self._l.oracle_schwarz(self.h, _d(Q), int(n_threads))
+        return Q
+
+    def screened_list(self, Q, thr):
+        h = ctypes.c_uint64(0)
+        n = lib().oracle_screened_list(self.h if not self.use_ref else Oracle(self.raw).h,
+                                       _d(np.ascontiguousarray(Q)), ctypes.c_double(thr),
+                                       ctypes.byref(h))
+        return int(n), int(h.value)
+
+    def jk_canonical(self, D, Q=None, thr=0.0, n_threads=0):
+        n = self.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        qp = _d(np.ascontiguousarray(Q)) if Q is not None else None
+        self._f("jk_canonical")(self.h, _d(D), qp, ctypes.c_double(thr), _d(J), _d(K), int(n_threads))
+        return J, K
+
+    def jk_full(self, D, n_threads=0, i_begin=0, i_end=0):
+        n = self.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        self._f("jk_full")(self.h, _d(D), _d(J), _d(K), int(n_threads), int(i_begin), int(i_end))
+        return J, K
