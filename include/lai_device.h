@@ -101,14 +101,20 @@ int lai_dev_set_schwarz(lai_basis* b, const double* Q_packed);
  * include/libaccint/screening/screening_options.hpp:147-149; loop
  * include/libaccint/engine/engine.hpp:1815-1819), sorted by angular-momentum
  * class.  thr <= 0 keeps every quartet (no Schwarz table needed).
- * Quartets with Q_ij*Q_kl*dmax < tau_fp32 are marked for the FP32 recursion
- * (used by LAI_JK_MIXED; tau_fp32 <= 0 marks none).  Returns the quartet
- * count, the FP32 count and an order-independent checksum of the (i,j,k,l)
- * list: sum over quartets of mix64(i,j,k,l) modulo 2^64 (mix64 in
- * lai_quartet_mix64).  Replaces the QuartetTask list of
- * compute_and_consume_screened_parallel_impl (engine.hpp:1877-1923). */
-int lai_dev_build_worklist(lai_basis* b, double thr, double tau_fp32, double dmax,
-                           int64_t* n_quartets, int64_t* n_fp32, uint64_t* checksum);
+ * shell_dmax (optional, n_shells*n_shells, row-major) holds max |D| over each
+ * shell-pair block.  density_screen != 0 additionally requires
+ * Dk * Q_ij * Q_kl >= thr with the exchange-type
+ * Dk = max(Dmax_ik, Dmax_il, Dmax_jk, Dmax_jl)
+ * (include/libaccint/screening/density_screening.hpp:80-104).
+ * tau_fp32 > 0 (with shell_dmax) marks quartets with
+ * Q_ij*Q_kl*max(Dmax_ij, Dmax_kl, Dmax_ik, Dmax_il, Dmax_jk, Dmax_jl) < tau_fp32
+ * for the FP32 recursion of LAI_JK_MIXED.  Returns the quartet count, the
+ * FP32 count and an order-independent checksum of the (i,j,k,l) list: the sum
+ * over quartets of lai_quartet_mix64(i,j,k,l) modulo 2^64.  Replaces the
+ * QuartetTask list of compute_and_consume_screened_parallel_impl
+ * (engine.hpp:1877-1923). */
+int lai_dev_build_worklist(lai_basis* b, double thr, const double* shell_dmax, int density_screen,
+                           double tau_fp32, int64_t* n_quartets, int64_t* n_fp32, uint64_t* checksum);
 uint64_t lai_quartet_mix64(int i, int j, int k, int l);
 /* Per-class quartet counts and algorithmic work of the current worklist.
  * Classes: pair class p(La,Lb) = La*(La+1)/2 + Lb for La >= Lb; quartet class
@@ -134,6 +140,19 @@ int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int fl
 int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags,
                              void* stream);
 int lai_dev_jk_graph_launch(lai_basis* b, void* stream);
+
+/* ---- nuclear gradient (new capability; the reference has no derivative
+ * integrals: every DerivOrder is static_assert(== 0), engine.hpp:169-170) ---
+ * atom_of_shell[n_shells] maps shells to atoms for the gradient. */
+int lai_dev_set_shell_atoms(lai_basis* b, const int* atom_of_shell, int n_atoms);
+/* d_grad[3*n_atoms] += dE/dR for E = 1/2 tr(DJ) - x/4 tr(DK) with fixed AO
+ * density d_D (device, symmetric) over the current worklist, x =
+ * exchange_scale.  Derivative integrals are contracted with D (x) D on the fly;
+ * the component layout per atom is (x, y, z), the centre-derivative
+ * convention of include/libaccint/core/derivative_utils.hpp:69-73.
+ * Shells up to d (LAI_ERR_UNSUPPORTED otherwise).  Asynchronous on stream. */
+int lai_dev_jk_grad(lai_basis* b, const double* d_D, double exchange_scale, double* d_grad, int n_atoms,
+                    void* stream);
 
 /* ---- one-electron integrals ---------------------------------------------------
  * kind 0 = overlap S, 1 = kinetic T, 2 = nuclear attraction V over point

@@ -1,0 +1,120 @@
+// consumers.hpp -- two-electron consumers: concepts, FockBuilder (host J/K)
+// and GpuFockBuilder (device-resident J/K/D).
+//
+//  * Concepts follow include/libaccint/consumers/consumer_concepts.hpp:32-70.
+//  * FockBuilder: J_mn += (mn|ls) D_ls, K_ml += (mn|ls) D_ns over full-N^4
+//    quartets (accumulate) or canonical quartets with the 8-permutation
+//    scatter (accumulate_symmetric); threading strategies Sequential /
+//    Atomic / ThreadLocal with the LIBACCINT_MAX_FOCK_THREADLOCAL_BYTES
+//    fallback; F = H + J - x K; set_density keeps the caller's pointer
+//    (src/shared/consumers/fock_builder.cpp:23-442).
+//  * GpuFockBuilder: owns device J, K and D; set_density copies D to the
+//    device on the builder's stream; getters copy back and synchronise
+//    (include/libaccint/consumers/fock_builder_gpu.hpp:46-159).
+#pragma once
+
+#include <libaccint/buffers.hpp>
+#include <libaccint/device.hpp>
+
+#include <concepts>
+#include <span>
+#include <vector>
+
+namespace libaccint {
+
+template <typename C>
+concept TwoElectronConsumer = requires(C& c, const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc,
+                                       Index fd, int na, int nb, int nc, int nd) {
+    c.accumulate(buf, fa, fb, fc, fd, na, nb, nc, nd);
+};
+template <typename C>
+concept EriConsumer = TwoElectronConsumer<C>;
+template <typename C>
+concept SymmetryAwareConsumer = TwoElectronConsumer<C> &&
+    requires(C& c, const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+             int nc, int nd, bool ij, bool kl, bool bk) {
+        c.accumulate_symmetric(buf, fa, fb, fc, fd, na, nb, nc, nd, ij, kl, bk);
+    };
+template <typename C>
+concept ParallelConsumer = TwoElectronConsumer<C> && requires(C& c, int n) {
+    c.prepare_parallel(n);
+    c.finalize_parallel();
+};
+
+namespace consumers {
+
+enum class FockThreadingStrategy { Sequential, Atomic, ThreadLocal };
+
+class FockBuilder {
+public:
+    explicit FockBuilder(Size nbf);
+    void set_density(const Real* D, Size nbf);
+    [[nodiscard]] const Real* density() const noexcept { return D_; }
+    void accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+                    int nc, int nd);
+    void accumulate_symmetric(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na,
+                              int nb, int nc, int nd, bool ij_same, bool kl_same, bool braket_same);
+    // Adds complete J and K contributions (row-major nbf x nbf); used when the
+    // device backend has already digested the integrals.
+    void add_jk(const Real* J, const Real* K);
+    [[nodiscard]] std::span<const Real> get_coulomb_matrix() const noexcept { return J_; }
+    [[nodiscard]] std::span<const Real> get_exchange_matrix() const noexcept { return K_; }
+    [[nodiscard]] std::vector<Real> get_fock_matrix(std::span<const Real> H_core, Real exchange_fraction = 1.0) const;
+    void reset() noexcept;
+    [[nodiscard]] Size nbf() const noexcept { return nbf_; }
+    void set_threading_strategy(FockThreadingStrategy s);
+    [[nodiscard]] FockThreadingStrategy threading_strategy() const noexcept { return strategy_; }
+    void prepare_parallel(int n_threads = 0);
+    void finalize_parallel();
+
+private:
+    void acc_plain(std::vector<Real>& J, std::vector<Real>& K, const TwoElectronBuffer<0>& buf, Index fa, Index fb,
+                   Index fc, Index fd, int na, int nb, int nc, int nd);
+    void acc_atomic(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+                    int nc, int nd);
+    Size nbf_;
+    const Real* D_{nullptr};
+    std::vector<Real> J_, K_;
+    FockThreadingStrategy strategy_{FockThreadingStrategy::Sequential};
+    bool auto_thread_local_{false}, auto_atomic_{false};
+    int n_threads_{1};
+    std::vector<std::vector<Real>> J_tl_, K_tl_;
+};
+
+class GpuFockBuilder {
+public:
+    explicit GpuFockBuilder(Size nbf, hipStream_t stream = nullptr);
+    ~GpuFockBuilder() = default;
+    GpuFockBuilder(GpuFockBuilder&&) noexcept = default;
+    GpuFockBuilder& operator=(GpuFockBuilder&&) noexcept = default;
+    GpuFockBuilder(const GpuFockBuilder&) = delete;
+    GpuFockBuilder& operator=(const GpuFockBuilder&) = delete;
+
+    void set_density(const Real* D, Size nbf);
+    // host ERI block -> device J/K (plain accumulate semantics)
+    void accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+                    int nc, int nd);
+    [[nodiscard]] std::vector<Real> get_coulomb_matrix() const;
+    [[nodiscard]] std::vector<Real> get_exchange_matrix() const;
+    [[nodiscard]] std::vector<Real> get_fock_matrix(std::span<const Real> H_core, Real exchange_fraction = 1.0) const;
+    void reset();
+    void synchronize();
+    [[nodiscard]] Size nbf() const noexcept { return nbf_; }
+    [[nodiscard]] hipStream_t stream() const noexcept { return stream_; }
+    [[nodiscard]] double* d_J() noexcept { return J_.get(); }
+    [[nodiscard]] double* d_K() noexcept { return K_.get(); }
+    [[nodiscard]] const double* d_D() const noexcept { return D_.get(); }
+    [[nodiscard]] bool density_set() const noexcept { return density_set_; }
+    void prepare_parallel(int = 0) {}
+    void finalize_parallel() {}
+
+private:
+    Size nbf_;
+    hipStream_t stream_;
+    device::DeviceArray J_, K_, D_;
+    std::vector<Real> D_host_;
+    bool density_set_{false};
+};
+
+}  // namespace consumers
+}  // namespace libaccint

@@ -1,0 +1,92 @@
+// core.hpp -- scalar types, geometry, exceptions and backend vocabulary of the
+// host layer.  Names and meanings follow the reference's
+// include/libaccint/core/types.hpp:24-33,141-170,
+// include/libaccint/utils/error_handling.hpp:12-95,
+// include/libaccint/core/backend.hpp:25-127 and
+// include/libaccint/engine/dispatch_policy.hpp:41-93.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+
+namespace libaccint {
+
+using Real = double;
+using Float = float;
+using Index = std::int64_t;
+using Size = std::size_t;
+
+inline constexpr int MAX_ANGULAR_MOMENTUM = 3;  // f: the highest class of the gfx950 kernels
+
+enum class AngularMomentum : int { S = 0, P = 1, D = 2, F = 3 };
+
+[[nodiscard]] constexpr int to_int(AngularMomentum am) noexcept { return static_cast<int>(am); }
+[[nodiscard]] constexpr int n_cartesian(int l) noexcept { return (l + 1) * (l + 2) / 2; }
+[[nodiscard]] constexpr int n_spherical(int l) noexcept { return 2 * l + 1; }
+
+struct Point3D {
+    Real x{0.0}, y{0.0}, z{0.0};
+    constexpr Point3D() = default;
+    constexpr Point3D(Real x_, Real y_, Real z_) : x(x_), y(y_), z(z_) {}
+    [[nodiscard]] constexpr Real distance_squared(const Point3D& o) const noexcept {
+        const Real dx = x - o.x, dy = y - o.y, dz = z - o.z;
+        return dx * dx + dy * dy + dz * dz;
+    }
+};
+
+// ---- exceptions (error_handling.hpp) --------------------------------------
+class Exception : public std::runtime_error {
+public:
+    explicit Exception(const std::string& m) : std::runtime_error(m) {}
+};
+class InvalidArgumentException : public Exception {
+public:
+    explicit InvalidArgumentException(const std::string& m) : Exception(m) {}
+};
+class InvalidStateException : public Exception {
+public:
+    explicit InvalidStateException(const std::string& m) : Exception(m) {}
+};
+class MemoryException : public Exception {
+public:
+    explicit MemoryException(const std::string& m) : Exception(m) {}
+};
+class NotImplementedException : public Exception {
+public:
+    explicit NotImplementedException(const std::string& m) : Exception(m) {}
+};
+class BackendError : public Exception {
+public:
+    explicit BackendError(const std::string& m) : Exception(m) {}
+};
+
+// ---- backends -------------------------------------------------------------
+// The device backend of this build is HIP on gfx950; there is no host
+// integral backend (the CPU value is kept for API compatibility and is never
+// selected for integral work).
+enum class BackendType { CPU, HIP };
+
+[[nodiscard]] constexpr std::string_view backend_name(BackendType b) noexcept {
+    return b == BackendType::CPU ? "CPU" : "HIP";
+}
+
+enum class BackendHint { Auto, ForceCPU, ForceGPU, PreferCPU, PreferGPU };
+
+struct DispatchConfig {
+    Size min_gpu_batch_size = 16;
+    Size min_gpu_primitives = 1000;
+    int high_am_threshold = 4;
+    Size min_gpu_shells = 10;
+    bool enable_auto_tuning = false;
+    Size auto_tune_min_batch = 100;
+    Size n_gpu_slots = 4;
+};
+
+// Precision vocabulary (include/libaccint/core/precision.hpp:310-352).
+enum class Precision { Float32, Float64 };
+enum class MixedPrecisionMode { FP64Only, FP32Compute, Adaptive };
+
+}  // namespace libaccint

@@ -16,6 +16,7 @@
 
 #include "aux_kernels.hpp"
 #include "eri_kernels.hpp"
+#include "grad_kernels.hpp"
 #include "launch.hpp"
 
 using namespace lai;
@@ -56,6 +57,15 @@ struct Launchers {
 const Launchers& launchers() {
   static Launchers L;
   return L;
+}
+
+struct GradLaunchers {
+  GradLauncher f[kNumGradClasses] = {};
+  GradLaunchers() { fill_grad_launchers(f); }
+};
+const GradLaunchers& grad_launchers() {
+  static GradLaunchers G;
+  return G;
 }
 
 template <typename T>
@@ -166,6 +176,10 @@ struct lai_basis {
   std::vector<int> pi, pj;    // shells of each pair
   DevBuf<PairRec> pairs;
   DevBuf<PrimPair> pp;
+  DevBuf<double2> ppab;  // exponents of the two primitives of each primitive pair
+  DevBuf<int> shell_atom;
+  int n_atoms = 0;
+  int max_l = 0;
   // device copies of shell data for the one-electron kernel
   DevBuf<int> dL, dnprim, dprim_off, dfunc_off;
   DevBuf<double> dcenter, dexps, dcoefs;
@@ -308,6 +322,7 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
     nbf += ncart(L[s]);
   }
   b->nbf = nbf;
+  for (int s = 0; s < ns; ++s) b->max_l = std::max(b->max_l, L[s]);
   b->exps.assign(exps, exps + np);
   b->coefs.assign(coefs, coefs + np);
 
@@ -319,6 +334,7 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
   b->pi.resize(npairs);
   b->pj.resize(npairs);
   std::vector<PrimPair> pph;
+  std::vector<double2> pabh;
   int p = 0;
   for (int i = 0; i < ns; ++i)
     for (int j = i; j < ns; ++j, ++p) {
@@ -353,6 +369,7 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
           q.PAz = q.Pz - A[2];
           q.Kc = ca * cb * std::exp(-mu * AB2) * sq2pi52 * iz;
           pph.push_back(q);
+          pabh.push_back(make_double2(al, be));
         }
       }
       r.npp = int(pph.size()) - r.pp_off;
@@ -364,6 +381,7 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
   hipError_t e;
   if ((e = b->pairs.assign(b->pairs_h.data(), b->pairs_h.size())) != hipSuccess ||
       (e = b->pp.assign(pph.data(), pph.size())) != hipSuccess ||
+      (e = b->ppab.assign(pabh.data(), pabh.size())) != hipSuccess ||
       (e = b->dL.assign(b->L.data(), ns)) != hipSuccess ||
       (e = b->dnprim.assign(b->nprim.data(), ns)) != hipSuccess ||
       (e = b->dprim_off.assign(b->prim_off.data(), ns)) != hipSuccess ||
@@ -386,6 +404,8 @@ extern "C" int lai_dev_basis_free(lai_basis* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->pairs.release();
   b->pp.release();
+  b->ppab.release();
+  b->shell_atom.release();
   b->dL.release();
   b->dnprim.release();
   b->dprim_off.release();
@@ -567,8 +587,8 @@ extern "C" uint64_t lai_quartet_mix64(int i, int j, int k, int l) {
   return splitmix64(key);
 }
 
-extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, double tau, double dmax, int64_t* n_out,
-                                      int64_t* n32_out, uint64_t* checksum) {
+extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* shell_dmax, int density_screen,
+                                      double tau, int64_t* n_out, int64_t* n32_out, uint64_t* checksum) {
   if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_build_worklist: null basis");
   const bool screen = thr > 0.0;
   if (screen && !b->have_Q)
@@ -580,7 +600,11 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, double tau, doub
       const int i = b->pi[p], j = b->pj[p];
       Qp[p] = b->Q[size_t(j) * (j + 1) / 2 + i];
     }
-  const bool mixed = tau > 0.0 && b->have_Q;
+  const bool mixed = tau > 0.0 && b->have_Q && shell_dmax != nullptr;
+  const bool dscreen = screen && density_screen && shell_dmax != nullptr;
+  if (screen && density_screen && !shell_dmax)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_build_worklist: density screening needs shell_dmax");
+  auto dm = [&](int x, int y) { return shell_dmax[size_t(x) * ns + y]; };
   const int nth = hw_threads();
   // per-thread per-class buckets, per bra pair order kept by pass structure
   struct Bucket {
@@ -609,6 +633,11 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, double tau, doub
     for (int r = p; r < npairs; ++r) {
       const double prod = qp * Qp[r];
       if (screen && !(prod >= thr)) continue;
+      const int k = b->pi[r], l = b->pj[r];
+      double dk = 0.0;
+      if (dscreen || mixed)
+        dk = std::max(std::max(dm(i, k), dm(i, l)), std::max(dm(j, k), dm(j, l)));
+      if (dscreen && !(dk * prod >= thr)) continue;
       const int cr = b->pair_cls[r];
       Quartet q;
       int id;
@@ -621,9 +650,8 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, double tau, doub
         q.ket = p;
         id = cr * (cr + 1) / 2 + cp;
       }
-      const bool f32 = mixed && prod * dmax < tau;
+      const bool f32 = mixed && prod * std::max(dk, std::max(dm(i, j), dm(k, l))) < tau;
       (f32 ? B.q32[id] : B.q64[id]).push_back(q);
-      const int k = b->pi[r], l = b->pj[r];
       B.sum += lai_quartet_mix64(i, j, k, l);
       B.counts[id]++;
       const double fl = double(npp[p]) * npp[r] * fp_tab[b->L[i]][b->L[j]][b->L[k]][b->L[l]];
@@ -836,6 +864,54 @@ extern "C" int lai_dev_boys(int nmax, int n, const double* d_T, double* d_out, v
   if (n == 0) return LAI_OK;
   hipLaunchKernelGGL(boys_probe_kernel, dim3((n + 127) / 128), dim3(128), 0, hipStream_t(stream), nmax, n, d_T,
                      d_out);
+  LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// nuclear gradient
+// ---------------------------------------------------------------------------
+extern "C" int lai_dev_set_shell_atoms(lai_basis* b, const int* atom, int n_atoms) {
+  if (!b || !atom || n_atoms <= 0) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_shell_atoms: invalid arguments");
+  for (int s = 0; s < b->ns; ++s)
+    if (atom[s] < 0 || atom[s] >= n_atoms)
+      return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_shell_atoms: atom index out of range");
+  hipError_t e = b->shell_atom.assign(atom, b->ns);
+  if (e != hipSuccess) return fail(LAI_ERR_MEMORY, "lai_dev_set_shell_atoms: upload");
+  b->n_atoms = n_atoms;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_jk_grad(lai_basis* b, const double* d_D, double xscale, double* d_grad, int n_atoms,
+                               void* stream) {
+  if (!b || !d_D || !d_grad) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_grad: invalid arguments");
+  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_grad: worklist not built");
+  if (!b->shell_atom.p || n_atoms < b->n_atoms)
+    return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_grad: shell atoms not set or gradient too short");
+  if (b->max_l > 2) return fail(LAI_ERR_UNSUPPORTED, "lai_dev_jk_grad: gradient kernels cover shells up to d");
+  hipStream_t st = hipStream_t(stream);
+  const auto& G = grad_launchers();
+  GradArgs a{};
+  a.pairs = b->pairs.p;
+  a.pp = b->pp.p;
+  a.ppab = b->ppab.p;
+  a.rys = b->ctx->rys.p;
+  a.D = d_D;
+  a.nbf = b->nbf;
+  a.xscale = xscale;
+  a.shell_atom = b->shell_atom.p;
+  a.grad = d_grad;
+  const int max_blocks = b->ctx->n_cu * 16;
+  for (int c = 0; c < kNumGradClasses; ++c) {
+    for (int part = 0; part < 2; ++part) {
+      const auto& lst = part == 0 ? b->cls[c].q64 : b->cls[c].q32;
+      const int64_t n = part == 0 ? b->cls[c].n64 : b->cls[c].n32;
+      if (n <= 0) continue;
+      a.q = lst.p;
+      a.nq = int(n);
+      G.f[c](a, st, max_blocks);
+    }
+  }
   LAI_HIP(hipGetLastError());
   return LAI_OK;
 }

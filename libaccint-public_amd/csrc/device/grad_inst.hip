@@ -1,0 +1,29 @@
+// grad_inst.hip -- instantiates the gradient kernels for the classes with
+// every shell at most d (quartet classes 0..20).
+#include "grad_kernels.hpp"
+#include "launch.hpp"
+
+namespace lai {
+namespace {
+
+template <int ID>
+void launch_grad(const GradArgs& a, hipStream_t s, int max_blocks) {
+  constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
+  constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
+  using P = GradPolicy<La, Lb, Lc, Ld>;
+  if (a.nq <= 0) return;
+  const int blocks = a.nq < max_blocks ? a.nq : max_blocks;
+  hipLaunchKernelGGL((grad_kernel<La, Lb, Lc, Ld, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
+}
+
+template <int ID>
+void fill(GradLauncher* f) {
+  f[ID] = &launch_grad<ID>;
+  if constexpr (ID + 1 < kNumGradClasses) fill<ID + 1>(f);
+}
+
+}  // namespace
+
+void fill_grad_launchers(GradLauncher* f) { fill<0>(f); }
+
+}  // namespace lai

@@ -7,6 +7,7 @@
 namespace lai {
 
 struct EriArgs;
+struct GradArgs;
 
 using ClassLauncher = void (*)(const EriArgs&, hipStream_t, int max_blocks);
 
@@ -26,5 +27,10 @@ void fill_launchers_part_4(ClassLauncher*, ClassLauncher*, ClassInfo*);
 void fill_launchers_part_5(ClassLauncher*, ClassLauncher*, ClassInfo*);
 void fill_launchers_part_6(ClassLauncher*, ClassLauncher*, ClassInfo*);
 void fill_launchers_part_7(ClassLauncher*, ClassLauncher*, ClassInfo*);
+
+// gradient kernels exist for the classes whose shells are at most d
+inline constexpr int kNumGradClasses = 21;
+using GradLauncher = void (*)(const GradArgs&, hipStream_t, int max_blocks);
+void fill_grad_launchers(GradLauncher*);
 
 }  // namespace lai

@@ -1,0 +1,295 @@
+// consumers.cpp -- FockBuilder (host J/K) and GpuFockBuilder (device J/K/D).
+#include <libaccint/consumers.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <limits>
+#include <string>
+#include <thread>
+
+namespace libaccint::consumers {
+
+namespace {
+constexpr std::size_t kDefaultMaxThreadLocalBytes = 512ull * 1024ull * 1024ull;
+
+std::size_t max_thread_local_bytes() {
+    const char* env = std::getenv("LIBACCINT_MAX_FOCK_THREADLOCAL_BYTES");
+    if (!env || !*env) return kDefaultMaxThreadLocalBytes;
+    char* end = nullptr;
+    const unsigned long long v = std::strtoull(env, &end, 10);
+    if (end != env && *end == '\0' && v > 0) return static_cast<std::size_t>(v);
+    return kDefaultMaxThreadLocalBytes;
+}
+
+thread_local int t_thread_id = 0;
+}  // namespace
+
+FockBuilder::FockBuilder(Size nbf) : nbf_(nbf), J_(nbf * nbf, 0.0), K_(nbf * nbf, 0.0) {}
+
+void FockBuilder::set_density(const Real* D, Size nbf) {
+    if (nbf != nbf_)
+        throw InvalidArgumentException("FockBuilder::set_density: nbf mismatch (expected " + std::to_string(nbf_) +
+                                       ", got " + std::to_string(nbf) + ")");
+    D_ = D;
+}
+
+void FockBuilder::acc_plain(std::vector<Real>& J, std::vector<Real>& K, const TwoElectronBuffer<0>& buf, Index fa,
+                            Index fb, Index fc, Index fd, int na, int nb, int nc, int nd) {
+    const Size n = nbf_;
+    for (int a = 0; a < na; ++a) {
+        const Size mu = fa + a;
+        for (int b = 0; b < nb; ++b) {
+            const Size nu = fb + b;
+            for (int c = 0; c < nc; ++c) {
+                const Size la = fc + c;
+                for (int d = 0; d < nd; ++d) {
+                    const Size si = fd + d;
+                    const Real g = buf(a, b, c, d);
+                    J[mu * n + nu] += g * D_[la * n + si];
+                    K[mu * n + la] += g * D_[nu * n + si];
+                }
+            }
+        }
+    }
+}
+
+void FockBuilder::acc_atomic(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na,
+                             int nb, int nc, int nd) {
+    const Size n = nbf_;
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < nd; ++d) {
+                    const Size mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+                    const Real g = buf(a, b, c, d);
+                    std::atomic_ref<Real>(J_[mu * n + nu]).fetch_add(g * D_[la * n + si], std::memory_order_relaxed);
+                    std::atomic_ref<Real>(K_[mu * n + la]).fetch_add(g * D_[nu * n + si], std::memory_order_relaxed);
+                }
+}
+
+void FockBuilder::accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na,
+                             int nb, int nc, int nd) {
+    if (!D_) throw InvalidStateException("FockBuilder::accumulate called without density matrix");
+    const FockThreadingStrategy eff = auto_atomic_ ? FockThreadingStrategy::Atomic
+                                      : auto_thread_local_ ? FockThreadingStrategy::ThreadLocal
+                                                           : strategy_;
+    if (eff == FockThreadingStrategy::Atomic) {
+        acc_atomic(buf, fa, fb, fc, fd, na, nb, nc, nd);
+    } else if (eff == FockThreadingStrategy::ThreadLocal && !J_tl_.empty()) {
+        const int tid = t_thread_id;
+        if (tid < 0 || static_cast<Size>(tid) >= J_tl_.size())
+            throw InvalidStateException("FockBuilder::accumulate without prepared thread-local buffers for thread " +
+                                        std::to_string(tid));
+        acc_plain(J_tl_[tid], K_tl_[tid], buf, fa, fb, fc, fd, na, nb, nc, nd);
+    } else {
+        acc_plain(J_, K_, buf, fa, fb, fc, fd, na, nb, nc, nd);
+    }
+}
+
+// 8-permutation scatter of a canonical quartet (fock_builder.cpp:123-222)
+void FockBuilder::accumulate_symmetric(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd,
+                                       int na, int nb, int nc, int nd, bool ij, bool kl, bool bk) {
+    if (!D_) throw InvalidStateException("FockBuilder::accumulate_symmetric called without density matrix");
+    const Size n = nbf_;
+    auto* J = J_.data();
+    auto* K = K_.data();
+    const Real* D = D_;
+    for (int a = 0; a < na; ++a) {
+        const Size mu = fa + a;
+        for (int b = 0; b < nb; ++b) {
+            const Size nu = fb + b;
+            for (int c = 0; c < nc; ++c) {
+                const Size la = fc + c;
+                for (int d = 0; d < nd; ++d) {
+                    const Size si = fd + d;
+                    const Real g = buf(a, b, c, d);
+                    const Real Dls = D[la * n + si], Dsl = D[si * n + la];
+                    const Real Dmn = D[mu * n + nu], Dnm = D[nu * n + mu];
+                    const Real Dns = D[nu * n + si], Dms = D[mu * n + si];
+                    const Real Dnl = D[nu * n + la], Dml = D[mu * n + la];
+                    J[mu * n + nu] += g * Dls;
+                    K[mu * n + la] += g * Dns;
+                    if (!kl) {
+                        J[mu * n + nu] += g * Dsl;
+                        K[mu * n + si] += g * Dnl;
+                    }
+                    if (!ij) {
+                        J[nu * n + mu] += g * Dls;
+                        K[nu * n + la] += g * Dms;
+                    }
+                    if (!ij && !kl) {
+                        J[nu * n + mu] += g * Dsl;
+                        K[nu * n + si] += g * Dml;
+                    }
+                    if (!bk) {
+                        J[la * n + si] += g * Dmn;
+                        K[la * n + mu] += g * Dns;
+                        if (!ij) {
+                            J[la * n + si] += g * Dnm;
+                            K[la * n + nu] += g * Dms;
+                        }
+                        if (!kl) {
+                            J[si * n + la] += g * Dmn;
+                            K[si * n + mu] += g * Dnl;
+                        }
+                        if (!ij && !kl) {
+                            J[si * n + la] += g * Dnm;
+                            K[si * n + nu] += g * Dml;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+void FockBuilder::add_jk(const Real* J, const Real* K) {
+    for (Size i = 0; i < nbf_ * nbf_; ++i) {
+        J_[i] += J[i];
+        K_[i] += K[i];
+    }
+}
+
+std::vector<Real> FockBuilder::get_fock_matrix(std::span<const Real> H, Real x) const {
+    if (H.size() != nbf_ * nbf_) throw InvalidArgumentException("FockBuilder::get_fock_matrix: H_core size mismatch");
+    std::vector<Real> F(nbf_ * nbf_);
+    for (Size i = 0; i < F.size(); ++i) F[i] = H[i] + J_[i] - x * K_[i];
+    return F;
+}
+
+void FockBuilder::reset() noexcept {
+    std::fill(J_.begin(), J_.end(), 0.0);
+    std::fill(K_.begin(), K_.end(), 0.0);
+    for (auto& v : J_tl_) std::fill(v.begin(), v.end(), 0.0);
+    for (auto& v : K_tl_) std::fill(v.begin(), v.end(), 0.0);
+}
+
+void FockBuilder::set_threading_strategy(FockThreadingStrategy s) {
+    strategy_ = s;
+    auto_thread_local_ = false;
+    auto_atomic_ = false;
+    if (s != FockThreadingStrategy::ThreadLocal) {
+        J_tl_.clear();
+        K_tl_.clear();
+    }
+}
+
+// prepare_parallel / finalize_parallel: thread-local buffers unless they
+// exceed LIBACCINT_MAX_FOCK_THREADLOCAL_BYTES, then atomics
+// (fock_builder.cpp:276-371)
+void FockBuilder::prepare_parallel(int n_threads) {
+    if (n_threads <= 0) n_threads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    n_threads_ = std::max(1, n_threads);
+    auto_atomic_ = false;
+    auto_thread_local_ = (strategy_ == FockThreadingStrategy::Sequential && n_threads_ > 1);
+    bool use_tl = strategy_ == FockThreadingStrategy::ThreadLocal || auto_thread_local_;
+    if (use_tl) {
+        const std::size_t mat = nbf_ * nbf_;
+        if (mat != 0 && static_cast<std::size_t>(n_threads_) > std::numeric_limits<std::size_t>::max() / mat / 2 /
+                                                                    sizeof(Real))
+            throw MemoryException("FockBuilder::prepare_parallel: overflow while sizing thread-local buffers");
+        const std::size_t bytes = static_cast<std::size_t>(n_threads_) * mat * 2 * sizeof(Real);
+        if (bytes > max_thread_local_bytes()) {
+            if (strategy_ == FockThreadingStrategy::ThreadLocal)
+                throw MemoryException("FockBuilder::prepare_parallel: requested ThreadLocal strategy needs " +
+                                      std::to_string(bytes) + " bytes, exceeding limit " +
+                                      std::to_string(max_thread_local_bytes()) +
+                                      ". Reduce threads, reduce basis size, or raise "
+                                      "LIBACCINT_MAX_FOCK_THREADLOCAL_BYTES.");
+            auto_thread_local_ = false;
+            auto_atomic_ = n_threads_ > 1;
+            use_tl = false;
+        } else {
+            J_tl_.assign(n_threads_, std::vector<Real>(mat, 0.0));
+            K_tl_.assign(n_threads_, std::vector<Real>(mat, 0.0));
+        }
+    }
+    if (!use_tl) {
+        J_tl_.clear();
+        K_tl_.clear();
+    }
+}
+
+void FockBuilder::finalize_parallel() {
+    const bool use_tl = strategy_ == FockThreadingStrategy::ThreadLocal || auto_thread_local_;
+    if (use_tl && !J_tl_.empty()) {
+        for (Size t = 0; t < J_tl_.size(); ++t)
+            for (Size i = 0; i < J_.size(); ++i) {
+                J_[i] += J_tl_[t][i];
+                K_[i] += K_tl_[t][i];
+            }
+        J_tl_.clear();
+        K_tl_.clear();
+    }
+    auto_thread_local_ = false;
+    auto_atomic_ = false;
+}
+
+// ---------------------------------------------------------------------------
+GpuFockBuilder::GpuFockBuilder(Size nbf, hipStream_t stream)
+    : nbf_(nbf), stream_(stream), J_(nbf * nbf), K_(nbf * nbf), D_(nbf * nbf) {
+    J_.zero(stream_);
+    K_.zero(stream_);
+    D_.zero(stream_);
+    device::check(lai_dev_stream_sync(stream_), "GpuFockBuilder");
+}
+
+void GpuFockBuilder::set_density(const Real* D, Size nbf) {
+    if (nbf != nbf_)
+        throw InvalidArgumentException("GpuFockBuilder::set_density: nbf mismatch (expected " +
+                                       std::to_string(nbf_) + ", got " + std::to_string(nbf) + ")");
+    D_host_.assign(D, D + nbf * nbf);
+    D_.upload(D, nbf * nbf, stream_);
+    density_set_ = true;
+}
+
+void GpuFockBuilder::accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na,
+                                int nb, int nc, int nd) {
+    if (!density_set_) throw InvalidStateException("GpuFockBuilder::accumulate called without density matrix");
+    const Size n = nbf_;
+    std::vector<Real> J(n * n), K(n * n);
+    J_.download(J.data(), n * n, stream_);
+    K_.download(K.data(), n * n, stream_);
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < nd; ++d) {
+                    const Size mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+                    const Real g = buf(a, b, c, d);
+                    J[mu * n + nu] += g * D_host_[la * n + si];
+                    K[mu * n + la] += g * D_host_[nu * n + si];
+                }
+    J_.upload(J.data(), n * n, stream_);
+    K_.upload(K.data(), n * n, stream_);
+}
+
+std::vector<Real> GpuFockBuilder::get_coulomb_matrix() const {
+    std::vector<Real> J(nbf_ * nbf_);
+    J_.download(J.data(), J.size(), stream_);
+    return J;
+}
+
+std::vector<Real> GpuFockBuilder::get_exchange_matrix() const {
+    std::vector<Real> K(nbf_ * nbf_);
+    K_.download(K.data(), K.size(), stream_);
+    return K;
+}
+
+std::vector<Real> GpuFockBuilder::get_fock_matrix(std::span<const Real> H, Real x) const {
+    if (H.size() != nbf_ * nbf_)
+        throw InvalidArgumentException("GpuFockBuilder::get_fock_matrix: H_core size mismatch");
+    auto J = get_coulomb_matrix();
+    auto K = get_exchange_matrix();
+    for (Size i = 0; i < J.size(); ++i) J[i] = H[i] + J[i] - x * K[i];
+    return J;
+}
+
+void GpuFockBuilder::reset() {
+    J_.zero(stream_);
+    K_.zero(stream_);
+}
+
+void GpuFockBuilder::synchronize() { device::check(lai_dev_stream_sync(stream_), "GpuFockBuilder::synchronize"); }
+
+}  // namespace libaccint::consumers

@@ -36,7 +36,7 @@ _SIGNATURES = {
     "lai_dev_eri_quartets": ([_vp, _i64, _ip, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
     "lai_dev_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_set_schwarz": ([_vp, _dp], ctypes.c_int),
-    "lai_dev_build_worklist": ([_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+    "lai_dev_build_worklist": ([_vp, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "lai_quartet_mix64": ([ctypes.c_int] * 4, ctypes.c_uint64),
@@ -44,6 +44,8 @@ _SIGNATURES = {
     "lai_dev_jk": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_jk_graph_capture": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_jk_graph_launch": ([_vp, _vp], ctypes.c_int),
+    "lai_dev_set_shell_atoms": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
+    "lai_dev_jk_grad": ([_vp, _vp, ctypes.c_double, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_onee": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp, _vp], ctypes.c_int),
     "lai_dev_rys": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_boys": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),

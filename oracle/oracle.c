@@ -837,30 +837,30 @@ void oracle_schwarz(const oracle_basis *b, double *Q, int n_threads)
     (void)n_threads;
 }
 
-static uint64_t fnv_mix(uint64_t h, uint32_t v)
+/* order-independent list checksum: sum over quartets of splitmix64 of the
+ * packed (i, j, k, l), modulo 2^64 */
+static uint64_t quartet_mix(uint32_t i, uint32_t j, uint32_t k, uint32_t l)
 {
-    for (int k = 0; k < 4; ++k) {
-        h ^= (v >> (8 * k)) & 0xffu;
-        h *= 1099511628211ull;
-    }
-    return h;
+    uint64_t x = ((uint64_t)(i & 0xffffu) << 48) | ((uint64_t)(j & 0xffffu) << 32) |
+                 ((uint64_t)(k & 0xffffu) << 16) | (uint64_t)(l & 0xffffu);
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
 }
 
 int64_t oracle_screened_list(const oracle_basis *b, const double *Q, double thr, uint64_t *hash_out)
 {
     const size_t n = (size_t)b->n_shells;
     int64_t count = 0;
-    uint64_t h = 1469598103934665603ull;
+    uint64_t h = 0;
     for (size_t i = 0; i < n; ++i)
         for (size_t j = i; j < n; ++j)
             for (size_t k = i; k < n; ++k)
                 for (size_t l = (k == i) ? j : k; l < n; ++l) {
-                    if (!(Q[packed(i, j)] * Q[packed(k, l)] >= thr)) continue;
+                    if (thr > 0.0 && !(Q[packed(i, j)] * Q[packed(k, l)] >= thr)) continue;
                     ++count;
-                    h = fnv_mix(h, (uint32_t)i);
-                    h = fnv_mix(h, (uint32_t)j);
-                    h = fnv_mix(h, (uint32_t)k);
-                    h = fnv_mix(h, (uint32_t)l);
+                    h += quartet_mix((uint32_t)i, (uint32_t)j, (uint32_t)k, (uint32_t)l);
                 }
     if (hash_out) *hash_out = h;
     return count;

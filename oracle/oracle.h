@@ -63,9 +63,10 @@ void oracle_nuclear(const oracle_basis *b, int n_charges, const double *Z,
 /* Q_ij = sqrt(max |(ij|ij)|), packed i<=j at j*(j+1)/2 + i... as reference:
  * index = packed_index(i,j) with i<=j (schwarz_bounds.hpp), see oracle.c */
 void oracle_schwarz(const oracle_basis *b, double *Q_packed, int n_threads);
-/* Number of canonical quartets passing Q_ij*Q_kl >= thr, and a 64-bit FNV-1a
- * hash of the ordered (i,j,k,l) list in the reference's canonical loop order
- * (include/libaccint/engine/engine.hpp:1815-1819). */
+/* Number of canonical quartets passing Q_ij*Q_kl >= thr (thr <= 0: all), in
+ * the reference's canonical loop (include/libaccint/engine/engine.hpp:1815-1819),
+ * and the list checksum: sum over quartets of splitmix64 of the packed
+ * (i, j, k, l) (16 bits each), modulo 2^64. */
 int64_t oracle_screened_list(const oracle_basis *b, const double *Q_packed, double thr,
                              uint64_t *hash_out);
 
