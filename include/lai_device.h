@@ -126,6 +126,11 @@ uint64_t lai_quartet_mix64(int i, int j, int k, int l);
 int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, double* flops_fp64,
                            double* flops_fp32, int64_t* n_eri);
 
+/* per-class FLOPs of the work the kernels execute (55 entries each): as
+ * flops_fp64/flops_fp32 of lai_dev_worklist_stats, but with K_q counting only
+ * the primitive pairs above the primitive screening bound */
+int lai_dev_worklist_class_flops(const lai_basis* b, double* flops_fp64, double* flops_fp32);
+
 /* ---- J/K build ---------------------------------------------------------------
  * d_J += J[D], d_K += K[D] over the current worklist (D symmetric, device).
  * J_mn = sum (mn|ls) D_ls, K_ml = sum (mn|ls) D_ns, i.e. the result of
@@ -134,6 +139,12 @@ int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, double* flops_fp
  * fock_accumulate_batch_kernel (src/device/cuda/consumers/fock_builder_gpu.cu:116-180,600-621)
  * with digestion fused into the ERI kernels.  Asynchronous on stream. */
 int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream);
+
+/* The same J/K build with a hipEvent pair around every class kernel; writes
+ * the per-class milliseconds of the FP64 and FP32 lists (55 entries each) and
+ * synchronises the stream.  Measurement aid; no reference counterpart. */
+int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream,
+                       double* ms_fp64, double* ms_fp32);
 
 /* Capture the same J/K build into a hipGraph bound to these pointers, and
  * replay it (the per-SCF-iteration Fock build). */

@@ -98,6 +98,9 @@ struct DevBuf {
   }
 };
 
+// primitive-quartet magnitude below which the J/K kernels skip a primitive pair
+constexpr double kPrimCut = 1e-20;
+
 inline int pair_index(int ns, int i, int j) {  // i <= j, reference loop order
   return i * ns - i * (i - 1) / 2 + (j - i);
 }
@@ -190,6 +193,7 @@ struct lai_basis {
   ClassList cls[kNumClasses];
   bool have_worklist = false;
   int64_t counts[kNumClasses] = {};
+  double cls_f64[kNumClasses] = {}, cls_f32[kNumClasses] = {};
   double flops64 = 0, flops32 = 0;
   int64_t n_eri = 0;
   // J/K scratch (unsymmetrised)
@@ -335,6 +339,8 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
   b->pj.resize(npairs);
   std::vector<PrimPair> pph;
   std::vector<double2> pabh;
+  std::vector<double> pbound;
+  double bmax = 0.0;
   int p = 0;
   for (int i = 0; i < ns; ++i)
     for (int j = i; j < ns; ++j, ++p) {
@@ -352,6 +358,9 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
       r.ABz = A[2] - B[2];
       const double AB2 = r.ABx * r.ABx + r.ABy * r.ABy + r.ABz * r.ABz;
       r.pp_off = int(pph.size());
+      std::vector<std::pair<double, int>> order;
+      std::vector<PrimPair> loc;
+      std::vector<double2> locab;
       for (int x = 0; x < nprim[sa]; ++x) {
         const double al = exps[b->prim_off[sa] + x], ca = coefs[b->prim_off[sa] + x];
         if (ca == 0.0) continue;
@@ -368,9 +377,19 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
           q.PAy = q.Py - A[1];
           q.PAz = q.Pz - A[2];
           q.Kc = ca * cb * std::exp(-mu * AB2) * sq2pi52 * iz;
-          pph.push_back(q);
-          pabh.push_back(make_double2(al, be));
+          // |prefactor| <= bound_ab * bound_cd since 1/sqrt(z+e) <= (z e)^(-1/4)
+          const double bound = std::fabs(q.Kc) / std::pow(zeta, 0.25);
+          order.emplace_back(bound, int(loc.size()));
+          loc.push_back(q);
+          locab.push_back(make_double2(al, be));
+          bmax = std::max(bmax, bound);
         }
+      }
+      std::stable_sort(order.begin(), order.end(), [](const auto& u, const auto& v) { return u.first > v.first; });
+      for (const auto& o : order) {
+        pph.push_back(loc[o.second]);
+        pabh.push_back(locab[o.second]);
+        pbound.push_back(o.first);
       }
       r.npp = int(pph.size()) - r.pp_off;
       b->pairs_h[p] = r;
@@ -378,6 +397,14 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
       b->pi[p] = i;
       b->pj[p] = j;
     }
+  // primitive screening of the J/K kernels: a primitive pair whose bound times
+  // the largest bound is below kPrimCut cannot contribute a primitive quartet
+  // above that size; such pairs are sorted last and skipped
+  for (auto& r : b->pairs_h) {
+    int k = 0;
+    while (k < r.npp && pbound[r.pp_off + k] * bmax >= kPrimCut) ++k;
+    r.npp_sig = k;
+  }
   hipError_t e;
   if ((e = b->pairs.assign(b->pairs_h.data(), b->pairs_h.size())) != hipSuccess ||
       (e = b->pp.assign(pph.data(), pph.size())) != hipSuccess ||
@@ -611,6 +638,7 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     std::vector<Quartet> q64[kNumClasses], q32[kNumClasses];
     uint64_t sum = 0;
     int64_t counts[kNumClasses] = {};
+    double cf64[kNumClasses] = {}, cf32[kNumClasses] = {};
     double f64 = 0, f32 = 0;
     int64_t neri = 0;
   };
@@ -623,8 +651,8 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     for (int y = 0; y <= kMaxL; ++y)
       for (int z = 0; z <= kMaxL; ++z)
         for (int w = 0; w <= kMaxL; ++w) fp_tab[x][y][z][w] = fprim(x, y, z, w);
-  std::vector<int> npp(npairs);
-  for (int p = 0; p < npairs; ++p) npp[p] = b->pairs_h[p].npp;
+  std::vector<int> npp(npairs), nsig(npairs);
+  for (int p = 0; p < npairs; ++p) npp[p] = b->pairs_h[p].npp, nsig[p] = b->pairs_h[p].npp_sig;
   parallel_for(npairs, [&](int p, int t) {
     Bucket& B = bk[t];
     const double qp = Qp[p];
@@ -656,12 +684,15 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
       B.counts[id]++;
       const double fl = double(npp[p]) * npp[r] * fp_tab[b->L[i]][b->L[j]][b->L[k]][b->L[l]];
       (f32 ? B.f32 : B.f64) += fl;
+      // executed work: the primitive pairs the J/K kernels actually visit
+      const double fx = double(nsig[p]) * nsig[r] * fp_tab[b->L[i]][b->L[j]][b->L[k]][b->L[l]];
+      (f32 ? B.cf32[id] : B.cf64[id]) += fx;
       B.neri += int64_t(ncart(b->L[i])) * ncart(b->L[j]) * ncart(b->L[k]) * ncart(b->L[l]);
     }
   });
   uint64_t sum = 0;
   int64_t total = 0, total32 = 0;
-  for (int c = 0; c < kNumClasses; ++c) b->counts[c] = 0;
+  for (int c = 0; c < kNumClasses; ++c) b->counts[c] = 0, b->cls_f64[c] = 0, b->cls_f32[c] = 0;
   b->flops64 = b->flops32 = 0;
   b->n_eri = 0;
   for (auto& B : bk) {
@@ -669,7 +700,11 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     b->flops64 += B.f64;
     b->flops32 += B.f32;
     b->n_eri += B.neri;
-    for (int c = 0; c < kNumClasses; ++c) b->counts[c] += B.counts[c];
+    for (int c = 0; c < kNumClasses; ++c) {
+      b->counts[c] += B.counts[c];
+      b->cls_f64[c] += B.cf64[c];
+      b->cls_f32[c] += B.cf32[c];
+    }
   }
   for (int c = 0; c < kNumClasses; ++c) {
     std::vector<Quartet> a64, a32;
@@ -686,6 +721,24 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
       std::vector<Quartet>().swap(B.q64[c]);
       std::vector<Quartet>().swap(B.q32[c]);
     }
+    // stable counting sort by (bra, ket) primitive-pair counts, largest first:
+    // lanes of a wavefront then run the same primitive loop trip counts, and
+    // the heaviest quartets start first
+    auto by_work = [&](std::vector<Quartet>& v) {
+      if (v.size() < 2) return;
+      auto key = [&](const Quartet& q) {
+        const int kb = std::min(nsig[q.bra], 255), kk = std::min(nsig[q.ket], 255);
+        return 65535 - ((kb << 8) | kk);
+      };
+      std::vector<uint32_t> cnt(65537, 0);
+      for (const auto& q : v) cnt[key(q) + 1]++;
+      for (int x = 0; x < 65536; ++x) cnt[x + 1] += cnt[x];
+      std::vector<Quartet> out(v.size());
+      for (const auto& q : v) out[cnt[key(q)]++] = q;
+      v.swap(out);
+    };
+    by_work(a64);
+    by_work(a32);
     hipError_t e;
     if ((e = b->cls[c].q64.assign(a64.data(), a64.size())) != hipSuccess ||
         (e = b->cls[c].q32.assign(a32.data(), a32.size())) != hipSuccess)
@@ -913,5 +966,68 @@ extern "C" int lai_dev_jk_grad(lai_basis* b, const double* d_D, double xscale, d
     }
   }
   LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// profiled J/K build: one hipEvent pair around each class kernel
+// ---------------------------------------------------------------------------
+extern "C" int lai_dev_worklist_class_flops(const lai_basis* b, double* f64, double* f32) {
+  if (!b || !b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_worklist_class_flops: no worklist");
+  if (f64) std::memcpy(f64, b->cls_f64, sizeof(b->cls_f64));
+  if (f32) std::memcpy(f32, b->cls_f32, sizeof(b->cls_f32));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream,
+                                  double* ms_fp64, double* ms_fp32) {
+  if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_profile: invalid arguments");
+  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_profile: worklist not built");
+  hipStream_t st = hipStream_t(stream);
+  const size_t nn = size_t(b->nbf) * b->nbf;
+  int rc;
+  if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
+  if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
+  const auto& Ls = launchers();
+  EriArgs a{};
+  a.pairs = b->pairs.p;
+  a.pp = b->pp.p;
+  a.rys = b->ctx->rys.p;
+  a.D = d_D;
+  a.Jt = b->Jt.p;
+  a.Kt = b->Kt.p;
+  a.nbf = b->nbf;
+  a.mode = 0;
+  const int max_blocks = b->ctx->n_cu * 16;
+  std::vector<hipEvent_t> ev(4 * kNumClasses, nullptr);
+  for (auto& e : ev) LAI_HIP(hipEventCreate(&e));
+  for (int c = kNumClasses - 1; c >= 0; --c) {
+    LAI_HIP(hipEventRecord(ev[4 * c], st));
+    if (b->cls[c].n64 > 0) {
+      a.q = b->cls[c].q64.p;
+      a.nq = int(b->cls[c].n64);
+      Ls.f64[c](a, st, max_blocks);
+    }
+    LAI_HIP(hipEventRecord(ev[4 * c + 1], st));
+    LAI_HIP(hipEventRecord(ev[4 * c + 2], st));
+    if (b->cls[c].n32 > 0) {
+      a.q = b->cls[c].q32.p;
+      a.nq = int(b->cls[c].n32);
+      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
+    }
+    LAI_HIP(hipEventRecord(ev[4 * c + 3], st));
+  }
+  LAI_HIP(hipGetLastError());
+  const unsigned blocks = unsigned(std::min<size_t>((nn + 255) / 256, 8192));
+  hipLaunchKernelGGL(symmetrize_add_kernel, dim3(blocks), dim3(256), 0, st, b->Jt.p, b->Kt.p, d_J, d_K, b->nbf);
+  LAI_HIP(hipStreamSynchronize(st));
+  for (int c = 0; c < kNumClasses; ++c) {
+    float t0 = 0.f, t1 = 0.f;
+    LAI_HIP(hipEventElapsedTime(&t0, ev[4 * c], ev[4 * c + 1]));
+    LAI_HIP(hipEventElapsedTime(&t1, ev[4 * c + 2], ev[4 * c + 3]));
+    if (ms_fp64) ms_fp64[c] = t0;
+    if (ms_fp32) ms_fp32[c] = t1;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
   return LAI_OK;
 }

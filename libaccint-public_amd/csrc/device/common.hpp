@@ -74,7 +74,8 @@ struct PairRec {
   int pp_off;         // first primitive pair in the PrimPair array
   int npp;            // number of primitive pairs (zero coefficients dropped)
   int same;           // 1 when sa == sb
-  int pad;
+  int npp_sig;        // leading primitive pairs used by the J/K and gradient kernels
+                      // (the rest are below the primitive screening bound)
   double ABx, ABy, ABz;  // A - B (horizontal transfer)
   double pad2;
 };

@@ -149,11 +149,15 @@ __device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
 
   const R ABx = R(bra.ABx), ABy = R(bra.ABy), ABz = R(bra.ABz);
   const R CDx = R(ket.ABx), CDy = R(ket.ABy), CDz = R(ket.ABz);
-  for (int ib = 0; ib < bra.npp; ++ib) {
+  // J/K digestion skips the negligible primitive pairs sorted to the end of
+  // each pair's list; raw ERIs (mode 1) use every primitive pair
+  const int nb_pp = g.mode == 1 ? bra.npp : bra.npp_sig;
+  const int nk_pp = g.mode == 1 ? ket.npp : ket.npp_sig;
+  for (int ib = 0; ib < nb_pp; ++ib) {
     const PrimPair P = g.pp[bra.pp_off + ib];
     const double zeta = P.zeta;
     const double h_z = 0.5 / zeta;
-    for (int ik = 0; ik < ket.npp; ++ik) {
+    for (int ik = 0; ik < nk_pp; ++ik) {
       const PrimPair Q = g.pp[ket.pp_off + ik];
       const double eta = Q.zeta;
       const double izpe = 1.0 / (zeta + eta);
@@ -503,7 +507,7 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
     R acc[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) acc[j] = R(0);
-    const int npp_b = bra.npp, npp_k = ket.npp;
+    const int npp_b = g.mode == 1 ? bra.npp : bra.npp_sig, npp_k = g.mode == 1 ? ket.npp : ket.npp_sig;
     for (int t = 0; t < npp_b * npp_k; ++t) {
       const int ib = t / npp_k, ik = t - ib * npp_k;
       __syncthreads();  // previous products finished with tab

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
       W[o] = deg * w * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
     }
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
-    const int npp_b = bra.npp, npp_k = ket.npp;
+    const int npp_b = bra.npp_sig, npp_k = ket.npp_sig;
     for (int t = 0; t < npp_b * npp_k; ++t) {
       const int ib = t / npp_k, ik = t - ib * npp_k;
       const PrimPair P = g.pp[bra.pp_off + ib];

@@ -266,6 +266,11 @@ std::vector<Real> Engine::shell_dmax(const Real* D) const {
 WorklistInfo Engine::prepare_worklist(const JKOptions& o, const Real* D_host) {
     lai_basis* b = device_basis();
     const bool need_d = (o.density_weighted && o.schwarz_threshold > 0.0) || o.mixed_precision;
+    const bool same_opts = wl_valid_ && wl_opts_.schwarz_threshold == o.schwarz_threshold &&
+                           wl_opts_.density_weighted == o.density_weighted &&
+                           wl_opts_.mixed_precision == o.mixed_precision && wl_opts_.fp32_threshold == o.fp32_threshold;
+    // no density given: a worklist built for the same options is reused as is
+    if (same_opts && (!need_d || (!D_host && density_.empty()))) return wl_info_;
     if (need_d && !D_host) {
         if (density_.empty())
             throw InvalidArgumentException("prepare_worklist: density-dependent options need a host density");
@@ -273,11 +278,7 @@ WorklistInfo Engine::prepare_worklist(const JKOptions& o, const Real* D_host) {
     }
     if (o.schwarz_threshold > 0.0 || o.mixed_precision) precompute_schwarz_bounds();
     std::vector<Real> dm = need_d ? shell_dmax(D_host) : std::vector<Real>{};
-    const bool same = wl_valid_ && wl_opts_.schwarz_threshold == o.schwarz_threshold &&
-                      wl_opts_.density_weighted == o.density_weighted &&
-                      wl_opts_.mixed_precision == o.mixed_precision && wl_opts_.fp32_threshold == o.fp32_threshold &&
-                      dm == wl_dmax_;
-    if (same) return wl_info_;
+    if (same_opts && dm == wl_dmax_) return wl_info_;
     WorklistInfo info;
     int64_t nq = 0, n32 = 0;
     uint64_t cs = 0;

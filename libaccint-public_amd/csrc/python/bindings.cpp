@@ -472,6 +472,7 @@ PYBIND11_MODULE(_core, m) {
              },
              py::arg("D"), py::arg("exchange_scale") = 1.0, py::arg("n_atoms") = -1)
         .def("worklist_info", [](const Engine& e) { return info_dict(e.worklist_info()); })
+        .def("device_basis_handle", [](Engine& e) { return reinterpret_cast<std::uintptr_t>(e.device_basis()); })
         .def("__repr__", [](const Engine& e) {
             return "Engine(n_shells=" + std::to_string(e.basis().n_shells()) +
                    ", gpu_available=" + (e.gpu_available() ? std::string("True") : std::string("False")) + ")";

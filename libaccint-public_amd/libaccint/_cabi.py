@@ -41,7 +41,9 @@ _SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "lai_quartet_mix64": ([ctypes.c_int] * 4, ctypes.c_uint64),
     "lai_dev_worklist_stats": ([_vp, ctypes.POINTER(_i64), _dp, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
+    "lai_dev_worklist_class_flops": ([_vp, _dp, _dp], ctypes.c_int),
     "lai_dev_jk": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "lai_dev_jk_profile": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _dp, _dp], ctypes.c_int),
     "lai_dev_jk_graph_capture": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_jk_graph_launch": ([_vp, _vp], ctypes.c_int),
     "lai_dev_set_shell_atoms": ([_vp, _ip, ctypes.c_int], ctypes.c_int),

@@ -280,6 +280,19 @@ class Oracle:
         self._f("jk_canonical")(self.h, _d(D), qp, ctypes.c_double(thr), _d(J), _d(K), int(n_threads))
         return J, K
 
+    def jk_canonical_sample(self, D, Q, thr, stride, offset=0, n_threads=0):
+        """(J, K, n_quartets, n_eri) over every stride-th bra pair."""
+        n = self.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        nq = ctypes.c_longlong(0)
+        ne = ctypes.c_longlong(0)
+        self._f("jk_canonical_sample")(self.h, _d(D), _d(np.ascontiguousarray(Q)), ctypes.c_double(thr),
+                                       int(stride), int(offset), _d(J), _d(K), int(n_threads),
+                                       ctypes.byref(nq), ctypes.byref(ne))
+        return J, K, nq.value, ne.value
+
     def jk_full(self, D, n_threads=0, i_begin=0, i_end=0):
         n = self.nbf
         D = np.ascontiguousarray(D, dtype=np.float64)

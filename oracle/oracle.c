@@ -1037,3 +1037,68 @@ void oracle_jk_full(const oracle_basis *b, const double *D, double *J, double *K
     }
     (void)n_threads;
 }
+
+/* canonical screened J/K over every stride-th bra pair (offset) of the
+ * reference order, counting quartets and ERIs (bounded CPU-baseline sample) */
+void oracle_jk_canonical_sample(const oracle_basis *b, const double *D, const double *Q, double thr, int stride,
+                                int offset, double *J, double *K, int n_threads, long long *n_quartets,
+                                long long *n_eri)
+{
+    const int n = b->n_shells;
+    const size_t nbf = (size_t)b->nbf;
+    memset(J, 0, sizeof(double) * nbf * nbf);
+    memset(K, 0, sizeof(double) * nbf * nbf);
+    const int npair = n * (n + 1) / 2;
+    int *pi = (int *)malloc(sizeof(int) * npair), *pj = (int *)malloc(sizeof(int) * npair);
+    int c = 0, p = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j, ++p)
+            if (p % stride == offset) {
+                pi[c] = i;
+                pj[c] = j;
+                ++c;
+            }
+    long long nq = 0, ne = 0;
+#ifdef _OPENMP
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads) reduction(+ : nq, ne)
+#endif
+    {
+        double *Jl = (double *)calloc(nbf * nbf, sizeof(double));
+        double *Kl = (double *)calloc(nbf * nbf, sizeof(double));
+        double *buf = (double *)malloc(sizeof(double) * 15 * 15 * 15 * 15);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int t = 0; t < c; ++t) {
+            const int i = pi[t], j = pj[t];
+            for (int k = i; k < n; ++k)
+                for (int l = (k == i) ? j : k; l < n; ++l) {
+                    if (!(Q[packed(i, j)] * Q[packed(k, l)] >= thr)) continue;
+                    oracle_eri_shell_quartet(b, i, j, k, l, buf);
+                    const int na = ncart(b->L[i]), nb = ncart(b->L[j]), nc = ncart(b->L[k]), nd = ncart(b->L[l]);
+                    acc_symmetric(buf, D, Jl, Kl, nbf, b->func_off[i], b->func_off[j], b->func_off[k],
+                                  b->func_off[l], na, nb, nc, nd, i == j, k == l, i == k && j == l);
+                    ++nq;
+                    ne += (long long)na * nb * nc * nd;
+                }
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            for (size_t x = 0; x < nbf * nbf; ++x) {
+                J[x] += Jl[x];
+                K[x] += Kl[x];
+            }
+        }
+        free(Jl);
+        free(Kl);
+        free(buf);
+    }
+    free(pi);
+    free(pj);
+    *n_quartets = nq;
+    *n_eri = ne;
+    (void)n_threads;
+}

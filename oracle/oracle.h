@@ -83,6 +83,12 @@ void oracle_jk_canonical(const oracle_basis *b, const double *D, const double *Q
 void oracle_jk_full(const oracle_basis *b, const double *D, double *J, double *K,
                     int n_threads, int i_begin, int i_end);
 
+/* Canonical screened J/K over every stride-th bra pair starting at offset;
+ * returns the quartet and ERI counts (a bounded sample for CPU timing). */
+void oracle_jk_canonical_sample(const oracle_basis *b, const double *D, const double *Q_packed,
+                                double thr, int stride, int offset, double *J, double *K,
+                                int n_threads, long long *n_quartets, long long *n_eri);
+
 #ifdef __cplusplus
 }
 #endif

@@ -177,6 +177,63 @@ void ref_jk_canonical(void* h, const double* D, const double* Q_packed, double t
     }
 }
 
+// The same canonical screened loop over a sample of bra pairs (every
+// stride-th pair of the reference order, starting at offset); counts the
+// quartets and ERIs computed so a bounded sample can be timed.
+void ref_jk_canonical_sample(void* h, const double* D, const double* Q_packed, double thr, int stride, int offset,
+                             double* J, double* K, int n_threads, long long* n_quartets, long long* n_eri) {
+    const auto& b = *static_cast<RefBasis*>(h)->basis;
+    const int n = static_cast<int>(b.n_shells());
+    const Size nbf = b.n_basis_functions();
+    std::memset(J, 0, sizeof(double) * nbf * nbf);
+    std::memset(K, 0, sizeof(double) * nbf * nbf);
+    auto Q = [&](int i, int j) {
+        if (i > j) std::swap(i, j);
+        return Q_packed[static_cast<Size>(j) * (j + 1) / 2 + i];
+    };
+    std::vector<std::pair<int, int>> pairs;
+    int p = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j, ++p)
+            if (p % stride == offset) pairs.emplace_back(i, j);
+    long long nq = 0, ne = 0;
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads) reduction(+ : nq, ne)
+    {
+        consumers::FockBuilder fb(nbf);
+        fb.set_density(D, nbf);
+        TwoElectronBuffer<0> buf;
+#pragma omp for schedule(dynamic, 1)
+        for (Size t = 0; t < pairs.size(); ++t) {
+            const int i = pairs[t].first, j = pairs[t].second;
+            for (int k = i; k < n; ++k)
+                for (int l = (k == i) ? j : k; l < n; ++l) {
+                    if (!(Q(i, j) * Q(k, l) >= thr)) continue;
+                    const auto& A = b.shell(i);
+                    const auto& B = b.shell(j);
+                    const auto& C = b.shell(k);
+                    const auto& E = b.shell(l);
+                    kernels::compute_eri(A, B, C, E, buf);
+                    fb.accumulate_symmetric(buf, A.function_index(), B.function_index(), C.function_index(),
+                                            E.function_index(), A.n_functions(), B.n_functions(), C.n_functions(),
+                                            E.n_functions(), i == j, k == l, i == k && j == l);
+                    ++nq;
+                    ne += static_cast<long long>(A.n_functions()) * B.n_functions() * C.n_functions() *
+                          E.n_functions();
+                }
+        }
+        auto Jl = fb.get_coulomb_matrix();
+        auto Kl = fb.get_exchange_matrix();
+#pragma omp critical
+        for (Size x = 0; x < nbf * nbf; ++x) {
+            J[x] += Jl[x];
+            K[x] += Kl[x];
+        }
+    }
+    *n_quartets = nq;
+    *n_eri = ne;
+}
+
 // Default CPU path (cpu_engine.hpp:553-622): all n_shells^4 quartets,
 // dynamic,8, the reference FockBuilder's thread-local strategy.  [i_begin,
 // i_end) restricts the first shell so a bounded sample can be timed.
