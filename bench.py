@@ -7,8 +7,9 @@ with the density resident in HBM): the seeded 16-water cluster in cc-pVDZ
 (400 Cartesian functions; O-H 1.81 bohr, HOH 104.5 deg, random orientations,
 14 bohr cube, O-O >= 5 bohr), D = C C^T with C 400x80 ~ N(0, 1/384), Schwarz
 threshold 1e-12, FP64.  The build is captured once in a hipGraph and replayed.
-Also reported: the mixed-precision build (FP32 recursion for quartets with
-Q_ij Q_kl max|D| < 1e-7) and its energy error, the nuclear-gradient time for
+Also reported: FP64 and mixed-precision builds of the linear alkane C40H82 in
+def2-SVP (FP32 recursion for quartets with Q_ij Q_kl max|D| < 1e-7) with the
+errors of J, K and E against the FP64 build, the nuclear-gradient time for
 the 32-water def2-SVP cluster, the per-class roofline of the dominant kernel
 and a CPU baseline timed on a bounded sample with the reference CPU sources
 (oracle/_ref) or, where those are not built, their C restatement.
@@ -151,21 +152,64 @@ def main():
     log("per-class (name, ms, TFLOP/s x1e3): %s" % per_class[:12])
 
     # ---------------------------------------------------------------- mixed
+    # FP64 vs mixed precision on the linear alkane C40H82 / def2-SVP
     mixed = {}
     if not args.no_mixed:
-        mopts = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
-        engine.prepare_worklist(mopts, D)
-        minfo = engine.worklist_info()
-        Jm, Km = jk_once(mopts)
-        Em = 0.5 * np.sum(D * Jm) - 0.25 * np.sum(D * Km)
-        msm, _ = timed_graph(mopts, args.steps, args.warmup)
-        mixed = {"jk_seconds_mixed": msm * 1e-3, "mixed_speedup": ms64 / msm,
-                 "mixed_fp32_quartet_fraction": minfo["n_fp32"] / max(minfo["n_quartets"], 1),
-                 "dE_mixed_Eh": float(Em - E64), "dJ_mixed_max": float(np.max(np.abs(Jm - J64))),
-                 "dK_mixed_max": float(np.max(np.abs(Km - K64))), "fp32_threshold": 1e-7}
-        log("mixed: %.3f ms/step, speedup %.2fx, dE %.2e Eh, fp32 fraction %.3f" %
-            (msm, ms64 / msm, Em - E64, mixed["mixed_fp32_quartet_fraction"]))
-        engine.prepare_worklist(opts)
+        matoms = systems.alkane(40)
+        mbasis = lai.basis_set("def2-svp", matoms)
+        mn = mbasis.n_basis_functions()
+        mD = systems.density(mn, 161, 11, 1.0 / math.sqrt(mn))
+        meng = lai.Engine(mbasis)
+        t0 = time.perf_counter()
+        o64 = lai.JKOptions(schwarz_threshold=1e-12)
+        mi64 = meng.prepare_worklist(o64)
+        log("C40H82 setup: %.2f s, %d quartets" % (time.perf_counter() - t0, mi64["n_quartets"]))
+        tmD = torch.from_numpy(mD).to(dev)
+        tmJ = torch.zeros((mn, mn), dtype=torch.float64, device=dev)
+        tmK = torch.zeros_like(tmJ)
+
+        def m_once(o):
+            tmJ.zero_()
+            tmK.zero_()
+            torch.cuda.synchronize()
+            meng.build_jk_device(tmD.data_ptr(), tmJ.data_ptr(), tmK.data_ptr(), o, stream.cuda_stream, mD)
+            torch.cuda.synchronize()
+            return tmJ.cpu().numpy().copy(), tmK.cpu().numpy().copy()
+
+        def m_timed(o, steps, warmup):
+            meng.capture_jk_graph(tmD.data_ptr(), tmJ.data_ptr(), tmK.data_ptr(), o, stream.cuda_stream)
+            for _ in range(warmup):
+                meng.launch_jk_graph(stream.cuda_stream)
+            torch.cuda.synchronize()
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(steps):
+                meng.launch_jk_graph(stream.cuda_stream)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1) / steps
+
+        msteps = max(2, args.steps // 2)
+        Jd, Kd = m_once(o64)
+        Ed = 0.5 * np.sum(mD * Jd) - 0.25 * np.sum(mD * Kd)
+        ms_d = m_timed(o64, msteps, 1)
+        om = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
+        meng.prepare_worklist(om, mD)
+        mim = meng.worklist_info()
+        Jm, Km = m_once(om)
+        Em = 0.5 * np.sum(mD * Jm) - 0.25 * np.sum(mD * Km)
+        ms_m = m_timed(om, msteps, 1)
+        mixed = {"mixed_workload": "C40H82 def2-SVP (1010 Cartesian bf), Schwarz 1e-12, %d quartets; FP32 Rys "
+                                   "recursion where Q_ij Q_kl max|D| < 1e-7" % mi64["n_quartets"],
+                 "jk_seconds_fp64_c40h82": ms_d * 1e-3, "jk_seconds_mixed": ms_m * 1e-3,
+                 "mixed_speedup": ms_d / ms_m,
+                 "mixed_fp32_quartet_fraction": mim["n_fp32"] / max(mim["n_quartets"], 1),
+                 "dE_mixed_Eh": float(Em - Ed), "dJ_mixed_max": float(np.max(np.abs(Jm - Jd))),
+                 "dK_mixed_max": float(np.max(np.abs(Km - Kd))), "fp32_threshold": 1e-7}
+        log("C40H82: FP64 %.3f ms, mixed %.3f ms, speedup %.2fx, dE %.2e Eh, fp32 fraction %.3f" %
+            (ms_d, ms_m, ms_d / ms_m, Em - Ed, mixed["mixed_fp32_quartet_fraction"]))
+        del meng, tmD, tmJ, tmK
 
     # ------------------------------------------------------------- gradient
     grad = {}

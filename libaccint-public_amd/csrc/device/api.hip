@@ -137,17 +137,18 @@ int hw_threads() {
   return int(std::min(n, 32u));
 }
 
+// f(k, thread) for k in [0, n), dynamically scheduled in blocks of grain
 template <typename F>
-void parallel_for(int n, F&& f) {
+void parallel_for(int n, F&& f, int grain = 16) {
   const int nt = std::min(hw_threads(), std::max(1, n));
   std::atomic<int> next{0};
   std::vector<std::thread> th;
   for (int t = 0; t < nt; ++t)
     th.emplace_back([&, t]() {
       for (;;) {
-        const int i = next.fetch_add(16);
+        const int i = next.fetch_add(grain);
         if (i >= n) break;
-        for (int k = i; k < std::min(n, i + 16); ++k) f(k, t);
+        for (int k = i; k < std::min(n, i + grain); ++k) f(k, t);
       }
     });
   for (auto& x : th) x.join();
@@ -164,8 +165,10 @@ struct lai_ctx {
 };
 
 struct ClassList {
-  DevBuf<Quartet> q64, q32;
+  DevBuf<Quartet> q64, q32;   // quartets grouped by bra pair (FP64 part, FP32 part)
+  DevBuf<JkChunk> c64, c32;   // runs of each list with one bra pair
   int64_t n64 = 0, n32 = 0;
+  int64_t nc64 = 0, nc32 = 0;
 };
 
 struct lai_basis {
@@ -230,12 +233,13 @@ extern "C" int lai_dev_init(int device, lai_ctx** out) {
   ctx->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
-  std::vector<double> tab(LAI_RYS_TABLE_SIZE);
+  std::vector<double> tab(LAI_DEVICE_TABLE_SIZE, 0.0);
   std::memcpy(tab.data(), lai_rys_cheb_h, sizeof(double) * LAI_RYS_CHEB_SIZE);
   for (int i = 0; i < int(sizeof(lai_rys_hx2) / sizeof(double)); ++i) {
     tab[LAI_RYS_HX2_OFF + i] = lai_rys_hx2[i];
     tab[LAI_RYS_HW_OFF + i] = lai_rys_hw[i];
   }
+  std::memcpy(tab.data() + LAI_BT_OFF, lai_boys_taylor_h, sizeof(double) * LAI_BT_ROWS * LAI_BT_M);
   hipError_t e = ctx->rys.assign(tab.data(), tab.size());
   if (e != hipSuccess) {
     delete ctx;
@@ -445,6 +449,8 @@ extern "C" int lai_dev_basis_free(lai_basis* b) {
   for (auto& c : b->cls) {
     c.q64.release();
     c.q32.release();
+    c.c64.release();
+    c.c32.release();
   }
   delete b;
   return LAI_OK;
@@ -645,7 +651,6 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
   std::vector<Bucket> bk(nth);
   // class F_prim table in the reference orientation (computed per quartet)
   const auto& Ls = launchers();
-  (void)Ls;
   double fp_tab[kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
   for (int x = 0; x <= kMaxL; ++x)
     for (int y = 0; y <= kMaxL; ++y)
@@ -706,47 +711,76 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
       b->cls_f32[c] += B.cf32[c];
     }
   }
+  // gather each class from the per-thread buckets
+  std::vector<std::vector<Quartet>> L64(kNumClasses), L32(kNumClasses);
   for (int c = 0; c < kNumClasses; ++c) {
-    std::vector<Quartet> a64, a32;
     size_t n64 = 0, n32 = 0;
     for (auto& B : bk) {
       n64 += B.q64[c].size();
       n32 += B.q32[c].size();
     }
-    a64.reserve(n64);
-    a32.reserve(n32);
+    L64[c].reserve(n64);
+    L32[c].reserve(n32);
     for (auto& B : bk) {
-      a64.insert(a64.end(), B.q64[c].begin(), B.q64[c].end());
-      a32.insert(a32.end(), B.q32[c].begin(), B.q32[c].end());
+      L64[c].insert(L64[c].end(), B.q64[c].begin(), B.q64[c].end());
+      L32[c].insert(L32[c].end(), B.q32[c].begin(), B.q32[c].end());
       std::vector<Quartet>().swap(B.q64[c]);
       std::vector<Quartet>().swap(B.q32[c]);
     }
-    // stable counting sort by (bra, ket) primitive-pair counts, largest first:
-    // lanes of a wavefront then run the same primitive loop trip counts, and
-    // the heaviest quartets start first
-    auto by_work = [&](std::vector<Quartet>& v) {
-      if (v.size() < 2) return;
-      auto key = [&](const Quartet& q) {
-        const int kb = std::min(nsig[q.bra], 255), kk = std::min(nsig[q.ket], 255);
-        return 65535 - ((kb << 8) | kk);
-      };
-      std::vector<uint32_t> cnt(65537, 0);
+  }
+  // Group every list by bra pair (bras with more significant primitive pairs
+  // first, each bra's kets likewise, so the lanes of a wavefront run equal
+  // primitive-loop trip counts), then cut each bra's run into chunks of at
+  // most chunk_max quartets: one workgroup per chunk in the J/K kernels.
+  auto anchor = [&](std::vector<Quartet>& v, std::vector<JkChunk>& ch, int chunk_max) {
+    ch.clear();
+    if (v.empty()) return;
+    std::vector<Quartet> tmp(v.size());
+    auto pass = [&](int nkeys, auto key) {  // stable counting sort
+      std::vector<uint32_t> cnt(size_t(nkeys) + 1, 0);
       for (const auto& q : v) cnt[key(q) + 1]++;
-      for (int x = 0; x < 65536; ++x) cnt[x + 1] += cnt[x];
-      std::vector<Quartet> out(v.size());
-      for (const auto& q : v) out[cnt[key(q)]++] = q;
-      v.swap(out);
+      for (int x = 0; x < nkeys; ++x) cnt[x + 1] += cnt[x];
+      for (const auto& q : v) tmp[cnt[key(q)]++] = q;
+      v.swap(tmp);
     };
-    by_work(a64);
-    by_work(a32);
+    pass(256, [&](const Quartet& q) { return 255 - std::min(nsig[q.ket], 255); });
+    pass(npairs, [&](const Quartet& q) { return q.bra; });
+    pass(256, [&](const Quartet& q) { return 255 - std::min(nsig[q.bra], 255); });
+    size_t s0 = 0;
+    while (s0 < v.size()) {
+      size_t s1 = s0 + 1;
+      while (s1 < v.size() && v[s1].bra == v[s0].bra) ++s1;
+      const int len = int(s1 - s0);
+      const int m = (len + chunk_max - 1) / chunk_max;
+      const int per = (len + m - 1) / m;
+      for (int st = 0; st < len; st += per)
+        ch.push_back(JkChunk{v[s0].bra, int(s0) + st, std::min(per, len - st), 0});
+      s0 = s1;
+    }
+  };
+  std::vector<std::vector<JkChunk>> C64(kNumClasses), C32(kNumClasses);
+  parallel_for(kNumClasses, [&](int c, int) {
+    // register-shape classes: 256 lanes per chunk, several quartets per lane;
+    // LDS-shape classes: one quartet at a time per workgroup
+    const int chunk_max = Ls.info[c].regs ? 2048 : 32;
+    anchor(L64[c], C64[c], chunk_max);
+    anchor(L32[c], C32[c], chunk_max);
+  }, 1);
+  for (int c = 0; c < kNumClasses; ++c) {
     hipError_t e;
-    if ((e = b->cls[c].q64.assign(a64.data(), a64.size())) != hipSuccess ||
-        (e = b->cls[c].q32.assign(a32.data(), a32.size())) != hipSuccess)
+    if ((e = b->cls[c].q64.assign(L64[c].data(), L64[c].size())) != hipSuccess ||
+        (e = b->cls[c].q32.assign(L32[c].data(), L32[c].size())) != hipSuccess ||
+        (e = b->cls[c].c64.assign(C64[c].data(), C64[c].size())) != hipSuccess ||
+        (e = b->cls[c].c32.assign(C32[c].data(), C32[c].size())) != hipSuccess)
       return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
-    b->cls[c].n64 = int64_t(a64.size());
-    b->cls[c].n32 = int64_t(a32.size());
+    b->cls[c].n64 = int64_t(L64[c].size());
+    b->cls[c].n32 = int64_t(L32[c].size());
+    b->cls[c].nc64 = int64_t(C64[c].size());
+    b->cls[c].nc32 = int64_t(C32[c].size());
     total += b->cls[c].n64 + b->cls[c].n32;
     total32 += b->cls[c].n32;
+    std::vector<Quartet>().swap(L64[c]);
+    std::vector<Quartet>().swap(L32[c]);
   }
   b->have_worklist = true;
   if (b->graph_exec) {
@@ -777,6 +811,36 @@ extern "C" int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, doubl
 // ---------------------------------------------------------------------------
 // J/K build
 // ---------------------------------------------------------------------------
+// Class kernels over the bra-anchored chunks, heavy classes first.  With ev
+// (4 events per class) a hipEvent pair brackets each of the two parts.
+static void launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
+  const auto& Ls = launchers();
+  const int max_blocks = b->ctx->n_cu * 16;
+  for (int c = kNumClasses - 1; c >= 0; --c) {
+    const ClassList& C = b->cls[c];
+    if (ev) (void)hipEventRecord(ev[4 * c], st);
+    if (C.nc64 > 0) {
+      a.q = C.q64.p;
+      a.nq = int(C.n64);
+      a.chunks = C.c64.p;
+      a.nchunks = int(C.nc64);
+      Ls.f64[c](a, st, max_blocks);
+    }
+    if (ev) {
+      (void)hipEventRecord(ev[4 * c + 1], st);
+      (void)hipEventRecord(ev[4 * c + 2], st);
+    }
+    if (C.nc32 > 0) {
+      a.q = C.q32.p;
+      a.nq = int(C.n32);
+      a.chunks = C.c32.p;
+      a.nchunks = int(C.nc32);
+      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
+    }
+    if (ev) (void)hipEventRecord(ev[4 * c + 3], st);
+  }
+}
+
 extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
   if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: invalid arguments");
   if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk: worklist not built");
@@ -787,7 +851,6 @@ extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* 
   int rc;
   if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
   if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
-  const auto& Ls = launchers();
   EriArgs a{};
   a.pairs = b->pairs.p;
   a.pp = b->pp.p;
@@ -797,19 +860,7 @@ extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* 
   a.Kt = b->Kt.p;
   a.nbf = b->nbf;
   a.mode = 0;
-  const int max_blocks = b->ctx->n_cu * 16;
-  for (int c = kNumClasses - 1; c >= 0; --c) {  // heavy classes first
-    if (b->cls[c].n64 > 0) {
-      a.q = b->cls[c].q64.p;
-      a.nq = int(b->cls[c].n64);
-      Ls.f64[c](a, st, max_blocks);
-    }
-    if (b->cls[c].n32 > 0) {
-      a.q = b->cls[c].q32.p;
-      a.nq = int(b->cls[c].n32);
-      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
-    }
-  }
+  launch_jk_classes(b, a, flags, st, nullptr);
   LAI_HIP(hipGetLastError());
   const unsigned blocks = unsigned(std::min<size_t>((nn + 255) / 256, 8192));
   hipLaunchKernelGGL(symmetrize_add_kernel, dim3(blocks), dim3(256), 0, st, b->Jt.p, b->Kt.p, d_J, d_K, b->nbf);
@@ -954,6 +1005,7 @@ extern "C" int lai_dev_jk_grad(lai_basis* b, const double* d_D, double xscale, d
   a.xscale = xscale;
   a.shell_atom = b->shell_atom.p;
   a.grad = d_grad;
+  a.n_atoms = b->n_atoms;
   const int max_blocks = b->ctx->n_cu * 16;
   for (int c = 0; c < kNumGradClasses; ++c) {
     for (int part = 0; part < 2; ++part) {
@@ -988,7 +1040,6 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
   int rc;
   if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
   if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
-  const auto& Ls = launchers();
   EriArgs a{};
   a.pairs = b->pairs.p;
   a.pp = b->pp.p;
@@ -998,25 +1049,9 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
   a.Kt = b->Kt.p;
   a.nbf = b->nbf;
   a.mode = 0;
-  const int max_blocks = b->ctx->n_cu * 16;
   std::vector<hipEvent_t> ev(4 * kNumClasses, nullptr);
   for (auto& e : ev) LAI_HIP(hipEventCreate(&e));
-  for (int c = kNumClasses - 1; c >= 0; --c) {
-    LAI_HIP(hipEventRecord(ev[4 * c], st));
-    if (b->cls[c].n64 > 0) {
-      a.q = b->cls[c].q64.p;
-      a.nq = int(b->cls[c].n64);
-      Ls.f64[c](a, st, max_blocks);
-    }
-    LAI_HIP(hipEventRecord(ev[4 * c + 1], st));
-    LAI_HIP(hipEventRecord(ev[4 * c + 2], st));
-    if (b->cls[c].n32 > 0) {
-      a.q = b->cls[c].q32.p;
-      a.nq = int(b->cls[c].n32);
-      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
-    }
-    LAI_HIP(hipEventRecord(ev[4 * c + 3], st));
-  }
+  launch_jk_classes(b, a, flags, st, ev.data());
   LAI_HIP(hipGetLastError());
   const unsigned blocks = unsigned(std::min<size_t>((nn + 255) / 256, 8192));
   hipLaunchKernelGGL(symmetrize_add_kernel, dim3(blocks), dim3(256), 0, st, b->Jt.p, b->Kt.p, d_J, d_K, b->nbf);

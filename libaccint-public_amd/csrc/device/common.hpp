@@ -7,6 +7,7 @@
 
 #include <cstdint>
 
+#include "boys_taylor.inc"
 #include "rys_table.inc"
 
 namespace lai {
@@ -95,34 +96,84 @@ struct Quartet {
   int bra, ket;
 };
 
+// A run of quartets of one class that share the bra pair: Quartet entries
+// [start, start + count) of the class list, all with bra == bra.
+struct JkChunk {
+  int bra, start, count, pad;
+};
+
 // ---------------------------------------------------------------------------
-// Rys roots/weights from the piecewise-Chebyshev table (device copy of
-// lai_rys_cheb_h), large-T Gauss-Hermite limit beyond T_cut(n).
+// Rys roots/weights.  The device table holds the piecewise-Chebyshev
+// interpolants (lai_rys_cheb_h), the Gauss-Hermite limits, and from
+// LAI_BT_OFF (128-byte aligned) the Boys Taylor rows (lai_boys_taylor_h).
 // ---------------------------------------------------------------------------
+#define LAI_BT_OFF (((LAI_RYS_TABLE_SIZE + 15) / 16) * 16)
+#define LAI_DEVICE_TABLE_SIZE (LAI_BT_OFF + LAI_BT_ROWS * LAI_BT_M)
+
+// Taylor step from the grid row r (F_m(T_k)) to F_m(T_k + d):
+// F_m + e1 (F_{m+1} + e2 (F_{m+2} + ... + e6 F_{m+6})) with e_j = -d / j.
+__device__ __forceinline__ double boys_step(const double* r, int m, const double (&e)[7]) {
+  double s = r[m + 6];
+#pragma unroll
+  for (int j = 6; j >= 1; --j) s = fma(e[j], s, r[m + j - 1]);
+  return s;
+}
+
 template <int N>
-__device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tab, double* u,
-                                          double* w) {
+__device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tab, double* u, double* w) {
   static_assert(N >= 1 && N <= LAI_RYS_NMAX, "Rys order out of table range");
   if (T < double(lai_rys_tcut[N])) {
-    const int k = int(T);
-    const double x = 2.0 * (T - double(k)) - 1.0;
-    const double x2 = 2.0 * x;
-    const double* c = tab + lai_rys_off[N] + k * (2 * N * LAI_RYS_NTERMS);
+    if constexpr (N <= 2) {
+      // moments F_0..F_{2N-1} from the Boys Taylor table (T < 45), then the
+      // Gauss rule of the moments in closed form
+      const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
+      const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
+      double e[7];
 #pragma unroll
-    for (int f = 0; f < 2 * N; ++f) {
-      const double* cf = c + f * LAI_RYS_NTERMS;
-      double d1 = 0.0, d2 = 0.0;
+      for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
+      const double* r = tab + LAI_BT_OFF + k * LAI_BT_M;
+      double rr[2 * N + 6];  // F_0 .. F_{2N+5}: orders up to 2N-1 plus six Taylor terms
 #pragma unroll
-      for (int j = LAI_RYS_NTERMS - 1; j >= 1; --j) {
-        const double dk = fma(x2, d1, cf[j] - d2);
-        d2 = d1;
-        d1 = dk;
+      for (int m = 0; m < 2 * N + 6; ++m) rr[m] = r[m];
+      const double F0 = boys_step(rr, 0, e), F1 = boys_step(rr, 1, e);
+      if constexpr (N == 1) {
+        u[0] = F1 / F0;
+        w[0] = F0;
+      } else {
+        const double F2 = boys_step(rr, 2, e), F3 = boys_step(rr, 3, e);
+        // x^2 + p x + q orthogonal to 1 and x under the moments
+        const double idet = 1.0 / fma(F0, F2, -F1 * F1);
+        const double p = fma(F1, F2, -F0 * F3) * idet;
+        const double q = fma(F1, F3, -F2 * F2) * idet;
+        const double x2 = 0.5 * (sqrt(fma(p, p, -4.0 * q)) - p);
+        const double x1 = q / x2;
+        const double w1 = fma(-x2, F0, F1) / (x1 - x2);
+        u[0] = x1;
+        u[1] = x2;
+        w[0] = w1;
+        w[1] = F0 - w1;
       }
-      const double v = fma(x, d1, cf[0] - d2);
-      if (f < N)
-        u[f] = v;
-      else
-        w[f - N] = v;
+    } else {
+      const int k = int(T);
+      const double x = 2.0 * (T - double(k)) - 1.0;
+      const double x2 = 2.0 * x;
+      const double* c = tab + lai_rys_off[N] + k * (2 * N * LAI_RYS_NTERMS);
+#pragma unroll
+      for (int f = 0; f < 2 * N; ++f) {
+        const double* cf = c + f * LAI_RYS_NTERMS;
+        double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int j = LAI_RYS_NTERMS - 1; j >= 1; --j) {
+          const double dk = fma(x2, d1, cf[j] - d2);
+          d2 = d1;
+          d1 = dk;
+        }
+        const double v = fma(x, d1, cf[0] - d2);
+        if (f < N)
+          u[f] = v;
+        else
+          w[f - N] = v;
+      }
     }
   } else {
     const double it = 1.0 / T;
@@ -136,5 +187,16 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
 }
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// FP64 add to LDS (ds_add_f64)
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 }  // namespace lai

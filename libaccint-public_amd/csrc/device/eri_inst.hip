@@ -11,20 +11,46 @@
 namespace lai {
 namespace {
 
+// LDS a workgroup may use (static tables plus the anchored exchange rows)
+constexpr size_t kLdsBudget = 64 * 1024;
+
 template <int ID, typename R>
-void launch_class(const EriArgs& a, hipStream_t s, int max_blocks) {
+void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
   constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
   using P = ClassPolicy<La, Lb, Lc, Ld>;
-  if (a.nq <= 0) return;
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  EriArgs a = a0;
+  if (a.mode == 1) {  // raw ERIs, one launch over the quartet list
+    if (a.nq <= 0) return;
+    if constexpr (P::regs) {
+      int blocks = (a.nq + 127) / 128;
+      if (blocks > max_blocks) blocks = max_blocks;
+      dim3 grid(blocks, P::NCH);
+      hipLaunchKernelGGL((eri_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(128), 0, s, a);
+    } else {
+      int blocks = a.nq < 8 * max_blocks ? a.nq : 8 * max_blocks;
+      hipLaunchKernelGGL((eri_kernel_lds<La, Lb, Lc, Ld, R, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
+    }
+    return;
+  }
+  // J/K digestion over chunks of quartets sharing the bra pair
+  if (a.nchunks <= 0) return;
+  const size_t nbf = size_t(a.nbf);
   if constexpr (P::regs) {
-    int blocks = (a.nq + 127) / 128;
-    if (blocks > max_blocks) blocks = max_blocks;
+    const size_t dyn = size_t(P::CHA + nb) * nbf * sizeof(double);
+    a.klds = dyn <= kLdsBudget ? 1 : 0;
+    int blocks = a.nchunks < max_blocks ? a.nchunks : max_blocks;
     dim3 grid(blocks, P::NCH);
-    hipLaunchKernelGGL((eri_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
   } else {
-    int blocks = a.nq < 8 * max_blocks ? a.nq : 8 * max_blocks;
-    hipLaunchKernelGGL((eri_kernel_lds<La, Lb, Lc, Ld, R, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
+    using Sh = LdsShape<La, Lb, Lc, Ld, R>;
+    constexpr size_t stat = size_t(Sh::NR) * 3 * (Sh::S + Sh::HS) * sizeof(R);
+    const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
+    a.klds = stat + dyn <= kLdsBudget ? 1 : 0;
+    int blocks = a.nchunks < 8 * max_blocks ? a.nchunks : 8 * max_blocks;
+    hipLaunchKernelGGL((jk_kernel_lds<La, Lb, Lc, Ld, R, P::NT>), dim3(blocks), dim3(P::NT), a.klds ? dyn : 0, s,
+                       a);
   }
 }
 

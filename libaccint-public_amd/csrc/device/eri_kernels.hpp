@@ -54,6 +54,10 @@ struct EriArgs {
   double* out;
   const int* __restrict__ perm;        // per quartet: packed original position of class positions
   const long long* __restrict__ obase;  // per quartet: offset of its block in out
+  // mode 0, anchored digestion: runs of quartets sharing the bra pair
+  const JkChunk* __restrict__ chunks;
+  int nchunks;
+  int klds;  // 1: the bra rows of Kt are accumulated in LDS (dynamic shared memory)
 };
 
 // --------------------------------------------------------------------------
@@ -129,14 +133,16 @@ __device__ __forceinline__ int orig_offset(int perm, const int (&idx)[4], const 
 // --------------------------------------------------------------------------
 // Kernel A: one lane per shell quartet, bra index a in [A0, A1)
 // --------------------------------------------------------------------------
+// Contracted, normalised block gv[a - A0][b][c][d] of the quartet (bra|ket);
+// all_prims selects every primitive pair (raw ERIs) or only those above the
+// primitive screening bound (J/K).
 template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1>
-__device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
+__device__ __forceinline__ void quartet_block_regs(
+    const EriArgs& g, const PairRec& bra, const PairRec& ket, bool all_prims,
+    double (&gv)[A1 - A0][ncart(Lb)][ncart(Lc)][ncart(Ld)]) {
   constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NA = A1 - A0;
   constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
-  const Quartet q = g.q[qi];
-  const PairRec bra = g.pairs[q.bra];
-  const PairRec ket = g.pairs[q.ket];
   R acc[NA][nb][nc][nd];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
@@ -149,10 +155,8 @@ __device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
 
   const R ABx = R(bra.ABx), ABy = R(bra.ABy), ABz = R(bra.ABz);
   const R CDx = R(ket.ABx), CDy = R(ket.ABy), CDz = R(ket.ABz);
-  // J/K digestion skips the negligible primitive pairs sorted to the end of
-  // each pair's list; raw ERIs (mode 1) use every primitive pair
-  const int nb_pp = g.mode == 1 ? bra.npp : bra.npp_sig;
-  const int nk_pp = g.mode == 1 ? ket.npp : ket.npp_sig;
+  const int nb_pp = all_prims ? bra.npp : bra.npp_sig;
+  const int nk_pp = all_prims ? ket.npp : ket.npp_sig;
   for (int ib = 0; ib < nb_pp; ++ib) {
     const PrimPair P = g.pp[bra.pp_off + ib];
     const double zeta = P.zeta;
@@ -205,7 +209,6 @@ __device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
   }
 
   // contracted integrals with per-component normalisation
-  double gv[NA][nb][nc][nd];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -216,6 +219,17 @@ __device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
         for (int d = 0; d < nd; ++d)
           gv[a][b][c][d] = double(acc[a][b][c][d]) * (norm_corr(La, A0 + a) * norm_corr(Lb, b) *
                                                         norm_corr(Lc, c) * norm_corr(Ld, d));
+}
+
+template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1>
+__device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
+  constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NA = A1 - A0;
+  const Quartet q = g.q[qi];
+  const PairRec bra = g.pairs[q.bra];
+  const PairRec ket = g.pairs[q.ket];
+  double gv[NA][nb][nc][nd];
+  quartet_block_regs<La, Lb, Lc, Ld, R, A0, A1>(g, bra, ket, g.mode == 1, gv);
 
   const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
   if (g.mode == 1) {
@@ -445,18 +459,12 @@ __device__ __forceinline__ void rys_root_one(double T, const double* __restrict_
   }
 }
 
-template <int La, int Lb, int Lc, int Ld, typename R, int NT>
-__global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
-  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-  constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
+// Table offsets of this lane's outputs o = tid + NT*j: the three per-axis
+// offsets into a (root, axis) table, 10 bits each.
+template <int La, int Lb, int Lc, int Ld, int NT, int NJ>
+__device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
-  constexpr int NJ = (NOUT + NT - 1) / NT;
-  __shared__ R tab[NR * 3 * S];
-  __shared__ R hs[NR * 3 * Sh::HS];
-  const int tid = threadIdx.x;
-
-  // this lane's outputs o = tid + NT*j and their table offsets
-  int ofs[NJ];
+  constexpr int NOUT = na * nb * nc * nd;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int o = tid + NT * j;
@@ -499,54 +507,82 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
       ofs[j] = 0;
     }
   }
+}
+
+// One shell quartet by the whole workgroup: for every primitive quartet 3*NR
+// lanes build the (root, axis) tables in LDS, then every lane accumulates its
+// outputs.  Starts and ends with a workgroup barrier.
+template <int La, int Lb, int Lc, int Ld, typename R, int NT, int NJ>
+__device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket,
+                                                bool all_prims, R* tab, R* hs, const int (&ofs)[NJ],
+                                                R (&acc)[NJ]) {
+  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
+  constexpr int NR = Sh::NR, S = Sh::S;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) acc[j] = R(0);
+  const int npp_b = all_prims ? bra.npp : bra.npp_sig, npp_k = all_prims ? ket.npp : ket.npp_sig;
+  __syncthreads();  // the previous user of tab and hs has finished
+  for (int t = 0; t < npp_b * npp_k; ++t) {
+    const int ib = t / npp_k, ik = t - ib * npp_k;
+    if (tid < 3 * NR) {
+      const int r = tid / 3, ax = tid - 3 * (tid / 3);
+      const PrimPair P = g.pp[bra.pp_off + ib];
+      const PrimPair Q = g.pp[ket.pp_off + ik];
+      const double zeta = P.zeta, eta = Q.zeta;
+      const double izpe = 1.0 / (zeta + eta);
+      const double rz = eta * izpe, re = zeta * izpe;
+      const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+      const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
+      double ur, wr;
+      rys_root_one<NR>(T, g.rys, r, ur, wr);
+      const double h_z = 0.5 / zeta, h_e = 0.5 / eta;
+      const R B00 = R(0.5 * ur * izpe);
+      const R B10 = R(h_z - h_z * rz * ur);
+      const R B01 = R(h_e - h_e * re * ur);
+      const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
+      const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
+      const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
+      const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
+      const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+      const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
+      build_2d_lds<La, Lb, Lc, Ld, R>(tab + (r * 3 + ax) * S, hs + (r * 3 + ax) * Sh::HS,
+                                      R(PA - rz * ur * PQ), R(QC + re * ur * PQ), R(AB), R(CD), B10, B01,
+                                      B00, R(sc));
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+      const R* tx = tab + (r * 3) * S;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int p = ofs[j];
+        acc[j] += tx[p & 1023] * tx[S + ((p >> 10) & 1023)] * tx[2 * S + (p >> 20)];
+      }
+    }
+    __syncthreads();  // products finished with tab before the next build
+  }
+}
+
+// Kernel B for raw ERIs (mode 1) and the direct-atomic J/K digestion.
+template <int La, int Lb, int Lc, int Ld, typename R, int NT>
+__global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
+  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
+  constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NJ = (NOUT + NT - 1) / NT;
+  __shared__ R tab[NR * 3 * S];
+  __shared__ R hs[NR * 3 * Sh::HS];
+  const int tid = threadIdx.x;
+  int ofs[NJ];
+  lds_component_offsets<La, Lb, Lc, Ld, NT, NJ>(tid, ofs);
 
   for (int qi = blockIdx.x; qi < g.nq; qi += gridDim.x) {
     const Quartet q = g.q[qi];
     const PairRec bra = g.pairs[q.bra];
     const PairRec ket = g.pairs[q.ket];
     R acc[NJ];
-#pragma unroll
-    for (int j = 0; j < NJ; ++j) acc[j] = R(0);
-    const int npp_b = g.mode == 1 ? bra.npp : bra.npp_sig, npp_k = g.mode == 1 ? ket.npp : ket.npp_sig;
-    for (int t = 0; t < npp_b * npp_k; ++t) {
-      const int ib = t / npp_k, ik = t - ib * npp_k;
-      __syncthreads();  // previous products finished with tab
-      if (tid < 3 * NR) {
-        const int r = tid / 3, ax = tid - 3 * (tid / 3);
-        const PrimPair P = g.pp[bra.pp_off + ib];
-        const PrimPair Q = g.pp[ket.pp_off + ik];
-        const double zeta = P.zeta, eta = Q.zeta;
-        const double izpe = 1.0 / (zeta + eta);
-        const double rz = eta * izpe, re = zeta * izpe;
-        const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
-        const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
-        double ur, wr;
-        rys_root_one<NR>(T, g.rys, r, ur, wr);
-        const double h_z = 0.5 / zeta, h_e = 0.5 / eta;
-        const R B00 = R(0.5 * ur * izpe);
-        const R B10 = R(h_z - h_z * rz * ur);
-        const R B01 = R(h_e - h_e * re * ur);
-        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
-        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
-        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
-        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
-        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
-        const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
-        build_2d_lds<La, Lb, Lc, Ld, R>(tab + (r * 3 + ax) * S, hs + (r * 3 + ax) * Sh::HS,
-                                        R(PA - rz * ur * PQ), R(QC + re * ur * PQ), R(AB), R(CD), B10,
-                                        B01, B00, R(sc));
-      }
-      __syncthreads();
-#pragma unroll 1
-      for (int r = 0; r < NR; ++r) {
-        const R* tx = tab + (r * 3) * S;
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          const int p = ofs[j];
-          acc[j] += tx[p & 1023] * tx[S + ((p >> 10) & 1023)] * tx[2 * S + (p >> 20)];
-        }
-      }
-    }
+    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ>(g, bra, ket, g.mode == 1, tab, hs, ofs, acc);
 
     const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
     const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
@@ -575,6 +611,272 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
       if (!kl) atomic_add(&g.Kt[mu * n + si], fK * v * D[nu * n + la]);
       if (!ij) atomic_add(&g.Kt[nu * n + la], fK * v * D[mu * n + si]);
       if (!ij && !kl) atomic_add(&g.Kt[nu * n + si], fK * v * D[mu * n + la]);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Anchored J/K digestion (mode 0).  The quartets of a class are grouped by
+// bra pair (A, B) and a workgroup takes one chunk of quartets with the same
+// bra.  Every exchange update of such a quartet lands in a row of A or B
+// (Kt[ac], Kt[ad], Kt[bc], Kt[bd]), so the workgroup accumulates those rows
+// (rows x nbf doubles) in LDS and adds them to Kt once per chunk; the bra
+// Coulomb block Jt[ab] is summed over the chunk as well.  Only the ket-side
+// Coulomb update Jt[cd] goes to HBM per quartet.  With klds == 0 (rows too
+// long for LDS) the exchange updates go straight to HBM atomics.
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1>
+__device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow) {
+  constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NA = A1 - A0;
+  constexpr int ROWS = NA + nb;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n = g.nbf;
+  const bool lds = g.klds != 0;
+  const PairRec bra = g.pairs[ch.bra];
+  const int fa = bra.fa + A0, fb = bra.fb;
+  const bool ij = bra.same;
+  const double* __restrict__ D = g.D;
+  if (lds) {
+    for (int e = tid; e < ROWS * n; e += nt) krow[e] = 0.0;
+    __syncthreads();
+  }
+  // row r of the chunk's exchange rows: bra function fa + r (r < NA) or fb + r - NA
+  auto kadd = [&](int row, int col, double v) {
+    if (lds)
+      lds_add(&krow[row * n + col], v);
+    else
+      atomic_add(&g.Kt[size_t(row < NA ? fa + row : fb + row - NA) * n + col], v);
+  };
+  double jab[NA][nb];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b) jab[a][b] = 0.0;
+
+  for (int k = tid; k < ch.count; k += nt) {
+    const Quartet q = g.q[ch.start + k];
+    const PairRec ket = g.pairs[q.ket];
+    double gv[NA][nb][nc][nd];
+    quartet_block_regs<La, Lb, Lc, Ld, R, A0, A1>(g, bra, ket, false, gv);
+    const int fc = ket.fa, fd = ket.fb;
+    const bool kl = ket.same, bk = (q.ket == ch.bra);
+    const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+    const double fK = bk ? 1.0 : 2.0;
+    // Jt[ab] += fJ sum_cd g D[cd]  (summed over the chunk)
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < nb; ++b) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[size_t(fc + c) * n + fd + d];
+        jab[a][b] += fJ * s;
+      }
+    if (!bk) {  // Jt[cd] += fJ sum_ab g D[ab]
+#pragma unroll
+      for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < nb; ++b) s += gv[a][b][c][d] * D[size_t(fa + a) * n + fb + b];
+          atomic_add(&g.Jt[size_t(fc + c) * n + fd + d], fJ * s);
+        }
+    }
+    // Kt[ac] += fK sum_bd g D[bd]
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int c = 0; c < nc; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < nb; ++b)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[size_t(fb + b) * n + fd + d];
+        kadd(a, fc + c, fK * s);
+      }
+    if (!kl) {  // Kt[ad] += fK sum_bc g D[bc]
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          double s = 0.0;
+#pragma unroll
+          for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int c = 0; c < nc; ++c) s += gv[a][b][c][d] * D[size_t(fb + b) * n + fc + c];
+          kadd(a, fd + d, fK * s);
+        }
+    }
+    if (!ij) {  // Kt[bc] += fK sum_ad g D[ad]
+#pragma unroll
+      for (int b = 0; b < nb; ++b)
+#pragma unroll
+        for (int c = 0; c < nc; ++c) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[size_t(fa + a) * n + fd + d];
+          kadd(NA + b, fc + c, fK * s);
+        }
+    }
+    if (!ij && !kl) {  // Kt[bd] += fK sum_ac g D[ac]
+#pragma unroll
+      for (int b = 0; b < nb; ++b)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int c = 0; c < nc; ++c) s += gv[a][b][c][d] * D[size_t(fa + a) * n + fc + c];
+          kadd(NA + b, fd + d, fK * s);
+        }
+    }
+  }
+  // Jt[ab]: one atomic per wavefront and element
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+      const double v = wave_sum(jab[a][b]);
+      if ((tid & 63) == 0 && v != 0.0) atomic_add(&g.Jt[size_t(fa + a) * n + fb + b], v);
+    }
+  if (lds) {
+    __syncthreads();
+    for (int e = tid; e < ROWS * n; e += nt) {
+      const double v = krow[e];
+      if (v != 0.0) {
+        const int row = e / n, col = e - row * n;
+        atomic_add(&g.Kt[size_t(row < NA ? fa + row : fb + row - NA) * n + col], v);
+      }
+    }
+    __syncthreads();  // krow is reused by the next chunk
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld, typename R, int CHA, int C>
+__device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChunk& ch, double* krow, int chunk) {
+  constexpr int na = ncart(La);
+  constexpr int A0 = C * CHA;
+  constexpr int A1 = (A0 + CHA < na) ? A0 + CHA : na;
+  if (chunk == C) {
+    jk_chunk_regs<La, Lb, Lc, Ld, R, A0, A1>(g, ch, krow);
+  } else if constexpr (A1 < na) {
+    dispatch_jk_chunk<La, Lb, Lc, Ld, R, CHA, C + 1>(g, ch, krow, chunk);
+  }
+}
+
+// Kernel A, anchored: 256 lanes per chunk, bra index chunk C = blockIdx.y.
+template <int La, int Lb, int Lc, int Ld, typename R, int CHA>
+__global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
+  extern __shared__ double krow[];
+  for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
+    const JkChunk ch = g.chunks[c];
+    dispatch_jk_chunk<La, Lb, Lc, Ld, R, CHA, 0>(g, ch, krow, blockIdx.y);
+  }
+}
+
+// Kernel B, anchored: one workgroup per chunk, the chunk's quartets one after
+// another.  Dynamic LDS: the exchange rows [na + nb][nbf], then the ket
+// Coulomb block [nc*nd] of the current quartet and the bra block [na*nb].
+template <int La, int Lb, int Lc, int Ld, typename R, int NT>
+__global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
+  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
+  constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NJ = (NOUT + NT - 1) / NT;
+  constexpr int ROWS = na + nb;
+  __shared__ R tab[NR * 3 * S];
+  __shared__ R hs[NR * 3 * Sh::HS];
+  extern __shared__ double dyn[];
+  const int tid = threadIdx.x;
+  const int n = g.nbf;
+  const bool lds = g.klds != 0;
+  double* krow = dyn;
+  double* jcd = dyn + ROWS * n;
+  double* jab = jcd + nc * nd;
+  int ofs[NJ];
+  lds_component_offsets<La, Lb, Lc, Ld, NT, NJ>(tid, ofs);
+  const double* __restrict__ D = g.D;
+
+  for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
+    const JkChunk ch = g.chunks[ci];
+    const PairRec bra = g.pairs[ch.bra];
+    const int fa = bra.fa, fb = bra.fb;
+    const bool ij = bra.same;
+    if (lds) {
+      for (int e = tid; e < ROWS * n + nc * nd + na * nb; e += NT) dyn[e] = 0.0;
+    }
+    for (int k = 0; k < ch.count; ++k) {
+      const Quartet q = g.q[ch.start + k];
+      const PairRec ket = g.pairs[q.ket];
+      R acc[NJ];
+      lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ>(g, bra, ket, false, tab, hs, ofs, acc);
+      const int fc = ket.fa, fd = ket.fb;
+      const bool kl = ket.same, bk = (q.ket == ch.bra);
+      const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+      const double fK = bk ? 1.0 : 2.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int o = tid + NT * j;
+        if (o >= NOUT) continue;
+        const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
+        const double v = double(acc[j]) * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) *
+                         norm_corr(Ld, d);
+        const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+        const double jv = fJ * v * D[la * n + si];
+        const double jw = bk ? 0.0 : fJ * v * D[mu * n + nu];
+        const double k1 = fK * v * D[nu * n + si];
+        const double k2 = kl ? 0.0 : fK * v * D[nu * n + la];
+        const double k3 = ij ? 0.0 : fK * v * D[mu * n + si];
+        const double k4 = (ij || kl) ? 0.0 : fK * v * D[mu * n + la];
+        if (lds) {
+          lds_add(&jab[a * nb + b], jv);
+          if (!bk) lds_add(&jcd[c * nd + d], jw);
+          lds_add(&krow[a * n + la], k1);
+          if (!kl) lds_add(&krow[a * n + si], k2);
+          if (!ij) lds_add(&krow[(na + b) * n + la], k3);
+          if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
+        } else {
+          atomic_add(&g.Jt[mu * n + nu], jv);
+          if (!bk) atomic_add(&g.Jt[la * n + si], jw);
+          atomic_add(&g.Kt[mu * n + la], k1);
+          if (!kl) atomic_add(&g.Kt[mu * n + si], k2);
+          if (!ij) atomic_add(&g.Kt[nu * n + la], k3);
+          if (!ij && !kl) atomic_add(&g.Kt[nu * n + si], k4);
+        }
+      }
+      if (lds && !bk) {  // this quartet's ket Coulomb block to HBM
+        __syncthreads();
+        for (int e = tid; e < nc * nd; e += NT) {
+          const double v = jcd[e];
+          jcd[e] = 0.0;
+          if (v != 0.0) atomic_add(&g.Jt[size_t(fc + e / nd) * n + fd + e % nd], v);
+        }
+        __syncthreads();
+      }
+    }
+    if (lds) {
+      __syncthreads();
+      for (int e = tid; e < ROWS * n; e += NT) {
+        const double v = krow[e];
+        if (v != 0.0) {
+          const int row = e / n, col = e - row * n;
+          atomic_add(&g.Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], v);
+        }
+      }
+      for (int e = tid; e < na * nb; e += NT) {
+        const double v = jab[e];
+        if (v != 0.0) atomic_add(&g.Jt[size_t(fa + e / nb) * n + fb + e % nb], v);
+      }
+      __syncthreads();  // dyn is cleared for the next chunk
     }
   }
 }

@@ -6,14 +6,20 @@
 namespace lai {
 namespace {
 
+// per-workgroup force accumulators in LDS up to this many atoms
+constexpr int kGradLdsAtoms = 2048;
+
 template <int ID>
-void launch_grad(const GradArgs& a, hipStream_t s, int max_blocks) {
+void launch_grad(const GradArgs& a0, hipStream_t s, int max_blocks) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
   constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
   using P = GradPolicy<La, Lb, Lc, Ld>;
-  if (a.nq <= 0) return;
+  if (a0.nq <= 0) return;
+  GradArgs a = a0;
+  a.glds = a.n_atoms <= kGradLdsAtoms ? 1 : 0;
+  const size_t dyn = a.glds ? size_t(3) * a.n_atoms * sizeof(double) : 0;
   const int blocks = a.nq < max_blocks ? a.nq : max_blocks;
-  hipLaunchKernelGGL((grad_kernel<La, Lb, Lc, Ld, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
+  hipLaunchKernelGGL((grad_kernel<La, Lb, Lc, Ld, P::NT>), dim3(blocks), dim3(P::NT), dyn, s, a);
 }
 
 template <int ID>

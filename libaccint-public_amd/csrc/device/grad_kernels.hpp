@@ -31,6 +31,8 @@ struct GradArgs {
   double xscale;
   const int* __restrict__ shell_atom;
   double* grad;
+  int n_atoms;
+  int glds;  // 1: each workgroup sums its quartets' forces in LDS ([3 * n_atoms])
 };
 
 template <int La, int Lb, int Lc, int Ld, int NT>
@@ -48,7 +50,11 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
   __shared__ double hs[NR * 3 * HS];
   __shared__ double W[NOUT];
   __shared__ double red[NT / 64][9];
+  extern __shared__ double gacc[];
   const int tid = threadIdx.x;
+  const bool glds = g.glds != 0;
+  if (glds)
+    for (int e = tid; e < 3 * g.n_atoms; e += NT) gacc[e] = 0.0;
 
   // per-lane component data: base offsets (8 bits per axis) and exponents
   int base[NJ], expo[NJ];
@@ -176,10 +182,27 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
     __syncthreads();
     if (tid < 3) {
       const double a = red[0][tid], b = red[0][3 + tid], c = red[0][6 + tid];
-      atomic_add(&g.grad[3 * g.shell_atom[bra.sa] + tid], a);
-      atomic_add(&g.grad[3 * g.shell_atom[bra.sb] + tid], b);
-      atomic_add(&g.grad[3 * g.shell_atom[ket.sa] + tid], c);
-      atomic_add(&g.grad[3 * g.shell_atom[ket.sb] + tid], -(a + b + c));
+      double* out = glds ? gacc : g.grad;
+      const int ia = 3 * g.shell_atom[bra.sa] + tid, ib = 3 * g.shell_atom[bra.sb] + tid;
+      const int ic = 3 * g.shell_atom[ket.sa] + tid, id = 3 * g.shell_atom[ket.sb] + tid;
+      if (glds) {
+        lds_add(&out[ia], a);
+        lds_add(&out[ib], b);
+        lds_add(&out[ic], c);
+        lds_add(&out[id], -(a + b + c));
+      } else {
+        atomic_add(&out[ia], a);
+        atomic_add(&out[ib], b);
+        atomic_add(&out[ic], c);
+        atomic_add(&out[id], -(a + b + c));
+      }
+    }
+  }
+  if (glds) {  // this workgroup's forces, once
+    __syncthreads();
+    for (int e = tid; e < 3 * g.n_atoms; e += NT) {
+      const double v = gacc[e];
+      if (v != 0.0) atomic_add(&g.grad[e], v);
     }
   }
 }

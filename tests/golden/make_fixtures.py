@@ -75,6 +75,9 @@ def literals():
         "h2o_T": _array_literal(one, "H2O_T_REF", 49),
         "h2o_V": _array_literal(one, "H2O_V_REF", 49),
         "rhf_h2o_sto3g": float(re.search(r"REF_ENERGY = (%s);" % NUM, hf).group(1)),
+        # H2O, H2, CH4 in the order of test_hf_energy.cpp
+        "rhf_sto3g_energies": [float(x) for x in re.findall(r"REF_ENERGY = (%s);" % NUM, hf)],
+        "ch4_b": float(re.search(r"constexpr Real b = (%s);" % NUM, hf).group(1)),
         "enuc_h2o": float(re.search(r"EXPECT_NEAR\(E_nuc, (%s)" % NUM, hf).group(1)),
     }
     with open(os.path.join(HERE, "reference_literals.json"), "w") as f:

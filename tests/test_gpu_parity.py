@@ -251,7 +251,10 @@ def test_graph_replays_match_eager():
 def test_device_rys_roots_exact():
     from exact import rys_exact as _rys_exact
     L = _cabi.lib()
-    Ts = np.array([0.0, 0.37, 1.0, 4.6, 17.2, 38.9, 44.0, 60.0, 85.0, 300.0])
+    # includes points half a grid step (1/32) from the Boys Taylor grid used
+    # by the one- and two-root rules, and both sides of their T cut-offs
+    Ts = np.array([0.0, 1e-10, 0.03124, 0.03126, 0.37, 1.0, 3.96875, 4.6, 12.53, 17.2, 38.9, 38.99, 39.01,
+                   44.0, 44.97, 45.02, 60.0, 85.0, 300.0])
     for n in range(1, 10):
         import torch
         tT = torch.from_numpy(Ts).cuda()
@@ -343,3 +346,21 @@ def test_gradient_rejects_f_shells():
     n = b.n_basis_functions()
     with pytest.raises(Exception, match="up to d"):
         e.compute_jk_gradient(np.eye(n), 1.0)
+
+
+# ------------------------------------------------------------ RHF (SCF loop)
+def test_rhf_energies_vs_reference_literals():
+    """RHF/STO-3G energies of H2O, H2 and CH4 through the device J/K build,
+    against the PySCF values the reference's test_hf_energy.cpp quotes (1e-10)."""
+    from libaccint.scf import rhf
+    e_h2o, e_h2, e_ch4 = LIT["rhf_sto3g_energies"]
+    b = LIT["ch4_b"]
+    cases = [
+        ([(z, tuple(p)) for z, p in LIT["h2o_geometry"]], 10, e_h2o),
+        ([(z, tuple(p)) for z, p in LIT["h2_geometry"]], 2, e_h2),
+        ([(6, (0.0, 0.0, 0.0)), (1, (b, b, b)), (1, (b, -b, -b)), (1, (-b, b, -b)), (1, (-b, -b, b))], 10, e_ch4),
+    ]
+    for atoms, nel, ref in cases:
+        r = rhf(atoms, "sto-3g", nel)
+        assert r.converged and r.iterations < 50
+        assert abs(r.energy - ref) < 1e-10, (r.energy, ref)

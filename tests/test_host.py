@@ -195,3 +195,13 @@ def test_quartet_checksum_matches_oracle_definition():
                 for l in range(j if k == i else k, ns):
                     s = (s + L.lai_quartet_mix64(i, j, k, l)) % (1 << 64)
     assert n == 120 and s == cs
+
+
+def test_nuclear_repulsion_literal():
+    """E_nuc of the H2O geometry, against the reference's test_hf_energy.cpp literal."""
+    import json
+    from conftest import golden
+    from libaccint.scf import nuclear_repulsion
+    lit = json.load(open(golden("reference_literals.json")))
+    atoms = [(z, tuple(p)) for z, p in lit["h2o_geometry"]]
+    assert abs(nuclear_repulsion(atoms) - lit["enuc_h2o"]) < 1e-10
