@@ -200,13 +200,20 @@ def main():
         Jm, Km = m_once(om)
         Em = 0.5 * np.sum(mD * Jm) - 0.25 * np.sum(mD * Km)
         ms_m = m_timed(om, msteps, 1)
+        # the same FP32 split without the FP32 primitive cut
+        on = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7, fp32_prim_cut=0.0)
+        Jn, Kn = m_once(on)
+        En = 0.5 * np.sum(mD * Jn) - 0.25 * np.sum(mD * Kn)
+        ms_n = m_timed(on, msteps, 1)
         mixed = {"mixed_workload": "C40H82 def2-SVP (1010 Cartesian bf), Schwarz 1e-12, %d quartets; FP32 Rys "
                                    "recursion where Q_ij Q_kl max|D| < 1e-7" % mi64["n_quartets"],
                  "jk_seconds_fp64_c40h82": ms_d * 1e-3, "jk_seconds_mixed": ms_m * 1e-3,
                  "mixed_speedup": ms_d / ms_m,
                  "mixed_fp32_quartet_fraction": mim["n_fp32"] / max(mim["n_quartets"], 1),
                  "dE_mixed_Eh": float(Em - Ed), "dJ_mixed_max": float(np.max(np.abs(Jm - Jd))),
-                 "dK_mixed_max": float(np.max(np.abs(Km - Kd))), "fp32_threshold": 1e-7}
+                 "dK_mixed_max": float(np.max(np.abs(Km - Kd))), "fp32_threshold": 1e-7,
+                 "fp32_prim_cut": 1e-7, "jk_seconds_mixed_no_prim_cut": ms_n * 1e-3,
+                 "dE_mixed_no_prim_cut_Eh": float(En - Ed)}
         log("C40H82: FP64 %.3f ms, mixed %.3f ms, speedup %.2fx, dE %.2e Eh, fp32 fraction %.3f" %
             (ms_d, ms_m, ms_d / ms_m, Em - Ed, mixed["mixed_fp32_quartet_fraction"]))
         del meng, tmD, tmJ, tmK

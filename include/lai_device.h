@@ -146,6 +146,13 @@ int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int fl
 int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream,
                        double* ms_fp64, double* ms_fp32);
 
+/* Relative primitive cut of the FP32 recursion of LAI_JK_MIXED: an FP32
+ * quartet skips the primitive quartets whose bound |Kc_ab| zeta^(-1/4)
+ * |Kc_cd| eta^(-1/4) is below rel_cut times the product of the two pairs'
+ * summed bounds (0 = none; default 1e-7; at most 1e-3).  FP64 quartets are
+ * unaffected.  No reference counterpart (the reference has no FP32 ERIs). */
+int lai_dev_set_mixed_prim_cut(lai_basis* b, double rel_cut);
+
 /* Capture the same J/K build into a hipGraph bound to these pointers, and
  * replay it (the per-SCF-iteration Fock build). */
 int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags,

@@ -38,6 +38,9 @@ struct JKOptions {
     bool density_weighted = false;  // also require Dmax_K(q) Q_ij Q_kl >= thr
     bool mixed_precision = false;   // FP32 recursion for small-bound quartets
     Real fp32_threshold = 1e-7;     // FP32 iff Q_ij Q_kl max|D|(q) < fp32_threshold
+    // FP32 quartets skip primitive quartets whose bound is below this fraction
+    // of the quartet's summed primitive bound (0: none)
+    Real fp32_prim_cut = 1e-7;
 };
 
 struct WorklistInfo {

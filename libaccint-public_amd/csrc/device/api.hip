@@ -100,6 +100,8 @@ struct DevBuf {
 
 // primitive-quartet magnitude below which the J/K kernels skip a primitive pair
 constexpr double kPrimCut = 1e-20;
+// default relative primitive cut of the FP32 recursion (mixed precision)
+constexpr double kMixedPrimCut = 1e-7;
 
 inline int pair_index(int ns, int i, int j) {  // i <= j, reference loop order
   return i * ns - i * (i - 1) / 2 + (j - i);
@@ -183,6 +185,9 @@ struct lai_basis {
   DevBuf<PairRec> pairs;
   DevBuf<PrimPair> pp;
   DevBuf<double2> ppab;  // exponents of the two primitives of each primitive pair
+  DevBuf<float> ppb;     // primitive-pair bounds |Kc| zeta^(-1/4)
+  DevBuf<double> ppih;   // 1 / (2 zeta) of each primitive pair
+  double pcut32 = kMixedPrimCut;
   DevBuf<int> shell_atom;
   int n_atoms = 0;
   int max_l = 0;
@@ -396,6 +401,8 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
         pbound.push_back(o.first);
       }
       r.npp = int(pph.size()) - r.pp_off;
+      r.bsum = 0.0;
+      for (int k = r.pp_off; k < r.pp_off + r.npp; ++k) r.bsum += pbound[k];
       b->pairs_h[p] = r;
       b->pair_cls[p] = pair_class(L[sa], L[sb]);
       b->pi[p] = i;
@@ -409,9 +416,14 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
     while (k < r.npp && pbound[r.pp_off + k] * bmax >= kPrimCut) ++k;
     r.npp_sig = k;
   }
+  std::vector<float> pbf(pbound.begin(), pbound.end());
+  std::vector<double> pih(pph.size());
+  for (size_t k = 0; k < pph.size(); ++k) pih[k] = 0.5 / pph[k].zeta;
   hipError_t e;
   if ((e = b->pairs.assign(b->pairs_h.data(), b->pairs_h.size())) != hipSuccess ||
       (e = b->pp.assign(pph.data(), pph.size())) != hipSuccess ||
+      (e = b->ppb.assign(pbf.data(), pbf.size())) != hipSuccess ||
+      (e = b->ppih.assign(pih.data(), pih.size())) != hipSuccess ||
       (e = b->ppab.assign(pabh.data(), pabh.size())) != hipSuccess ||
       (e = b->dL.assign(b->L.data(), ns)) != hipSuccess ||
       (e = b->dnprim.assign(b->nprim.data(), ns)) != hipSuccess ||
@@ -436,6 +448,8 @@ extern "C" int lai_dev_basis_free(lai_basis* b) {
   b->pairs.release();
   b->pp.release();
   b->ppab.release();
+  b->ppb.release();
+  b->ppih.release();
   b->shell_atom.release();
   b->dL.release();
   b->dnprim.release();
@@ -512,6 +526,9 @@ int run_eri_batch(lai_basis* b, EriBatch& eb, double* d_out, hipStream_t st) {
     a.pairs = b->pairs.p;
     a.pp = b->pp.p;
     a.rys = b->ctx->rys.p;
+  a.ppb = b->ppb.p;
+  a.ppih = b->ppih.p;
+  a.pcut32 = b->pcut32;
     a.mode = 1;
     a.out = d_out;
     a.perm = dp.p;
@@ -759,12 +776,18 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     }
   };
   std::vector<std::vector<JkChunk>> C64(kNumClasses), C32(kNumClasses);
+  // Register-shape classes: 256 lanes per chunk, up to 8 quartets per lane;
+  // LDS-shape classes: one quartet at a time per workgroup, up to 32.  Small
+  // lists get shorter chunks so that a launch still has a few workgroups per
+  // CU.
+  const int n_cu = b->ctx->n_cu;
+  auto chunk_max = [&](int c, size_t n) {
+    if (Ls.info[c].regs) return int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 256, 2048));
+    return int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 1, 32));
+  };
   parallel_for(kNumClasses, [&](int c, int) {
-    // register-shape classes: 256 lanes per chunk, several quartets per lane;
-    // LDS-shape classes: one quartet at a time per workgroup
-    const int chunk_max = Ls.info[c].regs ? 2048 : 32;
-    anchor(L64[c], C64[c], chunk_max);
-    anchor(L32[c], C32[c], chunk_max);
+    anchor(L64[c], C64[c], chunk_max(c, L64[c].size()));
+    anchor(L32[c], C32[c], chunk_max(c, L32[c].size()));
   }, 1);
   for (int c = 0; c < kNumClasses; ++c) {
     hipError_t e;
@@ -855,6 +878,9 @@ extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* 
   a.pairs = b->pairs.p;
   a.pp = b->pp.p;
   a.rys = b->ctx->rys.p;
+  a.ppb = b->ppb.p;
+  a.ppih = b->ppih.p;
+  a.pcut32 = b->pcut32;
   a.D = d_D;
   a.Jt = b->Jt.p;
   a.Kt = b->Kt.p;
@@ -975,6 +1001,19 @@ extern "C" int lai_dev_boys(int nmax, int n, const double* d_T, double* d_out, v
 // ---------------------------------------------------------------------------
 // nuclear gradient
 // ---------------------------------------------------------------------------
+extern "C" int lai_dev_set_mixed_prim_cut(lai_basis* b, double rel_cut) {
+  if (!b || !(rel_cut >= 0.0) || rel_cut > 1e-3)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_mixed_prim_cut: cut must lie in [0, 1e-3]");
+  if (b->pcut32 != rel_cut && b->graph_exec) {  // a captured graph holds the old value
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->pcut32 = rel_cut;
+  return LAI_OK;
+}
+
 extern "C" int lai_dev_set_shell_atoms(lai_basis* b, const int* atom, int n_atoms) {
   if (!b || !atom || n_atoms <= 0) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_shell_atoms: invalid arguments");
   for (int s = 0; s < b->ns; ++s)
@@ -1044,6 +1083,9 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
   a.pairs = b->pairs.p;
   a.pp = b->pp.p;
   a.rys = b->ctx->rys.p;
+  a.ppb = b->ppb.p;
+  a.ppih = b->ppih.p;
+  a.pcut32 = b->pcut32;
   a.D = d_D;
   a.Jt = b->Jt.p;
   a.Kt = b->Kt.p;

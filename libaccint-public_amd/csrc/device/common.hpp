@@ -78,7 +78,7 @@ struct PairRec {
   int npp_sig;        // leading primitive pairs used by the J/K and gradient kernels
                       // (the rest are below the primitive screening bound)
   double ABx, ABy, ABz;  // A - B (horizontal transfer)
-  double pad2;
+  double bsum;           // sum of the primitive-pair bounds |Kc| zeta^(-1/4)
 };
 
 // One primitive pair, 64 bytes (one cache line sector per lane):

@@ -11,8 +11,11 @@
 namespace lai {
 namespace {
 
-// LDS a workgroup may use (static tables plus the anchored exchange rows)
+// LDS a workgroup may use for the anchored exchange rows: register-shape
+// kernels keep several workgroups per CU; the LDS-shape kernels may take the
+// whole 160 KB of gfx950
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kLdsBudgetB = 160 * 1024;
 
 template <int ID, typename R>
 void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
@@ -45,12 +48,16 @@ void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
     hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
   } else {
     using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-    constexpr size_t stat = size_t(Sh::NR) * 3 * (Sh::S + Sh::HS) * sizeof(R);
+    constexpr size_t stat = size_t(Sh::template npq<P::NT>()) * (Sh::TS + Sh::HSS) * sizeof(R);
     const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
-    a.klds = stat + dyn <= kLdsBudget ? 1 : 0;
+    a.klds = stat + dyn <= kLdsBudgetB ? 1 : 0;
+    auto kern = jk_kernel_lds<La, Lb, Lc, Ld, R, P::NT>;
+    if (a.klds && stat + dyn > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(dyn)) != hipSuccess)
+      a.klds = 0;  // exchange rows straight to HBM
     int blocks = a.nchunks < 8 * max_blocks ? a.nchunks : 8 * max_blocks;
-    hipLaunchKernelGGL((jk_kernel_lds<La, Lb, Lc, Ld, R, P::NT>), dim3(blocks), dim3(P::NT), a.klds ? dyn : 0, s,
-                       a);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(P::NT), a.klds ? dyn : 0, s, a);
   }
 }
 

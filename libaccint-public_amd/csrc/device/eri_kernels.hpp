@@ -44,6 +44,12 @@ struct EriArgs {
   const PairRec* __restrict__ pairs;
   const PrimPair* __restrict__ pp;
   const double* __restrict__ rys;
+  // bounds |Kc| zeta^(-1/4) of the primitive pairs, and the relative cut of
+  // the FP32 recursion: primitive quartets with bound below
+  // pcut32 * bsum(bra) * bsum(ket) are skipped (0: none)
+  const float* __restrict__ ppb;
+  double pcut32;
+  const double* __restrict__ ppih;  // 1 / (2 zeta) of each primitive pair
   // mode 0: digestion
   const double* __restrict__ D;
   double* Jt;
@@ -157,22 +163,38 @@ __device__ __forceinline__ void quartet_block_regs(
   const R CDx = R(ket.ABx), CDy = R(ket.ABy), CDz = R(ket.ABz);
   const int nb_pp = all_prims ? bra.npp : bra.npp_sig;
   const int nk_pp = all_prims ? ket.npp : ket.npp_sig;
-  for (int ib = 0; ib < nb_pp; ++ib) {
-    const PrimPair P = g.pp[bra.pp_off + ib];
-    const double zeta = P.zeta;
-    const double h_z = 0.5 / zeta;
-    for (int ik = 0; ik < nk_pp; ++ik) {
-      const PrimPair Q = g.pp[ket.pp_off + ik];
-      const double eta = Q.zeta;
-      const double izpe = 1.0 / (zeta + eta);
+  // FP32 part: primitive pairs are sorted by bound, so both loops stop at the
+  // first primitive quartet below the relative cut
+  constexpr bool fp32 = sizeof(R) == 4;
+  const float qcut = fp32 && !all_prims ? float(g.pcut32 * bra.bsum * ket.bsum) : 0.0f;
+  const float bb0 = fp32 ? g.ppb[bra.pp_off] : 0.0f;
+  // ket primitive pairs outside (one load per lane and pair), bra primitive
+  // pairs inside: the bra is the same for the whole workgroup in the J/K
+  // kernels, so its records are uniform (scalar) loads
+  for (int ik = 0; ik < nk_pp; ++ik) {
+    const float bk = fp32 ? g.ppb[ket.pp_off + ik] : 0.0f;
+    if (fp32 && bk * bb0 < qcut) break;
+    const PrimPair Q = g.pp[ket.pp_off + ik];
+    const double eta = Q.zeta;
+    const double h_e = g.ppih[ket.pp_off + ik];  // 1 / (2 eta)
+    for (int ib = 0; ib < nb_pp; ++ib) {
+      if (fp32 && g.ppb[bra.pp_off + ib] * bk < qcut) break;
+      const PrimPair P = g.pp[bra.pp_off + ib];
+      const double zeta = P.zeta;
+      const double h_z = g.ppih[bra.pp_off + ib];  // 1 / (2 zeta)
+      // 1/sqrt(zeta + eta) by v_rsq_f64 and two Newton steps; 1/(zeta + eta) = its square
+      const double x = zeta + eta;
+      double y = __builtin_amdgcn_rsq(x);
+      y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+      y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+      const double izpe = y * y;
       const double rz = eta * izpe;   // rho / zeta
       const double re = zeta * izpe;  // rho / eta
       const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
       const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = P.Kc * Q.Kc * sqrt(izpe);
+      const double pref = P.Kc * Q.Kc * y;
       double u[NR], w[NR];
       rys_roots<NR>(T, g.rys, u, w);
-      const double h_e = 0.5 / eta;
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         const double ur = u[r];
@@ -363,6 +385,18 @@ struct LdsShape {
   static constexpr int LCD = Lc + Ld;
   static constexpr int HS = (La + 1) * (Lb + 1) * (LCD + 1);  // bra-transferred scratch
   static constexpr int NOUT = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld);
+  static constexpr int TS = NR * 3 * S;    // table entries of one primitive quartet
+  static constexpr int HSS = NR * 3 * HS;  // transfer scratch of one primitive quartet
+  // primitive quartets whose tables are built at once by NT lanes, within
+  // 16 KB of tables and scratch (more LDS costs workgroups per CU, which
+  // these one-wavefront workgroups need more)
+  template <int NT>
+  static constexpr int npq() {
+    constexpr int by_lanes = NT / (3 * NR);
+    constexpr int by_lds = int((16 * 1024) / ((TS + HSS) * sizeof(R)));
+    constexpr int n = by_lanes < by_lds ? by_lanes : by_lds;
+    return n < 1 ? 1 : n;
+  }
 };
 
 // Builds one (root, axis) table I[a][b][c][d] into tab (stride layout
@@ -509,58 +543,75 @@ __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
   }
 }
 
-// One shell quartet by the whole workgroup: for every primitive quartet 3*NR
-// lanes build the (root, axis) tables in LDS, then every lane accumulates its
-// outputs.  Starts and ends with a workgroup barrier.
-template <int La, int Lb, int Lc, int Ld, typename R, int NT, int NJ>
+// One shell quartet by the whole workgroup.  Primitive quartets are taken NPQ
+// at a time: 3*NR*NPQ lanes build the (primitive quartet, root, axis) tables
+// in LDS, then every lane accumulates its outputs over the batch.  Starts and
+// ends with a workgroup barrier.
+template <int La, int Lb, int Lc, int Ld, typename R, int NT, int NJ, int NPQ>
 __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket,
                                                 bool all_prims, R* tab, R* hs, const int (&ofs)[NJ],
                                                 R (&acc)[NJ]) {
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-  constexpr int NR = Sh::NR, S = Sh::S;
+  constexpr int NR = Sh::NR, S = Sh::S, TS = Sh::TS, HSS = Sh::HSS, TASKS = 3 * NR;
+  static_assert(NPQ * TASKS <= NT, "not enough lanes for the table batch");
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) acc[j] = R(0);
   const int npp_b = all_prims ? bra.npp : bra.npp_sig, npp_k = all_prims ? ket.npp : ket.npp_sig;
+  const int ntot = npp_b * npp_k;
+  constexpr bool fp32 = sizeof(R) == 4;
+  const float qcut = fp32 && !all_prims ? float(g.pcut32 * bra.bsum * ket.bsum) : 0.0f;
   __syncthreads();  // the previous user of tab and hs has finished
-  for (int t = 0; t < npp_b * npp_k; ++t) {
-    const int ib = t / npp_k, ik = t - ib * npp_k;
-    if (tid < 3 * NR) {
-      const int r = tid / 3, ax = tid - 3 * (tid / 3);
-      const PrimPair P = g.pp[bra.pp_off + ib];
-      const PrimPair Q = g.pp[ket.pp_off + ik];
-      const double zeta = P.zeta, eta = Q.zeta;
-      const double izpe = 1.0 / (zeta + eta);
-      const double rz = eta * izpe, re = zeta * izpe;
-      const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
-      const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
-      double ur, wr;
-      rys_root_one<NR>(T, g.rys, r, ur, wr);
-      const double h_z = 0.5 / zeta, h_e = 0.5 / eta;
-      const R B00 = R(0.5 * ur * izpe);
-      const R B10 = R(h_z - h_z * rz * ur);
-      const R B01 = R(h_e - h_e * re * ur);
-      const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
-      const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
-      const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
-      const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
-      const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
-      const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
-      build_2d_lds<La, Lb, Lc, Ld, R>(tab + (r * 3 + ax) * S, hs + (r * 3 + ax) * Sh::HS,
-                                      R(PA - rz * ur * PQ), R(QC + re * ur * PQ), R(AB), R(CD), B10, B01,
-                                      B00, R(sc));
-    }
-    __syncthreads();
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) {
-      const R* tx = tab + (r * 3) * S;
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const int p = ofs[j];
-        acc[j] += tx[p & 1023] * tx[S + ((p >> 10) & 1023)] * tx[2 * S + (p >> 20)];
+  for (int t0 = 0; t0 < ntot; t0 += NPQ) {
+    if (tid < NPQ * TASKS) {
+      const int pq = tid / TASKS, task = tid - pq * TASKS;
+      const int r = task / 3, ax = task - 3 * r;
+      const int t = t0 + pq;
+      R* tb = tab + pq * TS + task * S;
+      bool live = t < ntot;
+      const int ib = live ? t / npp_k : 0, ik = live ? t - ib * npp_k : 0;
+      if (live && fp32 && g.ppb[bra.pp_off + ib] * g.ppb[ket.pp_off + ik] < qcut) live = false;
+      if (live) {
+        const PrimPair P = g.pp[bra.pp_off + ib];
+        const PrimPair Q = g.pp[ket.pp_off + ik];
+        const double zeta = P.zeta, eta = Q.zeta;
+        const double izpe = 1.0 / (zeta + eta);
+        const double rz = eta * izpe, re = zeta * izpe;
+        const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+        const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
+        double ur, wr;
+        rys_root_one<NR>(T, g.rys, r, ur, wr);
+        const double h_z = g.ppih[bra.pp_off + ib], h_e = g.ppih[ket.pp_off + ik];
+        const R B00 = R(0.5 * ur * izpe);
+        const R B10 = R(h_z - h_z * rz * ur);
+        const R B01 = R(h_e - h_e * re * ur);
+        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
+        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
+        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
+        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
+        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+        const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
+        build_2d_lds<La, Lb, Lc, Ld, R>(tb, hs + pq * HSS + task * Sh::HS, R(PA - rz * ur * PQ),
+                                        R(QC + re * ur * PQ), R(AB), R(CD), B10, B01, B00, R(sc));
+      } else {
+        for (int e = 0; e < S; ++e) tb[e] = R(0);  // contributes nothing
       }
     }
-    __syncthreads();  // products finished with tab before the next build
+    __syncthreads();
+    const int npq = ntot - t0 < NPQ ? ntot - t0 : NPQ;
+#pragma unroll 1
+    for (int pq = 0; pq < npq; ++pq) {
+#pragma unroll 1
+      for (int r = 0; r < NR; ++r) {
+        const R* tx = tab + pq * TS + (r * 3) * S;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int p = ofs[j];
+          acc[j] += tx[p & 1023] * tx[S + ((p >> 10) & 1023)] * tx[2 * S + (p >> 20)];
+        }
+      }
+    }
+    __syncthreads();  // products finished with tab before the next batch
   }
 }
 
@@ -571,8 +622,9 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
   constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NJ = (NOUT + NT - 1) / NT;
-  __shared__ R tab[NR * 3 * S];
-  __shared__ R hs[NR * 3 * Sh::HS];
+  constexpr int NPQ = Sh::template npq<NT>();
+  __shared__ R tab[NPQ * Sh::TS];
+  __shared__ R hs[NPQ * Sh::HSS];
   const int tid = threadIdx.x;
   int ofs[NJ];
   lds_component_offsets<La, Lb, Lc, Ld, NT, NJ>(tid, ofs);
@@ -582,7 +634,7 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
     const PairRec bra = g.pairs[q.bra];
     const PairRec ket = g.pairs[q.ket];
     R acc[NJ];
-    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ>(g, bra, ket, g.mode == 1, tab, hs, ofs, acc);
+    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, g.mode == 1, tab, hs, ofs, acc);
 
     const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
     const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
@@ -793,8 +845,9 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NJ = (NOUT + NT - 1) / NT;
   constexpr int ROWS = na + nb;
-  __shared__ R tab[NR * 3 * S];
-  __shared__ R hs[NR * 3 * Sh::HS];
+  constexpr int NPQ = Sh::template npq<NT>();
+  __shared__ R tab[NPQ * Sh::TS];
+  __shared__ R hs[NPQ * Sh::HSS];
   extern __shared__ double dyn[];
   const int tid = threadIdx.x;
   const int n = g.nbf;
@@ -818,7 +871,7 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
       const Quartet q = g.q[ch.start + k];
       const PairRec ket = g.pairs[q.ket];
       R acc[NJ];
-      lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ>(g, bra, ket, false, tab, hs, ofs, acc);
+      lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, false, tab, hs, ofs, acc);
       const int fc = ket.fa, fd = ket.fb;
       const bool kl = ket.same, bk = (q.ket == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);

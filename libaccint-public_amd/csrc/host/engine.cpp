@@ -303,6 +303,7 @@ WorklistInfo Engine::prepare_worklist(const JKOptions& o, const Real* D_host) {
 
 void Engine::build_jk(const JKOptions& o, const Real* d_D, Real* d_J, Real* d_K, hipStream_t s, const Real* D_host) {
     prepare_worklist(o, D_host);
+    device::check(lai_dev_set_mixed_prim_cut(device_basis(), o.fp32_prim_cut), "build_jk");
     device::check(lai_dev_jk(device_basis(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
                   "build_jk");
 }
@@ -321,6 +322,7 @@ void Engine::build_jk_host(const JKOptions& o, const Real* D, Real* J, Real* K) 
 
 void Engine::capture_jk_graph(const JKOptions& o, const Real* d_D, Real* d_J, Real* d_K, hipStream_t s) {
     prepare_worklist(o, nullptr);
+    device::check(lai_dev_set_mixed_prim_cut(device_basis(), o.fp32_prim_cut), "capture_jk_graph");
     device::check(
         lai_dev_jk_graph_capture(device_basis(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
         "capture_jk_graph");

@@ -46,6 +46,7 @@ _SIGNATURES = {
     "lai_dev_jk_profile": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _dp, _dp], ctypes.c_int),
     "lai_dev_jk_graph_capture": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_jk_graph_launch": ([_vp, _vp], ctypes.c_int),
+    "lai_dev_set_mixed_prim_cut": ([_vp, ctypes.c_double], ctypes.c_int),
     "lai_dev_set_shell_atoms": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
     "lai_dev_jk_grad": ([_vp, _vp, ctypes.c_double, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_onee": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp, _vp], ctypes.c_int),
