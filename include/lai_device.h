@@ -73,7 +73,27 @@ int lai_dev_upload_basis(lai_ctx* ctx, int n_shells, const int* L, const double*
                          const int* nprim, const double* exps, const double* coefs,
                          lai_basis** out);
 int lai_dev_basis_free(lai_basis* b);
+/* basis functions of all molecules of the handle */
 int lai_dev_basis_nbf(const lai_basis* b);
+
+/* ---- batches of molecules ------------------------------------------------------
+ * One handle for several independent molecules (the uneven multi-molecule
+ * workload): molecule m owns shells [mol_shell_off[m], mol_shell_off[m+1])
+ * of the arrays (same layout as lai_dev_upload_basis).  Shell pairs, Schwarz
+ * bounds and quartets never mix molecules; every J/K class kernel covers all
+ * molecules in one launch.  D, J and K are the molecules' nbf_m x nbf_m
+ * row-major matrices one after another (lai_dev_batch_layout); the Schwarz
+ * table is the molecules' packed tables one after another; shell_dmax of
+ * lai_dev_build_worklist is the molecules' n_shells_m x n_shells_m blocks one
+ * after another.  One-electron matrices need a single-molecule handle.
+ * No reference counterpart (the reference computes one BasisSet per Engine). */
+int lai_dev_upload_batch(lai_ctx* ctx, int n_mol, const int* mol_shell_off, int n_shells, const int* L,
+                         const double* centers, const int* nprim, const double* exps, const double* coefs,
+                         lai_basis** out);
+/* n_mol; per molecule its basis functions (nbf[n_mol]) and the offset of its
+ * block in the batched matrices (mat_off[n_mol + 1], the last entry the total
+ * length); any output may be NULL */
+int lai_dev_batch_layout(const lai_basis* b, int* n_mol, int* nbf, int64_t* mat_off);
 
 /* ---- raw ERIs ---------------------------------------------------------------
  * (ab|cd) blocks for n arbitrary shell quartets (shells4 = 4n shell indices),

@@ -18,6 +18,7 @@
 #include "eri_kernels.hpp"
 #include "grad_kernels.hpp"
 #include "launch.hpp"
+#include "wl_kernels.hpp"
 
 using namespace lai;
 
@@ -167,18 +168,25 @@ struct lai_ctx {
 };
 
 struct ClassList {
-  DevBuf<Quartet> q64, q32;   // quartets grouped by bra pair (FP64 part, FP32 part)
-  DevBuf<JkChunk> c64, c32;   // runs of each list with one bra pair
-  int64_t n64 = 0, n32 = 0;
-  int64_t nc64 = 0, nc32 = 0;
+  DevBuf<JkChunk> chunks;  // bra-anchored chunks over the ket segments
+  int64_t nchunks = 0;
 };
 
 struct lai_basis {
   lai_ctx* ctx = nullptr;
-  int ns = 0, nbf = 0;
+  int ns = 0, nbf = 0;  // shells and basis functions of all molecules
+  // molecule m: shells [mol_shell[m], mol_shell[m+1]), pairs
+  // [mol_pair[m], mol_pair[m+1]), functions 0..mol_nbf[m]-1 and its D, J, K
+  // blocks at mol_moff[m] in the batched matrices
+  int n_mol = 1;
+  std::vector<int> mol_shell, mol_pair, mol_nbf;
+  std::vector<long long> mol_moff, mol_dmoff;
+  std::vector<int> sh_mol, sh_loc;
+  long long mat_size = 0;
+  int max_nbf = 0;
   std::vector<int> L, nprim, prim_off, func_off;
   std::vector<double> center, exps, coefs;
-  // pairs in reference order (i <= j)
+  // pairs, molecule by molecule, each in reference order (i <= j)
   std::vector<PairRec> pairs_h;
   std::vector<int> pair_cls;  // pair class of each pair
   std::vector<int> pi, pj;    // shells of each pair
@@ -194,22 +202,61 @@ struct lai_basis {
   // device copies of shell data for the one-electron kernel
   DevBuf<int> dL, dnprim, dprim_off, dfunc_off;
   DevBuf<double> dcenter, dexps, dcoefs;
-  // Schwarz
-  std::vector<double> Q;  // packed j*(j+1)/2+i
+  DevBuf<long long> dmoff;  // mol_moff on the device
+  DevBuf<int> dmnbf;        // mol_nbf on the device
+  DevBuf<int> dshloc;       // sh_loc on the device
+  DevBuf<long long> ddmoff; // mol_dmoff on the device
+  DevBuf<int> ddmns;        // shells per molecule on the device
+  // Schwarz: packed per molecule, mol_pair[m] + j*(j+1)/2 + i (molecule-local shells)
+  std::vector<double> Q;
   bool have_Q = false;
+  DevBuf<double> dQ;  // per pair, pair order
   // worklist
   ClassList cls[kNumClasses];
+  DevBuf<JkSegs> segs;
+  DevBuf<int> kets;
+  DevBuf<double> sdmax;  // per molecule: ns_m x ns_m shell-pair max|D|
   bool have_worklist = false;
+  int wl_dscreen = 0, wl_mixed = 0;
+  double wl_thr = 0.0, wl_tau = 0.0;
   int64_t counts[kNumClasses] = {};
   double cls_f64[kNumClasses] = {}, cls_f32[kNumClasses] = {};
   double flops64 = 0, flops32 = 0;
-  int64_t n_eri = 0;
+  int64_t n_eri = 0, n_fp32 = 0;
+  uint64_t checksum = 0;
   // J/K scratch (unsymmetrised)
   DevBuf<double> Jt, Kt;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
 };
+
+// pair index of molecule-local shells il <= jl of molecule m
+inline int mol_pair_index(const lai_basis* b, int m, int il, int jl) {
+  const int ns = b->mol_shell[m + 1] - b->mol_shell[m];
+  return b->mol_pair[m] + pair_index(ns, il, jl);
+}
+// packed Schwarz index of pair p
+inline size_t packed_q_index(const lai_basis* b, int p) {
+  const int m = b->sh_mol[b->pi[p]];
+  const size_t i = size_t(b->sh_loc[b->pi[p]]), j = size_t(b->sh_loc[b->pj[p]]);
+  return size_t(b->mol_pair[m]) + j * (j + 1) / 2 + i;
+}
+
+// the per-quartet selection of the current worklist
+inline QuartetSel make_sel(const lai_basis* b) {
+  QuartetSel s{};
+  s.Qpair = b->dQ.p;
+  s.sdmax = b->sdmax.p;
+  s.dm_off = b->ddmoff.p;
+  s.dm_ns = b->ddmns.p;
+  s.sh_loc = b->dshloc.p;
+  s.dscreen = b->wl_dscreen;
+  s.thr = b->wl_thr;
+  s.mixed = b->wl_mixed;
+  s.tau = b->wl_tau;
+  return s;
+}
 
 // ---------------------------------------------------------------------------
 // context and memory
@@ -311,9 +358,22 @@ extern "C" int lai_dev_stream_sync(void* stream) {
 extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const double* centers,
                                     const int* nprim, const double* exps, const double* coefs,
                                     lai_basis** out) {
-  if (!ctx || !out || ns < 0 || (ns > 0 && (!L || !centers || !nprim || !exps || !coefs)))
-    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_basis: invalid arguments");
-  if (ns > 65535) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_basis: more than 65535 shells");
+  const int off[2] = {0, ns < 0 ? 0 : ns};
+  return lai_dev_upload_batch(ctx, 1, off, ns, L, centers, nprim, exps, coefs, out);
+}
+
+extern "C" int lai_dev_upload_batch(lai_ctx* ctx, int n_mol, const int* mol_shell_off, int ns, const int* L,
+                                    const double* centers, const int* nprim, const double* exps,
+                                    const double* coefs, lai_basis** out) {
+  if (!ctx || !out || ns < 0 || n_mol < 1 || !mol_shell_off ||
+      (ns > 0 && (!L || !centers || !nprim || !exps || !coefs)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_batch: invalid arguments");
+  if (ns > 65535) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_batch: more than 65535 shells");
+  if (mol_shell_off[0] != 0 || mol_shell_off[n_mol] != ns)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_batch: molecule offsets must run from 0 to n_shells");
+  for (int m = 0; m < n_mol; ++m)
+    if (mol_shell_off[m + 1] < mol_shell_off[m])
+      return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_upload_batch: decreasing molecule offsets");
   *out = nullptr;
   for (int s = 0; s < ns; ++s) {
     if (L[s] < 0 || L[s] > kMaxL)
@@ -324,24 +384,44 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
   auto* b = new lai_basis;
   b->ctx = ctx;
   b->ns = ns;
+  b->n_mol = n_mol;
   b->L.assign(L, L + ns);
   b->nprim.assign(nprim, nprim + ns);
   b->center.assign(centers, centers + 3 * ns);
+  b->mol_shell.assign(mol_shell_off, mol_shell_off + n_mol + 1);
+  b->sh_mol.resize(ns);
+  b->sh_loc.resize(ns);
   int np = 0, nbf = 0;
-  for (int s = 0; s < ns; ++s) {
-    b->prim_off.push_back(np);
-    b->func_off.push_back(nbf);
-    np += nprim[s];
-    nbf += ncart(L[s]);
+  b->mol_pair.assign(1, 0);
+  b->mol_moff.assign(1, 0);
+  b->mol_dmoff.assign(1, 0);
+  for (int m = 0; m < n_mol; ++m) {
+    int mf = 0;
+    for (int s = b->mol_shell[m]; s < b->mol_shell[m + 1]; ++s) {
+      b->sh_mol[s] = m;
+      b->sh_loc[s] = s - b->mol_shell[m];
+      b->prim_off.push_back(np);
+      b->func_off.push_back(mf);  // within the molecule
+      np += nprim[s];
+      mf += ncart(L[s]);
+    }
+    const long long nsm = b->mol_shell[m + 1] - b->mol_shell[m];
+    b->mol_nbf.push_back(mf);
+    b->mol_pair.push_back(b->mol_pair.back() + int(nsm * (nsm + 1) / 2));
+    b->mol_moff.push_back(b->mol_moff.back() + (long long)mf * mf);
+    b->mol_dmoff.push_back(b->mol_dmoff.back() + nsm * nsm);
+    b->max_nbf = std::max(b->max_nbf, mf);
+    nbf += mf;
   }
   b->nbf = nbf;
+  b->mat_size = b->mol_moff.back();
   for (int s = 0; s < ns; ++s) b->max_l = std::max(b->max_l, L[s]);
   b->exps.assign(exps, exps + np);
   b->coefs.assign(coefs, coefs + np);
 
-  // pair records and primitive-pair tables
+  // pair records and primitive-pair tables, molecule by molecule
   const double sq2pi52 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
-  const int npairs = ns * (ns + 1) / 2;
+  const int npairs = b->mol_pair.back();
   b->pairs_h.resize(npairs);
   b->pair_cls.resize(npairs);
   b->pi.resize(npairs);
@@ -351,8 +431,9 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
   std::vector<double> pbound;
   double bmax = 0.0;
   int p = 0;
-  for (int i = 0; i < ns; ++i)
-    for (int j = i; j < ns; ++j, ++p) {
+  for (int m = 0; m < n_mol; ++m)
+  for (int i = b->mol_shell[m]; i < b->mol_shell[m + 1]; ++i)
+    for (int j = i; j < b->mol_shell[m + 1]; ++j, ++p) {
       const int sa = (L[i] >= L[j]) ? i : j, sb = (sa == i) ? j : i;
       PairRec r{};
       r.sa = sa;
@@ -360,6 +441,9 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
       r.fa = b->func_off[sa];
       r.fb = b->func_off[sb];
       r.same = (i == j);
+      r.mn = b->mol_nbf[m];
+      r.mol = m;
+      r.moff = b->mol_moff[m];
       const double* A = &centers[3 * sa];
       const double* B = &centers[3 * sb];
       r.ABx = A[0] - B[0];
@@ -417,6 +501,8 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
     r.npp_sig = k;
   }
   std::vector<float> pbf(pbound.begin(), pbound.end());
+  std::vector<int> dmns(n_mol);
+  for (int m = 0; m < n_mol; ++m) dmns[m] = b->mol_shell[m + 1] - b->mol_shell[m];
   std::vector<double> pih(pph.size());
   for (size_t k = 0; k < pph.size(); ++k) pih[k] = 0.5 / pph[k].zeta;
   hipError_t e;
@@ -432,8 +518,13 @@ extern "C" int lai_dev_upload_basis(lai_ctx* ctx, int ns, const int* L, const do
       (e = b->dcenter.assign(b->center.data(), b->center.size())) != hipSuccess ||
       (e = b->dexps.assign(b->exps.data(), b->exps.size())) != hipSuccess ||
       (e = b->dcoefs.assign(b->coefs.data(), b->coefs.size())) != hipSuccess ||
-      (e = b->Jt.alloc(size_t(nbf) * nbf)) != hipSuccess ||
-      (e = b->Kt.alloc(size_t(nbf) * nbf)) != hipSuccess) {
+      (e = b->dmoff.assign(b->mol_moff.data(), b->mol_moff.size())) != hipSuccess ||
+      (e = b->dmnbf.assign(b->mol_nbf.data(), b->mol_nbf.size())) != hipSuccess ||
+      (e = b->dshloc.assign(b->sh_loc.data(), b->sh_loc.size())) != hipSuccess ||
+      (e = b->ddmoff.assign(b->mol_dmoff.data(), b->mol_dmoff.size())) != hipSuccess ||
+      (e = b->ddmns.assign(dmns.data(), dmns.size())) != hipSuccess ||
+      (e = b->Jt.alloc(size_t(b->mat_size))) != hipSuccess ||
+      (e = b->Kt.alloc(size_t(b->mat_size))) != hipSuccess) {
     lai_dev_basis_free(b);
     return fail(LAI_ERR_MEMORY, std::string("lai_dev_upload_basis: ") + hipGetErrorString(e));
   }
@@ -460,17 +551,32 @@ extern "C" int lai_dev_basis_free(lai_basis* b) {
   b->dcoefs.release();
   b->Jt.release();
   b->Kt.release();
-  for (auto& c : b->cls) {
-    c.q64.release();
-    c.q32.release();
-    c.c64.release();
-    c.c32.release();
-  }
+  for (auto& c : b->cls) c.chunks.release();
+  b->segs.release();
+  b->kets.release();
+  b->sdmax.release();
+  b->dQ.release();
+  b->dmoff.release();
+  b->dmnbf.release();
+  b->dshloc.release();
+  b->ddmoff.release();
+  b->ddmns.release();
   delete b;
   return LAI_OK;
 }
 
 extern "C" int lai_dev_basis_nbf(const lai_basis* b) { return b ? b->nbf : -1; }
+
+extern "C" int lai_dev_batch_layout(const lai_basis* b, int* n_mol, int* nbf, int64_t* mat_off) {
+  if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_batch_layout: null basis");
+  if (n_mol) *n_mol = b->n_mol;
+  for (int m = 0; m < b->n_mol; ++m) {
+    if (nbf) nbf[m] = b->mol_nbf[m];
+    if (mat_off) mat_off[m] = b->mol_moff[m];
+  }
+  if (mat_off) mat_off[b->n_mol] = b->mol_moff[b->n_mol];
+  return LAI_OK;
+}
 
 // ---------------------------------------------------------------------------
 // raw ERI blocks
@@ -479,10 +585,16 @@ namespace {
 // Orders a shell quartet into class order and returns the class id, the
 // (bra, ket) pair indices and the permutation code (class position p ->
 // original position, 2 bits each).
+// Returns -1 when a pair's two shells belong to different molecules.
 int classify(const lai_basis* b, const int s[4], Quartet& q, int& perm) {
-  const int ns = b->ns;
-  auto pidx = [&](int x, int y) { return x <= y ? pair_index(ns, x, y) : pair_index(ns, y, x); };
+  auto pidx = [&](int x, int y) -> int {
+    const int m = b->sh_mol[x];
+    if (b->sh_mol[y] != m) return -1;
+    const int lx = b->sh_loc[x], ly = b->sh_loc[y];
+    return lx <= ly ? mol_pair_index(b, m, lx, ly) : mol_pair_index(b, m, ly, lx);
+  };
   const int p0 = pidx(s[0], s[1]), p1 = pidx(s[2], s[3]);
+  if (p0 < 0 || p1 < 0) return -1;
   const int c0 = b->pair_cls[p0], c1 = b->pair_cls[p1];
   const bool swap_bk = c0 < c1;
   const int pb = swap_bk ? p1 : p0, pk = swap_bk ? p0 : p1;
@@ -526,14 +638,14 @@ int run_eri_batch(lai_basis* b, EriBatch& eb, double* d_out, hipStream_t st) {
     a.pairs = b->pairs.p;
     a.pp = b->pp.p;
     a.rys = b->ctx->rys.p;
-  a.ppb = b->ppb.p;
-  a.ppih = b->ppih.p;
-  a.pcut32 = b->pcut32;
+    a.ppb = b->ppb.p;
+    a.ppih = b->ppih.p;
+    a.pcut32 = b->pcut32;
     a.mode = 1;
     a.out = d_out;
     a.perm = dp.p;
     a.obase = dob.p;
-    a.nbf = b->nbf;
+    a.nbf = b->max_nbf;
     Ls.f64[c](a, st, b->ctx->n_cu * 16);
     LAI_HIP(hipGetLastError());
     LAI_HIP(hipStreamSynchronize(st));
@@ -554,6 +666,7 @@ extern "C" int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* s4, doub
     Quartet q;
     int perm;
     const int c = classify(b, s, q, perm);
+    if (c < 0) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: the shells of a pair must belong to one molecule");
     offsets[t] = off;
     eb.q[c].push_back(q);
     eb.perm[c].push_back(perm);
@@ -576,7 +689,7 @@ extern "C" int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* s4, doub
 // ---------------------------------------------------------------------------
 extern "C" int lai_dev_schwarz(lai_basis* b, double* Qout) {
   if (!b) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_schwarz: null basis");
-  const int ns = b->ns, npairs = ns * (ns + 1) / 2;
+  const int npairs = b->mol_pair.back();
   EriBatch eb;
   std::vector<long long> offs(npairs);
   std::vector<int> ni(npairs), nj(npairs);
@@ -610,12 +723,9 @@ extern "C" int lai_dev_schwarz(lai_basis* b, double* Qout) {
   LAI_HIP(hipGetLastError());
   std::vector<double> Qp(npairs);
   LAI_HIP(hipMemcpy(Qp.data(), dQ.p, sizeof(double) * npairs, hipMemcpyDeviceToHost));
-  // reference packing j*(j+1)/2 + i
+  // reference packing j*(j+1)/2 + i, molecule by molecule
   b->Q.assign(npairs, 0.0);
-  for (int p = 0; p < npairs; ++p) {
-    const int i = b->pi[p], j = b->pj[p];
-    b->Q[size_t(j) * (j + 1) / 2 + i] = Qp[p];
-  }
+  for (int p = 0; p < npairs; ++p) b->Q[packed_q_index(b, p)] = Qp[p];
   b->have_Q = true;
   if (Qout) std::memcpy(Qout, b->Q.data(), sizeof(double) * npairs);
   return LAI_OK;
@@ -623,7 +733,7 @@ extern "C" int lai_dev_schwarz(lai_basis* b, double* Qout) {
 
 extern "C" int lai_dev_set_schwarz(lai_basis* b, const double* Q) {
   if (!b || !Q) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_schwarz: invalid arguments");
-  b->Q.assign(Q, Q + size_t(b->ns) * (b->ns + 1) / 2);
+  b->Q.assign(Q, Q + size_t(b->mol_pair.back()));
   b->have_Q = true;
   return LAI_OK;
 }
@@ -643,168 +753,199 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
   const bool screen = thr > 0.0;
   if (screen && !b->have_Q)
     return fail(LAI_ERR_INVALID_STATE, "lai_dev_build_worklist: Schwarz bounds not computed");
-  const int ns = b->ns, npairs = ns * (ns + 1) / 2;
-  std::vector<double> Qp(npairs, 1.0);
-  if (b->have_Q)
-    for (int p = 0; p < npairs; ++p) {
-      const int i = b->pi[p], j = b->pj[p];
-      Qp[p] = b->Q[size_t(j) * (j + 1) / 2 + i];
-    }
-  const bool mixed = tau > 0.0 && b->have_Q && shell_dmax != nullptr;
-  const bool dscreen = screen && density_screen && shell_dmax != nullptr;
   if (screen && density_screen && !shell_dmax)
     return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_build_worklist: density screening needs shell_dmax");
-  auto dm = [&](int x, int y) { return shell_dmax[size_t(x) * ns + y]; };
-  const int nth = hw_threads();
-  // per-thread per-class buckets, per bra pair order kept by pass structure
-  struct Bucket {
-    std::vector<Quartet> q64[kNumClasses], q32[kNumClasses];
-    uint64_t sum = 0;
-    int64_t counts[kNumClasses] = {};
-    double cf64[kNumClasses] = {}, cf32[kNumClasses] = {};
-    double f64 = 0, f32 = 0;
-    int64_t neri = 0;
-  };
-  std::vector<Bucket> bk(nth);
-  // class F_prim table in the reference orientation (computed per quartet)
+  const int npairs = b->mol_pair.back();
+  std::vector<double> Qp(npairs, 1.0);
+  if (b->have_Q)
+    for (int p = 0; p < npairs; ++p) Qp[p] = b->Q[packed_q_index(b, p)];
+  const bool mixed = tau > 0.0 && b->have_Q && shell_dmax != nullptr;
+  const bool dscreen = screen && density_screen && shell_dmax != nullptr;
   const auto& Ls = launchers();
-  double fp_tab[kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
+  const int nm = b->n_mol;
+  constexpr int NPC = kNumPairClasses, NB = kMaxSegs;
+
+  // Ket lists: per (molecule, pair class, primitive-count bucket), the pairs in
+  // order of decreasing Schwarz bound.  Bucket k holds npp_sig in [2^k, 2^(k+1))
+  // (the last one everything above), so the lanes of a wavefront run similar
+  // primitive-loop trip counts.
+  auto bucket = [&](int p) {
+    const int n = b->pairs_h[p].npp_sig;
+    int k = 0;
+    while (k < NB - 1 && (2 << k) <= n) ++k;
+    return k;
+  };
+  const size_t nlists = size_t(nm) * NPC * NB;
+  std::vector<std::vector<int>> lists(nlists);
+  std::vector<int> pbk(npairs), ppos(npairs);
+  for (int p = 0; p < npairs; ++p) {
+    pbk[p] = bucket(p);
+    const int m = b->sh_mol[b->pi[p]];
+    lists[(size_t(m) * NPC + b->pair_cls[p]) * NB + pbk[p]].push_back(p);
+  }
+  std::vector<int> loff(nlists + 1, 0);
+  for (size_t l = 0; l < nlists; ++l) {
+    auto& v = lists[l];
+    std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return Qp[x] > Qp[y]; });
+    for (int k = 0; k < int(v.size()); ++k) ppos[v[k]] = k;
+    loff[l + 1] = loff[l] + int(v.size());
+  }
+  std::vector<int> kets_h(npairs);
+  for (size_t l = 0; l < nlists; ++l) std::copy(lists[l].begin(), lists[l].end(), kets_h.begin() + loff[l]);
+
+  // Per quartet class: for every bra pair (heaviest primitive bucket first,
+  // then decreasing bound) the segments of kets that pass the Schwarz test.
+  // Pairs of the same class count once: with the order (bucket, position), a
+  // ket is taken when it does not precede the bra.
+  struct Entry {
+    int bra;
+    JkSegs sg;
+  };
+  std::vector<std::vector<Entry>> E(kNumClasses);
+  std::vector<int64_t> etot(kNumClasses, 0);
+  parallel_for(kNumClasses, [&](int c, int) {
+    const int pb = cls_bra(c), pk = cls_ket(c);
+    int64_t tot = 0;
+    for (int m = 0; m < nm; ++m)
+      for (int bb = NB - 1; bb >= 0; --bb) {
+        const size_t lb = (size_t(m) * NPC + pb) * NB + bb;
+        for (int p : lists[lb]) {
+          Entry e{};
+          e.bra = p;
+          int acc = 0;
+          for (int kb = NB - 1; kb >= 0; --kb) {
+            const size_t lk = (size_t(m) * NPC + pk) * NB + kb;
+            const int len = loff[lk + 1] - loff[lk];
+            if (len == 0) continue;
+            int lo = 0;
+            if (pb == pk) {
+              if (kb < bb) continue;
+              if (kb == bb) lo = ppos[p];
+            }
+            int hi = len;
+            if (screen) {  // first position where Q_p Q_r >= thr fails
+              const int* v = kets_h.data() + loff[lk];
+              int a = 0, z = len;
+              const double qp = Qp[p];
+              while (a < z) {
+                const int mid = (a + z) / 2;
+                if (qp * Qp[v[mid]] >= thr)
+                  a = mid + 1;
+                else
+                  z = mid;
+              }
+              hi = a;
+            }
+            if (hi <= lo) continue;
+            e.sg.start[e.sg.n] = loff[lk] + lo;
+            e.sg.cum[e.sg.n] = acc;
+            acc += hi - lo;
+            ++e.sg.n;
+          }
+          if (acc == 0) continue;
+          e.sg.cum[e.sg.n] = acc;
+          tot += acc;
+          E[c].push_back(e);
+        }
+      }
+    etot[c] = tot;
+  }, 1);
+
+  // Chunks: register-shape classes take up to 2048 quartets per workgroup
+  // (8 per lane), LDS-shape classes up to 32; small classes get shorter chunks
+  // so that a launch still has a few workgroups per CU.
+  const int n_cu = b->ctx->n_cu;
+  std::vector<JkSegs> segs_h;
+  std::vector<std::vector<JkChunk>> C(kNumClasses);
+  for (int c = 0; c < kNumClasses; ++c) {
+    const size_t n = size_t(etot[c]);
+    const int cmax = Ls.info[c].regs ? int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 256, 2048))
+                                     : int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 1, 32));
+    for (const auto& e : E[c]) {
+      const int si = int(segs_h.size());
+      segs_h.push_back(e.sg);
+      const int len = e.sg.cum[e.sg.n];
+      const int pieces = (len + cmax - 1) / cmax;
+      const int per = (len + pieces - 1) / pieces;
+      for (int k0 = 0; k0 < len; k0 += per) C[c].push_back(JkChunk{e.bra, si, k0, std::min(len, k0 + per)});
+    }
+    std::vector<Entry>().swap(E[c]);
+  }
+
+  // shell-pair max|D| per molecule (density screen, FP32 split)
+  std::vector<double> sdm;
+  if (shell_dmax) sdm.assign(shell_dmax, shell_dmax + b->mol_dmoff.back());
+  else sdm.assign(1, 0.0);
+  hipError_t e;
+  if ((e = b->kets.assign(kets_h.data(), kets_h.size())) != hipSuccess ||
+      (e = b->segs.assign(segs_h.data(), segs_h.size())) != hipSuccess ||
+      (e = b->dQ.assign(Qp.data(), Qp.size())) != hipSuccess ||
+      (e = b->sdmax.assign(sdm.data(), sdm.size())) != hipSuccess)
+    return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
+  for (int c = 0; c < kNumClasses; ++c) {
+    if ((e = b->cls[c].chunks.assign(C[c].data(), C[c].size())) != hipSuccess)
+      return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
+    b->cls[c].nchunks = int64_t(C[c].size());
+  }
+  b->wl_dscreen = dscreen ? 1 : 0;
+  b->wl_mixed = mixed ? 1 : 0;
+  b->wl_thr = thr;
+  b->wl_tau = tau;
+
+  // statistics on the device, with the kernels' own selection
+  double fp_tab[4 * 4 * 4 * 4];
   for (int x = 0; x <= kMaxL; ++x)
     for (int y = 0; y <= kMaxL; ++y)
       for (int z = 0; z <= kMaxL; ++z)
-        for (int w = 0; w <= kMaxL; ++w) fp_tab[x][y][z][w] = fprim(x, y, z, w);
-  std::vector<int> npp(npairs), nsig(npairs);
-  for (int p = 0; p < npairs; ++p) npp[p] = b->pairs_h[p].npp, nsig[p] = b->pairs_h[p].npp_sig;
-  parallel_for(npairs, [&](int p, int t) {
-    Bucket& B = bk[t];
-    const double qp = Qp[p];
-    const int i = b->pi[p], j = b->pj[p];
-    const int cp = b->pair_cls[p];
-    for (int r = p; r < npairs; ++r) {
-      const double prod = qp * Qp[r];
-      if (screen && !(prod >= thr)) continue;
-      const int k = b->pi[r], l = b->pj[r];
-      double dk = 0.0;
-      if (dscreen || mixed)
-        dk = std::max(std::max(dm(i, k), dm(i, l)), std::max(dm(j, k), dm(j, l)));
-      if (dscreen && !(dk * prod >= thr)) continue;
-      const int cr = b->pair_cls[r];
-      Quartet q;
-      int id;
-      if (cp >= cr) {
-        q.bra = p;
-        q.ket = r;
-        id = cp * (cp + 1) / 2 + cr;
-      } else {
-        q.bra = r;
-        q.ket = p;
-        id = cr * (cr + 1) / 2 + cp;
-      }
-      const bool f32 = mixed && prod * std::max(dk, std::max(dm(i, j), dm(k, l))) < tau;
-      (f32 ? B.q32[id] : B.q64[id]).push_back(q);
-      B.sum += lai_quartet_mix64(i, j, k, l);
-      B.counts[id]++;
-      const double fl = double(npp[p]) * npp[r] * fp_tab[b->L[i]][b->L[j]][b->L[k]][b->L[l]];
-      (f32 ? B.f32 : B.f64) += fl;
-      // executed work: the primitive pairs the J/K kernels actually visit
-      const double fx = double(nsig[p]) * nsig[r] * fp_tab[b->L[i]][b->L[j]][b->L[k]][b->L[l]];
-      (f32 ? B.cf32[id] : B.cf64[id]) += fx;
-      B.neri += int64_t(ncart(b->L[i])) * ncart(b->L[j]) * ncart(b->L[k]) * ncart(b->L[l]);
-    }
-  });
-  uint64_t sum = 0;
+        for (int w = 0; w <= kMaxL; ++w) fp_tab[((x * 4 + y) * 4 + z) * 4 + w] = fprim(x, y, z, w);
+  DevBuf<double> dfp, dfl;
+  DevBuf<unsigned long long> dcnt, dsum;
+  if ((e = dfp.assign(fp_tab, 256)) != hipSuccess || (e = dfl.alloc(4 * kNumClasses)) != hipSuccess ||
+      (e = dcnt.alloc(3 * kNumClasses)) != hipSuccess || (e = dsum.alloc(1)) != hipSuccess)
+    return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
+  LAI_HIP(hipMemset(dfl.p, 0, sizeof(double) * 4 * kNumClasses));
+  LAI_HIP(hipMemset(dcnt.p, 0, sizeof(unsigned long long) * 3 * kNumClasses));
+  LAI_HIP(hipMemset(dsum.p, 0, sizeof(unsigned long long)));
+  WlStatsArgs sa{};
+  sa.segs = b->segs.p;
+  sa.kets = b->kets.p;
+  sa.pairs = b->pairs.p;
+  sa.sel = make_sel(b);
+  sa.L = b->dL.p;
+  sa.fp = dfp.p;
+  sa.counts = dcnt.p;
+  sa.flops = dfl.p;
+  sa.checksum = dsum.p;
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (b->cls[c].nchunks == 0) continue;
+    sa.cls = c;
+    sa.chunks = b->cls[c].chunks.p;
+    sa.nchunks = int(b->cls[c].nchunks);
+    const int blocks = int(std::min<int64_t>(b->cls[c].nchunks, int64_t(n_cu) * 16));
+    hipLaunchKernelGGL(worklist_stats_kernel, dim3(blocks), dim3(256), 0, nullptr, sa);
+  }
+  LAI_HIP(hipGetLastError());
+  std::vector<unsigned long long> cnt(3 * kNumClasses);
+  std::vector<double> fl(4 * kNumClasses);
+  unsigned long long sum = 0;
+  LAI_HIP(hipMemcpy(cnt.data(), dcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost));
+  LAI_HIP(hipMemcpy(fl.data(), dfl.p, sizeof(double) * fl.size(), hipMemcpyDeviceToHost));
+  LAI_HIP(hipMemcpy(&sum, dsum.p, sizeof(sum), hipMemcpyDeviceToHost));
   int64_t total = 0, total32 = 0;
-  for (int c = 0; c < kNumClasses; ++c) b->counts[c] = 0, b->cls_f64[c] = 0, b->cls_f32[c] = 0;
-  b->flops64 = b->flops32 = 0;
+  b->flops64 = b->flops32 = 0.0;
   b->n_eri = 0;
-  for (auto& B : bk) {
-    sum += B.sum;
-    b->flops64 += B.f64;
-    b->flops32 += B.f32;
-    b->n_eri += B.neri;
-    for (int c = 0; c < kNumClasses; ++c) {
-      b->counts[c] += B.counts[c];
-      b->cls_f64[c] += B.cf64[c];
-      b->cls_f32[c] += B.cf32[c];
-    }
-  }
-  // gather each class from the per-thread buckets
-  std::vector<std::vector<Quartet>> L64(kNumClasses), L32(kNumClasses);
   for (int c = 0; c < kNumClasses; ++c) {
-    size_t n64 = 0, n32 = 0;
-    for (auto& B : bk) {
-      n64 += B.q64[c].size();
-      n32 += B.q32[c].size();
-    }
-    L64[c].reserve(n64);
-    L32[c].reserve(n32);
-    for (auto& B : bk) {
-      L64[c].insert(L64[c].end(), B.q64[c].begin(), B.q64[c].end());
-      L32[c].insert(L32[c].end(), B.q32[c].begin(), B.q32[c].end());
-      std::vector<Quartet>().swap(B.q64[c]);
-      std::vector<Quartet>().swap(B.q32[c]);
-    }
+    b->counts[c] = int64_t(cnt[3 * c]);
+    total += int64_t(cnt[3 * c]);
+    total32 += int64_t(cnt[3 * c + 1]);
+    b->n_eri += int64_t(cnt[3 * c + 2]);
+    b->flops64 += fl[4 * c];
+    b->flops32 += fl[4 * c + 1];
+    b->cls_f64[c] = fl[4 * c + 2];
+    b->cls_f32[c] = fl[4 * c + 3];
   }
-  // Group every list by bra pair (bras with more significant primitive pairs
-  // first, each bra's kets likewise, so the lanes of a wavefront run equal
-  // primitive-loop trip counts), then cut each bra's run into chunks of at
-  // most chunk_max quartets: one workgroup per chunk in the J/K kernels.
-  auto anchor = [&](std::vector<Quartet>& v, std::vector<JkChunk>& ch, int chunk_max) {
-    ch.clear();
-    if (v.empty()) return;
-    std::vector<Quartet> tmp(v.size());
-    auto pass = [&](int nkeys, auto key) {  // stable counting sort
-      std::vector<uint32_t> cnt(size_t(nkeys) + 1, 0);
-      for (const auto& q : v) cnt[key(q) + 1]++;
-      for (int x = 0; x < nkeys; ++x) cnt[x + 1] += cnt[x];
-      for (const auto& q : v) tmp[cnt[key(q)]++] = q;
-      v.swap(tmp);
-    };
-    pass(256, [&](const Quartet& q) { return 255 - std::min(nsig[q.ket], 255); });
-    pass(npairs, [&](const Quartet& q) { return q.bra; });
-    pass(256, [&](const Quartet& q) { return 255 - std::min(nsig[q.bra], 255); });
-    size_t s0 = 0;
-    while (s0 < v.size()) {
-      size_t s1 = s0 + 1;
-      while (s1 < v.size() && v[s1].bra == v[s0].bra) ++s1;
-      const int len = int(s1 - s0);
-      const int m = (len + chunk_max - 1) / chunk_max;
-      const int per = (len + m - 1) / m;
-      for (int st = 0; st < len; st += per)
-        ch.push_back(JkChunk{v[s0].bra, int(s0) + st, std::min(per, len - st), 0});
-      s0 = s1;
-    }
-  };
-  std::vector<std::vector<JkChunk>> C64(kNumClasses), C32(kNumClasses);
-  // Register-shape classes: 256 lanes per chunk, up to 8 quartets per lane;
-  // LDS-shape classes: one quartet at a time per workgroup, up to 32.  Small
-  // lists get shorter chunks so that a launch still has a few workgroups per
-  // CU.
-  const int n_cu = b->ctx->n_cu;
-  auto chunk_max = [&](int c, size_t n) {
-    if (Ls.info[c].regs) return int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 256, 2048));
-    return int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 1, 32));
-  };
-  parallel_for(kNumClasses, [&](int c, int) {
-    anchor(L64[c], C64[c], chunk_max(c, L64[c].size()));
-    anchor(L32[c], C32[c], chunk_max(c, L32[c].size()));
-  }, 1);
-  for (int c = 0; c < kNumClasses; ++c) {
-    hipError_t e;
-    if ((e = b->cls[c].q64.assign(L64[c].data(), L64[c].size())) != hipSuccess ||
-        (e = b->cls[c].q32.assign(L32[c].data(), L32[c].size())) != hipSuccess ||
-        (e = b->cls[c].c64.assign(C64[c].data(), C64[c].size())) != hipSuccess ||
-        (e = b->cls[c].c32.assign(C32[c].data(), C32[c].size())) != hipSuccess)
-      return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
-    b->cls[c].n64 = int64_t(L64[c].size());
-    b->cls[c].n32 = int64_t(L32[c].size());
-    b->cls[c].nc64 = int64_t(C64[c].size());
-    b->cls[c].nc32 = int64_t(C32[c].size());
-    total += b->cls[c].n64 + b->cls[c].n32;
-    total32 += b->cls[c].n32;
-    std::vector<Quartet>().swap(L64[c]);
-    std::vector<Quartet>().swap(L32[c]);
-  }
+  b->n_fp32 = total32;
+  b->checksum = sum;
   b->have_worklist = true;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
@@ -839,41 +980,24 @@ extern "C" int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, doubl
 static void launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
   const auto& Ls = launchers();
   const int max_blocks = b->ctx->n_cu * 16;
+  a.segs = b->segs.p;
+  a.kets = b->kets.p;
+  a.sel = make_sel(b);
+  if (flags != LAI_JK_MIXED) a.sel.mixed = 0;
   for (int c = kNumClasses - 1; c >= 0; --c) {
     const ClassList& C = b->cls[c];
-    if (ev) (void)hipEventRecord(ev[4 * c], st);
-    if (C.nc64 > 0) {
-      a.q = C.q64.p;
-      a.nq = int(C.n64);
-      a.chunks = C.c64.p;
-      a.nchunks = int(C.nc64);
-      Ls.f64[c](a, st, max_blocks);
-    }
-    if (ev) {
-      (void)hipEventRecord(ev[4 * c + 1], st);
-      (void)hipEventRecord(ev[4 * c + 2], st);
-    }
-    if (C.nc32 > 0) {
-      a.q = C.q32.p;
-      a.nq = int(C.n32);
-      a.chunks = C.c32.p;
-      a.nchunks = int(C.nc32);
+    if (ev) (void)hipEventRecord(ev[2 * c], st);
+    if (C.nchunks > 0) {
+      a.chunks = C.chunks.p;
+      a.nchunks = int(C.nchunks);
       (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
     }
-    if (ev) (void)hipEventRecord(ev[4 * c + 3], st);
+    if (ev) (void)hipEventRecord(ev[2 * c + 1], st);
   }
 }
 
-extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
-  if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: invalid arguments");
-  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk: worklist not built");
-  if (flags != LAI_JK_FP64 && flags != LAI_JK_MIXED)
-    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: unknown flags");
-  hipStream_t st = hipStream_t(stream);
-  const size_t nn = size_t(b->nbf) * b->nbf;
-  int rc;
-  if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
-  if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
+// arguments of a J/K build into the scratch Jt, Kt
+static EriArgs jk_args(lai_basis* b, const double* d_D) {
   EriArgs a{};
   a.pairs = b->pairs.p;
   a.pp = b->pp.p;
@@ -884,14 +1008,33 @@ extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* 
   a.D = d_D;
   a.Jt = b->Jt.p;
   a.Kt = b->Kt.p;
-  a.nbf = b->nbf;
+  a.nbf = b->max_nbf;
   a.mode = 0;
-  launch_jk_classes(b, a, flags, st, nullptr);
-  LAI_HIP(hipGetLastError());
-  const unsigned blocks = unsigned(std::min<size_t>((nn + 255) / 256, 8192));
-  hipLaunchKernelGGL(symmetrize_add_kernel, dim3(blocks), dim3(256), 0, st, b->Jt.p, b->Kt.p, d_J, d_K, b->nbf);
+  return a;
+}
+
+static int symmetrize(lai_basis* b, double* d_J, double* d_K, hipStream_t st) {
+  const size_t nn = size_t(b->max_nbf) * b->max_nbf;
+  const unsigned blocks = unsigned(std::max<size_t>(1, std::min<size_t>((nn + 255) / 256, 2048)));
+  hipLaunchKernelGGL(symmetrize_add_batch_kernel, dim3(blocks, unsigned(b->n_mol)), dim3(256), 0, st, b->Jt.p,
+                     b->Kt.p, d_J, d_K, b->dmoff.p, b->dmnbf.p);
   LAI_HIP(hipGetLastError());
   return LAI_OK;
+}
+
+extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
+  if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: invalid arguments");
+  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk: worklist not built");
+  if (flags != LAI_JK_FP64 && flags != LAI_JK_MIXED)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: unknown flags");
+  hipStream_t st = hipStream_t(stream);
+  const size_t nn = size_t(b->mat_size);
+  int rc;
+  if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
+  if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
+  launch_jk_classes(b, jk_args(b, d_D), flags, st, nullptr);
+  LAI_HIP(hipGetLastError());
+  return symmetrize(b, d_J, d_K, st);
 }
 
 extern "C" int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags,
@@ -932,6 +1075,8 @@ extern "C" int lai_dev_jk_graph_launch(lai_basis* b, void* stream) {
 // ---------------------------------------------------------------------------
 extern "C" int lai_dev_onee(lai_basis* b, int kind, int nch, const double* Z, const double* xyz, double* d_out,
                             void* stream) {
+  if (b && b->n_mol != 1)
+    return fail(LAI_ERR_UNSUPPORTED, "lai_dev_onee: one-electron matrices need a single-molecule basis handle");
   if (!b || !d_out || kind < 0 || kind > 2 || (kind == 2 && nch > 0 && (!Z || !xyz)))
     return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_onee: invalid arguments");
   DevBuf<double> dZ, dxyz;
@@ -1040,21 +1185,22 @@ extern "C" int lai_dev_jk_grad(lai_basis* b, const double* d_D, double xscale, d
   a.ppab = b->ppab.p;
   a.rys = b->ctx->rys.p;
   a.D = d_D;
-  a.nbf = b->nbf;
+  a.nbf = b->max_nbf;
   a.xscale = xscale;
   a.shell_atom = b->shell_atom.p;
   a.grad = d_grad;
   a.n_atoms = b->n_atoms;
+  a.ppih = b->ppih.p;
+  a.segs = b->segs.p;
+  a.kets = b->kets.p;
+  a.sel = make_sel(b);
+  a.sel.mixed = 0;  // the gradient runs every quartet in FP64
   const int max_blocks = b->ctx->n_cu * 16;
   for (int c = 0; c < kNumGradClasses; ++c) {
-    for (int part = 0; part < 2; ++part) {
-      const auto& lst = part == 0 ? b->cls[c].q64 : b->cls[c].q32;
-      const int64_t n = part == 0 ? b->cls[c].n64 : b->cls[c].n32;
-      if (n <= 0) continue;
-      a.q = lst.p;
-      a.nq = int(n);
-      G.f[c](a, st, max_blocks);
-    }
+    if (b->cls[c].nchunks <= 0) continue;
+    a.chunks = b->cls[c].chunks.p;
+    a.nchunks = int(b->cls[c].nchunks);
+    G.f[c](a, st, max_blocks);
   }
   LAI_HIP(hipGetLastError());
   return LAI_OK;
@@ -1075,35 +1221,23 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
   if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_profile: invalid arguments");
   if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_profile: worklist not built");
   hipStream_t st = hipStream_t(stream);
-  const size_t nn = size_t(b->nbf) * b->nbf;
+  const size_t nn = size_t(b->mat_size);
   int rc;
   if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
   if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
-  EriArgs a{};
-  a.pairs = b->pairs.p;
-  a.pp = b->pp.p;
-  a.rys = b->ctx->rys.p;
-  a.ppb = b->ppb.p;
-  a.ppih = b->ppih.p;
-  a.pcut32 = b->pcut32;
-  a.D = d_D;
-  a.Jt = b->Jt.p;
-  a.Kt = b->Kt.p;
-  a.nbf = b->nbf;
-  a.mode = 0;
-  std::vector<hipEvent_t> ev(4 * kNumClasses, nullptr);
+  std::vector<hipEvent_t> ev(2 * kNumClasses, nullptr);
   for (auto& e : ev) LAI_HIP(hipEventCreate(&e));
-  launch_jk_classes(b, a, flags, st, ev.data());
+  launch_jk_classes(b, jk_args(b, d_D), flags, st, ev.data());
   LAI_HIP(hipGetLastError());
-  const unsigned blocks = unsigned(std::min<size_t>((nn + 255) / 256, 8192));
-  hipLaunchKernelGGL(symmetrize_add_kernel, dim3(blocks), dim3(256), 0, st, b->Jt.p, b->Kt.p, d_J, d_K, b->nbf);
+  if ((rc = symmetrize(b, d_J, d_K, st)) != LAI_OK) return rc;
   LAI_HIP(hipStreamSynchronize(st));
+  // one launch per class serves both precisions: its time is reported as
+  // FP64 for LAI_JK_FP64 and as FP32 (mixed) for LAI_JK_MIXED
   for (int c = 0; c < kNumClasses; ++c) {
-    float t0 = 0.f, t1 = 0.f;
-    LAI_HIP(hipEventElapsedTime(&t0, ev[4 * c], ev[4 * c + 1]));
-    LAI_HIP(hipEventElapsedTime(&t1, ev[4 * c + 2], ev[4 * c + 3]));
-    if (ms_fp64) ms_fp64[c] = t0;
-    if (ms_fp32) ms_fp32[c] = t1;
+    float t = 0.f;
+    LAI_HIP(hipEventElapsedTime(&t, ev[2 * c], ev[2 * c + 1]));
+    if (ms_fp64) ms_fp64[c] = flags == LAI_JK_MIXED ? 0.0 : t;
+    if (ms_fp32) ms_fp32[c] = flags == LAI_JK_MIXED ? t : 0.0;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return LAI_OK;

@@ -68,15 +68,20 @@ __host__ __device__ constexpr int cls_ket(int id) { return id - cls_bra(id) * (c
 // HBM layout
 // ---------------------------------------------------------------------------
 // One significant shell pair in "class order": shell a has La >= Lb of shell
-// b.  The same record serves as bra or ket.
+// b.  The same record serves as bra or ket.  Both shells belong to one
+// molecule of the basis; fa, fb are function offsets inside that molecule and
+// (moff, mn) locate the molecule's mn x mn block of D, J and K.
 struct PairRec {
   int sa, sb;         // shell indices (class order)
-  int fa, fb;         // first basis function of each shell
+  int fa, fb;         // first basis function of each shell, within the molecule
   int pp_off;         // first primitive pair in the PrimPair array
   int npp;            // number of primitive pairs (zero coefficients dropped)
   int same;           // 1 when sa == sb
   int npp_sig;        // leading primitive pairs used by the J/K and gradient kernels
                       // (the rest are below the primitive screening bound)
+  int mn;             // basis functions of the molecule
+  int mol;            // molecule index
+  long long moff;     // offset of the molecule's matrices
   double ABx, ABy, ABz;  // A - B (horizontal transfer)
   double bsum;           // sum of the primitive-pair bounds |Kc| zeta^(-1/4)
 };
@@ -96,10 +101,20 @@ struct Quartet {
   int bra, ket;
 };
 
-// A run of quartets of one class that share the bra pair: Quartet entries
-// [start, start + count) of the class list, all with bra == bra.
+// Worklist of a quartet class.  For one bra pair, the kets that pass the
+// Schwarz test form up to kMaxSegs contiguous runs ("segments") of the ket
+// array, one per primitive-count bucket of the ket pair class: segment s
+// holds kets[start[s] .. start[s] + cum[s+1] - cum[s]).  A chunk is the part
+// [k0, k1) of that concatenated run handled by one workgroup.
+inline constexpr int kMaxSegs = 8;
+struct JkSegs {
+  int n;
+  int start[kMaxSegs];
+  int cum[kMaxSegs + 1];
+  int pad;
+};
 struct JkChunk {
-  int bra, start, count, pad;
+  int bra, seg, k0, k1;
 };
 
 // ---------------------------------------------------------------------------

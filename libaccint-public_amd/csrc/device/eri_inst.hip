@@ -17,7 +17,9 @@ namespace {
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kLdsBudgetB = 160 * 1024;
 
-template <int ID, typename R>
+// MIXED = false: every quartet in FP64 (and the raw-ERI mode); true: the
+// quartets the worklist marks for FP32 run the recursion in float.
+template <int ID, bool MIXED>
 void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
   constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
@@ -25,19 +27,19 @@ void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   EriArgs a = a0;
   if (a.mode == 1) {  // raw ERIs, one launch over the quartet list
-    if (a.nq <= 0) return;
+    if (MIXED || a.nq <= 0) return;
     if constexpr (P::regs) {
       int blocks = (a.nq + 127) / 128;
       if (blocks > max_blocks) blocks = max_blocks;
       dim3 grid(blocks, P::NCH);
-      hipLaunchKernelGGL((eri_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(128), 0, s, a);
+      hipLaunchKernelGGL((eri_kernel_regs<La, Lb, Lc, Ld, double, P::CHA>), grid, dim3(128), 0, s, a);
     } else {
       int blocks = a.nq < 8 * max_blocks ? a.nq : 8 * max_blocks;
-      hipLaunchKernelGGL((eri_kernel_lds<La, Lb, Lc, Ld, R, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
+      hipLaunchKernelGGL((eri_kernel_lds<La, Lb, Lc, Ld, double, P::NT>), dim3(blocks), dim3(P::NT), 0, s, a);
     }
     return;
   }
-  // J/K digestion over chunks of quartets sharing the bra pair
+  // J/K digestion over the bra-anchored chunks
   if (a.nchunks <= 0) return;
   const size_t nbf = size_t(a.nbf);
   if constexpr (P::regs) {
@@ -45,13 +47,13 @@ void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
     a.klds = dyn <= kLdsBudget ? 1 : 0;
     int blocks = a.nchunks < max_blocks ? a.nchunks : max_blocks;
     dim3 grid(blocks, P::NCH);
-    hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, R, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
+    hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
   } else {
-    using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-    constexpr size_t stat = size_t(Sh::template npq<P::NT>()) * (Sh::TS + Sh::HSS) * sizeof(R);
+    using Sh = LdsShape<La, Lb, Lc, Ld, double>;
+    constexpr size_t stat = size_t(Sh::template npq<P::NT>()) * (Sh::TS + Sh::HSS) * sizeof(double);
     const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
     a.klds = stat + dyn <= kLdsBudgetB ? 1 : 0;
-    auto kern = jk_kernel_lds<La, Lb, Lc, Ld, R, P::NT>;
+    auto kern = jk_kernel_lds<La, Lb, Lc, Ld, MIXED, P::NT>;
     if (a.klds && stat + dyn > 64 * 1024 &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             int(dyn)) != hipSuccess)
@@ -67,15 +69,15 @@ void fill(ClassLauncher* f64, ClassLauncher* f32, ClassInfo* info) {
     constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
     constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
     using P = ClassPolicy<La, Lb, Lc, Ld>;
-    f64[ID] = &launch_class<ID, double>;
-    f32[ID] = &launch_class<ID, float>;
+    f64[ID] = &launch_class<ID, false>;
+    f32[ID] = &launch_class<ID, true>;
     info[ID].La = La;
     info[ID].Lb = Lb;
     info[ID].Lc = Lc;
     info[ID].Ld = Ld;
     info[ID].regs = P::regs ? 1 : 0;
     info[ID].nchunk = P::regs ? P::NCH : 0;
-    info[ID].nt = P::regs ? 128 : P::NT;
+    info[ID].nt = P::regs ? 256 : P::NT;
   }
   if constexpr (ID + 1 < kNumClasses) fill<ID + 1>(f64, f32, info);
 }

@@ -38,6 +38,44 @@
 
 namespace lai {
 
+// Per-quartet selection of the J/K and gradient kernels: the density screen
+// and the FP32 split, decided from the Schwarz bounds of the two pairs and the
+// shell-pair max|D| of the quartet's molecule.
+struct QuartetSel {
+  const double* __restrict__ Qpair;      // Schwarz bound of each pair
+  const double* __restrict__ sdmax;      // per molecule: ns_m x ns_m shell-pair max|D|
+  const long long* __restrict__ dm_off;  // per molecule: offset of its block
+  const int* __restrict__ dm_ns;         // per molecule: shells
+  const int* __restrict__ sh_loc;        // per shell: index within its molecule
+  int dscreen;                           // require Dk Q_ij Q_kl >= thr
+  double thr;
+  int mixed;                             // FP32 recursion where Q_ij Q_kl max|D| < tau
+  double tau;
+};
+
+// 0: removed by the density screen, 1: FP64, 2: FP32 recursion.  The same
+// arithmetic as the host worklist rules (exchange-type Dk over (ik, il, jk,
+// jl); the FP32 test adds D_ij and D_kl).
+__device__ __forceinline__ int quartet_select(const QuartetSel& s, int bp, const PairRec& bra, int kp,
+                                              const PairRec& ket) {
+  if (!s.dscreen && !s.mixed) return 1;
+  const double prod = s.Qpair[bp] * s.Qpair[kp];
+  const double* dm = s.sdmax + s.dm_off[bra.mol];
+  const int ns = s.dm_ns[bra.mol];
+  const int i = s.sh_loc[bra.sa], j = s.sh_loc[bra.sb], k = s.sh_loc[ket.sa], l = s.sh_loc[ket.sb];
+  const double dk = fmax(fmax(dm[i * ns + k], dm[i * ns + l]), fmax(dm[j * ns + k], dm[j * ns + l]));
+  if (s.dscreen && !(dk * prod >= s.thr)) return 0;
+  if (s.mixed && prod * fmax(dk, fmax(dm[i * ns + j], dm[k * ns + l])) < s.tau) return 2;
+  return 1;
+}
+
+// Ket pair of position k of a chunk's run; seg walks forward with k.
+__device__ __forceinline__ int chunk_ket(const int* __restrict__ kets, const JkSegs* __restrict__ sg, int k,
+                                         int& seg) {
+  while (seg + 1 < sg->n && k >= sg->cum[seg + 1]) ++seg;
+  return kets[sg->start[seg] + (k - sg->cum[seg])];
+}
+
 struct EriArgs {
   const Quartet* __restrict__ q;
   int nq;
@@ -50,20 +88,23 @@ struct EriArgs {
   const float* __restrict__ ppb;
   double pcut32;
   const double* __restrict__ ppih;  // 1 / (2 zeta) of each primitive pair
-  // mode 0: digestion
+  // mode 0: digestion into D, Jt, Kt (molecule blocks at PairRec::moff)
   const double* __restrict__ D;
   double* Jt;
   double* Kt;
-  int nbf;
+  int nbf;  // largest molecule (LDS rows)
   int mode;
   // mode 1: raw ERI output, [q][block] in the original order given by perm
   double* out;
   const int* __restrict__ perm;        // per quartet: packed original position of class positions
   const long long* __restrict__ obase;  // per quartet: offset of its block in out
-  // mode 0, anchored digestion: runs of quartets sharing the bra pair
+  // mode 0 worklist: bra-anchored chunks over ket segments
   const JkChunk* __restrict__ chunks;
   int nchunks;
+  const JkSegs* __restrict__ segs;
+  const int* __restrict__ kets;
   int klds;  // 1: the bra rows of Kt are accumulated in LDS (dynamic shared memory)
+  QuartetSel sel;
 };
 
 // --------------------------------------------------------------------------
@@ -251,109 +292,23 @@ __device__ __forceinline__ void quartet_regs(const EriArgs& g, int qi) {
   const PairRec bra = g.pairs[q.bra];
   const PairRec ket = g.pairs[q.ket];
   double gv[NA][nb][nc][nd];
-  quartet_block_regs<La, Lb, Lc, Ld, R, A0, A1>(g, bra, ket, g.mode == 1, gv);
+  quartet_block_regs<La, Lb, Lc, Ld, R, A0, A1>(g, bra, ket, true, gv);
 
-  const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
-  if (g.mode == 1) {
-    constexpr int na = ncart(La);
-    const int perm = g.perm[qi];
-    const int n4[4] = {na, nb, nc, nd};
-    double* out = g.out + g.obase[qi];
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-      for (int b = 0; b < nb; ++b)
-#pragma unroll
-        for (int c = 0; c < nc; ++c)
-#pragma unroll
-          for (int d = 0; d < nd; ++d) {
-            const int idx[4] = {A0 + a, b, c, d};
-            out[orig_offset(perm, idx, n4)] = gv[a][b][c][d];
-          }
-    return;
-  }
-
-  const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
-  const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
-  const double fK = bk ? 1.0 : 2.0;
-  const size_t n = size_t(g.nbf);
-  const double* __restrict__ D = g.D;
-  // Jt[ab] += fJ sum_cd g D[cd]
+  constexpr int na = ncart(La);
+  const int perm = g.perm[qi];
+  const int n4[4] = {na, nb, nc, nd};
+  double* out = g.out + g.obase[qi];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
-    for (int b = 0; b < nb; ++b) {
-      double s = 0.0;
+    for (int b = 0; b < nb; ++b)
 #pragma unroll
       for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[(fc + c) * n + fd + d];
-      atomic_add(&g.Jt[(fa + A0 + a) * n + fb + b], fJ * s);
-    }
-  if (!bk) {
-#pragma unroll
-    for (int c = 0; c < nc; ++c)
-#pragma unroll
-      for (int d = 0; d < nd; ++d) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < NA; ++a)
-#pragma unroll
-          for (int b = 0; b < nb; ++b) s += gv[a][b][c][d] * D[(fa + A0 + a) * n + fb + b];
-        atomic_add(&g.Jt[(fc + c) * n + fd + d], fJ * s);
-      }
-  }
-  // Kt[ac] += fK sum_bd g D[bd]
-#pragma unroll
-  for (int a = 0; a < NA; ++a)
-#pragma unroll
-    for (int c = 0; c < nc; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int b = 0; b < nb; ++b)
-#pragma unroll
-        for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[(fb + b) * n + fd + d];
-      atomic_add(&g.Kt[(fa + A0 + a) * n + fc + c], fK * s);
-    }
-  if (!kl) {  // Kt[ad] += fK sum_bc g D[bc]
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-      for (int d = 0; d < nd; ++d) {
-        double s = 0.0;
-#pragma unroll
-        for (int b = 0; b < nb; ++b)
-#pragma unroll
-          for (int c = 0; c < nc; ++c) s += gv[a][b][c][d] * D[(fb + b) * n + fc + c];
-        atomic_add(&g.Kt[(fa + A0 + a) * n + fd + d], fK * s);
-      }
-  }
-  if (!ij) {  // Kt[bc] += fK sum_ad g D[ad]
-#pragma unroll
-    for (int b = 0; b < nb; ++b)
-#pragma unroll
-      for (int c = 0; c < nc; ++c) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < NA; ++a)
-#pragma unroll
-          for (int d = 0; d < nd; ++d) s += gv[a][b][c][d] * D[(fa + A0 + a) * n + fd + d];
-        atomic_add(&g.Kt[(fb + b) * n + fc + c], fK * s);
-      }
-  }
-  if (!ij && !kl) {  // Kt[bd] += fK sum_ac g D[ac]
-#pragma unroll
-    for (int b = 0; b < nb; ++b)
-#pragma unroll
-      for (int d = 0; d < nd; ++d) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < NA; ++a)
-#pragma unroll
-          for (int c = 0; c < nc; ++c) s += gv[a][b][c][d] * D[(fa + A0 + a) * n + fc + c];
-        atomic_add(&g.Kt[(fb + b) * n + fd + d], fK * s);
-      }
-  }
+        for (int d = 0; d < nd; ++d) {
+          const int idx[4] = {A0 + a, b, c, d};
+          out[orig_offset(perm, idx, n4)] = gv[a][b][c][d];
+        }
 }
 
 template <int La, int Lb, int Lc, int Ld, typename R, int CHA, int C>
@@ -615,7 +570,7 @@ __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec&
   }
 }
 
-// Kernel B for raw ERIs (mode 1) and the direct-atomic J/K digestion.
+// Kernel B for raw ERIs (mode 1).
 template <int La, int Lb, int Lc, int Ld, typename R, int NT>
 __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
@@ -634,14 +589,8 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
     const PairRec bra = g.pairs[q.bra];
     const PairRec ket = g.pairs[q.ket];
     R acc[NJ];
-    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, g.mode == 1, tab, hs, ofs, acc);
-
-    const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
-    const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
-    const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
-    const double fK = bk ? 1.0 : 2.0;
-    const size_t n = size_t(g.nbf);
-    const int perm = g.mode == 1 ? g.perm[qi] : 0;
+    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, true, tab, hs, ofs, acc);
+    const int perm = g.perm[qi];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       const int o = tid + NT * j;
@@ -649,46 +598,38 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
       const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
       const double v = double(acc[j]) * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) *
                        norm_corr(Ld, d);
-      if (g.mode == 1) {
-        const int idx[4] = {a, b, c, d};
-        const int n4[4] = {na, nb, nc, nd};
-        g.out[g.obase[qi] + orig_offset(perm, idx, n4)] = v;
-        continue;
-      }
-      const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
-      const double* __restrict__ D = g.D;
-      atomic_add(&g.Jt[mu * n + nu], fJ * v * D[la * n + si]);
-      if (!bk) atomic_add(&g.Jt[la * n + si], fJ * v * D[mu * n + nu]);
-      atomic_add(&g.Kt[mu * n + la], fK * v * D[nu * n + si]);
-      if (!kl) atomic_add(&g.Kt[mu * n + si], fK * v * D[nu * n + la]);
-      if (!ij) atomic_add(&g.Kt[nu * n + la], fK * v * D[mu * n + si]);
-      if (!ij && !kl) atomic_add(&g.Kt[nu * n + si], fK * v * D[mu * n + la]);
+      const int idx[4] = {a, b, c, d};
+      const int n4[4] = {na, nb, nc, nd};
+      g.out[g.obase[qi] + orig_offset(perm, idx, n4)] = v;
     }
   }
 }
 
 // --------------------------------------------------------------------------
-// Anchored J/K digestion (mode 0).  The quartets of a class are grouped by
-// bra pair (A, B) and a workgroup takes one chunk of quartets with the same
-// bra.  Every exchange update of such a quartet lands in a row of A or B
-// (Kt[ac], Kt[ad], Kt[bc], Kt[bd]), so the workgroup accumulates those rows
-// (rows x nbf doubles) in LDS and adds them to Kt once per chunk; the bra
-// Coulomb block Jt[ab] is summed over the chunk as well.  Only the ket-side
-// Coulomb update Jt[cd] goes to HBM per quartet.  With klds == 0 (rows too
-// long for LDS) the exchange updates go straight to HBM atomics.
+// Anchored J/K digestion (mode 0).  A workgroup takes one chunk: one bra pair
+// (A, B) and a run of its kets.  Every exchange update of such a quartet lands
+// in a row of A or B (Kt[ac], Kt[ad], Kt[bc], Kt[bd]), so the workgroup
+// accumulates those rows (rows x mn doubles) in LDS and adds them to Kt once
+// per chunk; the bra Coulomb block Jt[ab] is summed over the chunk as well.
+// Only the ket-side Coulomb update Jt[cd] goes to HBM per quartet.  With
+// klds == 0 (rows too long for LDS) the exchange updates go straight to HBM.
+// MIXED: quartets selected for FP32 run the recursion in float.
 // --------------------------------------------------------------------------
-template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int A0, int A1>
 __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow) {
   constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NA = A1 - A0;
   constexpr int ROWS = NA + nb;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int n = g.nbf;
   const bool lds = g.klds != 0;
   const PairRec bra = g.pairs[ch.bra];
+  const int n = bra.mn;
+  const double* __restrict__ D = g.D + bra.moff;
+  double* Jt = g.Jt + bra.moff;
+  double* Kt = g.Kt + bra.moff;
   const int fa = bra.fa + A0, fb = bra.fb;
   const bool ij = bra.same;
-  const double* __restrict__ D = g.D;
+  const JkSegs* sg = g.segs + ch.seg;
   if (lds) {
     for (int e = tid; e < ROWS * n; e += nt) krow[e] = 0.0;
     __syncthreads();
@@ -698,7 +639,7 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
     if (lds)
       lds_add(&krow[row * n + col], v);
     else
-      atomic_add(&g.Kt[size_t(row < NA ? fa + row : fb + row - NA) * n + col], v);
+      atomic_add(&Kt[size_t(row < NA ? fa + row : fb + row - NA) * n + col], v);
   };
   double jab[NA][nb];
 #pragma unroll
@@ -706,13 +647,19 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
 #pragma unroll
     for (int b = 0; b < nb; ++b) jab[a][b] = 0.0;
 
-  for (int k = tid; k < ch.count; k += nt) {
-    const Quartet q = g.q[ch.start + k];
-    const PairRec ket = g.pairs[q.ket];
+  int seg = 0;
+  for (int k = ch.k0 + tid; k < ch.k1; k += nt) {
+    const int kp = chunk_ket(g.kets, sg, k, seg);
+    const PairRec ket = g.pairs[kp];
+    const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
+    if (mode == 0) continue;
     double gv[NA][nb][nc][nd];
-    quartet_block_regs<La, Lb, Lc, Ld, R, A0, A1>(g, bra, ket, false, gv);
+    if (MIXED && mode == 2)
+      quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
+    else
+      quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1>(g, bra, ket, false, gv);
     const int fc = ket.fa, fd = ket.fb;
-    const bool kl = ket.same, bk = (q.ket == ch.bra);
+    const bool kl = ket.same, bk = (kp == ch.bra);
     const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
     const double fK = bk ? 1.0 : 2.0;
     // Jt[ab] += fJ sum_cd g D[cd]  (summed over the chunk)
@@ -737,7 +684,7 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
           for (int a = 0; a < NA; ++a)
 #pragma unroll
             for (int b = 0; b < nb; ++b) s += gv[a][b][c][d] * D[size_t(fa + a) * n + fb + b];
-          atomic_add(&g.Jt[size_t(fc + c) * n + fd + d], fJ * s);
+          atomic_add(&Jt[size_t(fc + c) * n + fd + d], fJ * s);
         }
     }
     // Kt[ac] += fK sum_bd g D[bd]
@@ -798,7 +745,7 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
       const double v = wave_sum(jab[a][b]);
-      if ((tid & 63) == 0 && v != 0.0) atomic_add(&g.Jt[size_t(fa + a) * n + fb + b], v);
+      if ((tid & 63) == 0 && v != 0.0) atomic_add(&Jt[size_t(fa + a) * n + fb + b], v);
     }
   if (lds) {
     __syncthreads();
@@ -806,74 +753,92 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
       const double v = krow[e];
       if (v != 0.0) {
         const int row = e / n, col = e - row * n;
-        atomic_add(&g.Kt[size_t(row < NA ? fa + row : fb + row - NA) * n + col], v);
+        atomic_add(&Kt[size_t(row < NA ? fa + row : fb + row - NA) * n + col], v);
       }
     }
     __syncthreads();  // krow is reused by the next chunk
   }
 }
 
-template <int La, int Lb, int Lc, int Ld, typename R, int CHA, int C>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
 __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChunk& ch, double* krow, int chunk) {
   constexpr int na = ncart(La);
   constexpr int A0 = C * CHA;
   constexpr int A1 = (A0 + CHA < na) ? A0 + CHA : na;
   if (chunk == C) {
-    jk_chunk_regs<La, Lb, Lc, Ld, R, A0, A1>(g, ch, krow);
+    jk_chunk_regs<La, Lb, Lc, Ld, MIXED, A0, A1>(g, ch, krow);
   } else if constexpr (A1 < na) {
-    dispatch_jk_chunk<La, Lb, Lc, Ld, R, CHA, C + 1>(g, ch, krow, chunk);
+    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, C + 1>(g, ch, krow, chunk);
   }
 }
 
 // Kernel A, anchored: 256 lanes per chunk, bra index chunk C = blockIdx.y.
-template <int La, int Lb, int Lc, int Ld, typename R, int CHA>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
 __global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
   extern __shared__ double krow[];
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
     const JkChunk ch = g.chunks[c];
-    dispatch_jk_chunk<La, Lb, Lc, Ld, R, CHA, 0>(g, ch, krow, blockIdx.y);
+    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, blockIdx.y);
   }
 }
 
 // Kernel B, anchored: one workgroup per chunk, the chunk's quartets one after
-// another.  Dynamic LDS: the exchange rows [na + nb][nbf], then the ket
+// another.  Dynamic LDS: the exchange rows [na + nb][mn], then the ket
 // Coulomb block [nc*nd] of the current quartet and the bra block [na*nb].
-template <int La, int Lb, int Lc, int Ld, typename R, int NT>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int NT>
 __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
-  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-  constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
+  using Sh = LdsShape<La, Lb, Lc, Ld, double>;
+  constexpr int NOUT = Sh::NOUT;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NJ = (NOUT + NT - 1) / NT;
   constexpr int ROWS = na + nb;
   constexpr int NPQ = Sh::template npq<NT>();
-  __shared__ R tab[NPQ * Sh::TS];
-  __shared__ R hs[NPQ * Sh::HSS];
+  // FP64 tables; the FP32 recursion uses the same storage as float
+  __shared__ double tab[NPQ * Sh::TS];
+  __shared__ double hs[NPQ * Sh::HSS];
   extern __shared__ double dyn[];
   const int tid = threadIdx.x;
-  const int n = g.nbf;
   const bool lds = g.klds != 0;
-  double* krow = dyn;
-  double* jcd = dyn + ROWS * n;
-  double* jab = jcd + nc * nd;
   int ofs[NJ];
   lds_component_offsets<La, Lb, Lc, Ld, NT, NJ>(tid, ofs);
-  const double* __restrict__ D = g.D;
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
     const PairRec bra = g.pairs[ch.bra];
+    const int n = bra.mn;
+    const double* __restrict__ D = g.D + bra.moff;
+    double* Jt = g.Jt + bra.moff;
+    double* Kt = g.Kt + bra.moff;
+    double* krow = dyn;
+    double* jcd = dyn + ROWS * n;
+    double* jab = jcd + nc * nd;
     const int fa = bra.fa, fb = bra.fb;
     const bool ij = bra.same;
+    const JkSegs* sg = g.segs + ch.seg;
     if (lds) {
       for (int e = tid; e < ROWS * n + nc * nd + na * nb; e += NT) dyn[e] = 0.0;
     }
-    for (int k = 0; k < ch.count; ++k) {
-      const Quartet q = g.q[ch.start + k];
-      const PairRec ket = g.pairs[q.ket];
-      R acc[NJ];
-      lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, false, tab, hs, ofs, acc);
+    int seg = 0;
+    for (int k = ch.k0; k < ch.k1; ++k) {
+      const int kp = chunk_ket(g.kets, sg, k, seg);
+      const PairRec ket = g.pairs[kp];
+      const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
+      if (mode == 0) continue;  // uniform over the workgroup
+      double v[NJ];
+      if (MIXED && mode == 2) {
+        float acc[NJ];
+        lds_quartet_acc<La, Lb, Lc, Ld, float, NT, NJ, NPQ>(g, bra, ket, false, reinterpret_cast<float*>(tab),
+                                                            reinterpret_cast<float*>(hs), ofs, acc);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) v[j] = acc[j];
+      } else {
+        double acc[NJ];
+        lds_quartet_acc<La, Lb, Lc, Ld, double, NT, NJ, NPQ>(g, bra, ket, false, tab, hs, ofs, acc);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) v[j] = acc[j];
+      }
       const int fc = ket.fa, fd = ket.fb;
-      const bool kl = ket.same, bk = (q.ket == ch.bra);
+      const bool kl = ket.same, bk = (kp == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
       const double fK = bk ? 1.0 : 2.0;
 #pragma unroll
@@ -881,15 +846,14 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
         const int o = tid + NT * j;
         if (o >= NOUT) continue;
         const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
-        const double v = double(acc[j]) * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) *
-                         norm_corr(Ld, d);
+        const double x = v[j] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
         const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
-        const double jv = fJ * v * D[la * n + si];
-        const double jw = bk ? 0.0 : fJ * v * D[mu * n + nu];
-        const double k1 = fK * v * D[nu * n + si];
-        const double k2 = kl ? 0.0 : fK * v * D[nu * n + la];
-        const double k3 = ij ? 0.0 : fK * v * D[mu * n + si];
-        const double k4 = (ij || kl) ? 0.0 : fK * v * D[mu * n + la];
+        const double jv = fJ * x * D[la * n + si];
+        const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
+        const double k1 = fK * x * D[nu * n + si];
+        const double k2 = kl ? 0.0 : fK * x * D[nu * n + la];
+        const double k3 = ij ? 0.0 : fK * x * D[mu * n + si];
+        const double k4 = (ij || kl) ? 0.0 : fK * x * D[mu * n + la];
         if (lds) {
           lds_add(&jab[a * nb + b], jv);
           if (!bk) lds_add(&jcd[c * nd + d], jw);
@@ -898,20 +862,20 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
           if (!ij) lds_add(&krow[(na + b) * n + la], k3);
           if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
         } else {
-          atomic_add(&g.Jt[mu * n + nu], jv);
-          if (!bk) atomic_add(&g.Jt[la * n + si], jw);
-          atomic_add(&g.Kt[mu * n + la], k1);
-          if (!kl) atomic_add(&g.Kt[mu * n + si], k2);
-          if (!ij) atomic_add(&g.Kt[nu * n + la], k3);
-          if (!ij && !kl) atomic_add(&g.Kt[nu * n + si], k4);
+          atomic_add(&Jt[mu * n + nu], jv);
+          if (!bk) atomic_add(&Jt[la * n + si], jw);
+          atomic_add(&Kt[mu * n + la], k1);
+          if (!kl) atomic_add(&Kt[mu * n + si], k2);
+          if (!ij) atomic_add(&Kt[nu * n + la], k3);
+          if (!ij && !kl) atomic_add(&Kt[nu * n + si], k4);
         }
       }
       if (lds && !bk) {  // this quartet's ket Coulomb block to HBM
         __syncthreads();
         for (int e = tid; e < nc * nd; e += NT) {
-          const double v = jcd[e];
+          const double x = jcd[e];
           jcd[e] = 0.0;
-          if (v != 0.0) atomic_add(&g.Jt[size_t(fc + e / nd) * n + fd + e % nd], v);
+          if (x != 0.0) atomic_add(&Jt[size_t(fc + e / nd) * n + fd + e % nd], x);
         }
         __syncthreads();
       }
@@ -919,15 +883,15 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
     if (lds) {
       __syncthreads();
       for (int e = tid; e < ROWS * n; e += NT) {
-        const double v = krow[e];
-        if (v != 0.0) {
+        const double x = krow[e];
+        if (x != 0.0) {
           const int row = e / n, col = e - row * n;
-          atomic_add(&g.Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], v);
+          atomic_add(&Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], x);
         }
       }
       for (int e = tid; e < na * nb; e += NT) {
-        const double v = jab[e];
-        if (v != 0.0) atomic_add(&g.Jt[size_t(fa + e / nb) * n + fb + e % nb], v);
+        const double x = jab[e];
+        if (x != 0.0) atomic_add(&Jt[size_t(fa + e / nb) * n + fb + e % nb], x);
       }
       __syncthreads();  // dyn is cleared for the next chunk
     }

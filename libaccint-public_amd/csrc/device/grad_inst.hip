@@ -14,12 +14,20 @@ void launch_grad(const GradArgs& a0, hipStream_t s, int max_blocks) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
   constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
   using P = GradPolicy<La, Lb, Lc, Ld>;
-  if (a0.nq <= 0) return;
+  if (a0.nchunks <= 0) return;
   GradArgs a = a0;
   a.glds = a.n_atoms <= kGradLdsAtoms ? 1 : 0;
   const size_t dyn = a.glds ? size_t(3) * a.n_atoms * sizeof(double) : 0;
-  const int blocks = a.nq < max_blocks ? a.nq : max_blocks;
-  hipLaunchKernelGGL((grad_kernel<La, Lb, Lc, Ld, P::NT>), dim3(blocks), dim3(P::NT), dyn, s, a);
+  if constexpr (P::regs) {
+    const int blocks = a.nchunks < max_blocks ? a.nchunks : max_blocks;
+    hipLaunchKernelGGL((grad_kernel_regs<La, Lb, Lc, Ld>), dim3(blocks), dim3(256), dyn, s, a);
+  } else {
+    // the worklist chunks follow the J/K kernel shape of the class: long ones
+    // for its register shape are split over gridDim.y workgroups
+    constexpr int split = ClassPolicy<La, Lb, Lc, Ld>::regs ? 64 : 1;
+    const int blocks = a.nchunks < max_blocks ? a.nchunks : max_blocks;
+    hipLaunchKernelGGL((grad_kernel<La, Lb, Lc, Ld, P::NT>), dim3(blocks, split), dim3(P::NT), dyn, s, a);
+  }
 }
 
 template <int ID>

@@ -20,8 +20,6 @@
 namespace lai {
 
 struct GradArgs {
-  const Quartet* __restrict__ q;
-  int nq;
   const PairRec* __restrict__ pairs;
   const PrimPair* __restrict__ pp;
   const double2* __restrict__ ppab;  // exponents (alpha_a, alpha_b) of each primitive pair
@@ -33,6 +31,13 @@ struct GradArgs {
   double* grad;
   int n_atoms;
   int glds;  // 1: each workgroup sums its quartets' forces in LDS ([3 * n_atoms])
+  // the J/K worklist: bra-anchored chunks over ket segments
+  const JkChunk* __restrict__ chunks;
+  int nchunks;
+  const JkSegs* __restrict__ segs;
+  const int* __restrict__ kets;
+  QuartetSel sel;
+  const double* __restrict__ ppih;  // 1 / (2 zeta) of each primitive pair
 };
 
 template <int La, int Lb, int Lc, int Ld, int NT>
@@ -77,18 +82,28 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
     }
   }
 
-  for (int qi = blockIdx.x; qi < g.nq; qi += gridDim.x) {
-    const Quartet q = g.q[qi];
-    const PairRec bra = g.pairs[q.bra];
-    const PairRec ket = g.pairs[q.ket];
-    const bool ij = bra.same, kl = ket.same, bk = (q.bra == q.ket);
+  for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
+   const JkChunk ch = g.chunks[ci];
+   const PairRec bra = g.pairs[ch.bra];
+   const JkSegs* sg = g.segs + ch.seg;
+   int seg = 0;
+   // gridDim.y workgroups share a chunk (chunks of the register-shape J/K
+   // classes hold up to 2048 quartets)
+   const int len = ch.k1 - ch.k0;
+   const int kb = ch.k0 + int((long long)len * blockIdx.y / gridDim.y);
+   const int ke = ch.k0 + int((long long)len * (blockIdx.y + 1) / gridDim.y);
+   for (int kq = kb; kq < ke; ++kq) {
+    const int kp = chunk_ket(g.kets, sg, kq, seg);
+    const PairRec ket = g.pairs[kp];
+    if (quartet_select(g.sel, ch.bra, bra, kp, ket) == 0) continue;  // uniform over the workgroup
+    const bool ij = bra.same, kl = ket.same, bk = (kp == ch.bra);
     const double deg = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0) * (bk ? 1.0 : 2.0);
-    const size_t n = size_t(g.nbf);
+    const size_t n = size_t(bra.mn);
     __syncthreads();  // W of the previous quartet is no longer read
     for (int o = tid; o < NOUT; o += NT) {
       const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
       const size_t mu = bra.fa + a, nu = bra.fb + b, la = ket.fa + c, si = ket.fb + d;
-      const double* D = g.D;
+      const double* D = g.D + bra.moff;
       const double w = 0.5 * D[mu * n + nu] * D[la * n + si] -
                        0.125 * g.xscale * (D[mu * n + la] * D[nu * n + si] + D[mu * n + si] * D[nu * n + la]);
       W[o] = deg * w * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
@@ -197,6 +212,7 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
         atomic_add(&out[id], -(a + b + c));
       }
     }
+   }
   }
   if (glds) {  // this workgroup's forces, once
     __syncthreads();
@@ -207,10 +223,172 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
   }
 }
 
+// --------------------------------------------------------------------------
+// Register shape for the light classes: one lane per shell quartet of a
+// bra-anchored chunk, the 2D tables raised by one on A, B and C held in
+// registers.  The forces on the bra atoms are summed over the lane's quartets
+// and reduced over the wavefront; those on the ket atoms go to the
+// workgroup's LDS force array (or to HBM atomics without one).
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld>
+__device__ __forceinline__ void grad_quartet_regs(const GradArgs& g, const PairRec& bra, const PairRec& ket,
+                                                  bool bk, double (&f)[9]) {
+  constexpr int NR = (La + Lb + Lc + Ld + 1) / 2 + 1;
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  const size_t n = size_t(bra.mn);
+  const bool ij = bra.sa == bra.sb, kl = ket.sa == ket.sb;
+  const double deg = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0) * (bk ? 1.0 : 2.0);
+  const double* __restrict__ D = g.D + bra.moff;
+  double W[na][nb][nc][nd];
+#pragma unroll
+  for (int a = 0; a < na; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b)
+#pragma unroll
+      for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          const size_t mu = bra.fa + a, nu = bra.fb + b, la = ket.fa + c, si = ket.fb + d;
+          const double w = 0.5 * D[mu * n + nu] * D[la * n + si] -
+                           0.125 * g.xscale * (D[mu * n + la] * D[nu * n + si] + D[mu * n + si] * D[nu * n + la]);
+          W[a][b][c][d] = deg * w * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
+        }
+#pragma unroll
+  for (int x = 0; x < 9; ++x) f[x] = 0.0;
+  const double ABx = bra.ABx, ABy = bra.ABy, ABz = bra.ABz;
+  const double CDx = ket.ABx, CDy = ket.ABy, CDz = ket.ABz;
+  for (int ik = 0; ik < ket.npp_sig; ++ik) {
+    const PrimPair Q = g.pp[ket.pp_off + ik];
+    const double2 ek = g.ppab[ket.pp_off + ik];
+    const double eta = Q.zeta, h_e = g.ppih[ket.pp_off + ik];
+    const double tc2 = 2.0 * ek.x;
+    for (int ib = 0; ib < bra.npp_sig; ++ib) {
+      const PrimPair P = g.pp[bra.pp_off + ib];
+      const double2 eb = g.ppab[bra.pp_off + ib];
+      const double zeta = P.zeta, h_z = g.ppih[bra.pp_off + ib];
+      const double ta2 = 2.0 * eb.x, tb2 = 2.0 * eb.y;
+      const double x = zeta + eta;
+      double y = __builtin_amdgcn_rsq(x);
+      y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+      y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+      const double izpe = y * y;
+      const double rz = eta * izpe, re = zeta * izpe;
+      const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+      const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = P.Kc * Q.Kc * y;
+      double u[NR], w[NR];
+      rys_roots<NR>(T, g.rys, u, w);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double ur = u[r];
+        const double B00 = 0.5 * ur * izpe, B10 = h_z - h_z * rz * ur, B01 = h_e - h_e * re * ur;
+        const double sz = rz * ur, se = re * ur;
+        double Ix[La + 2][Lb + 2][Lc + 2][Ld + 1], Iy[La + 2][Lb + 2][Lc + 2][Ld + 1],
+            Iz[La + 2][Lb + 2][Lc + 2][Ld + 1];
+        build_2d<La + 1, Lb + 1, Lc + 1, Ld, double>(Ix, P.PAx - sz * PQx, Q.PAx + se * PQx, ABx, CDx, B10, B01,
+                                                     B00, 1.0);
+        build_2d<La + 1, Lb + 1, Lc + 1, Ld, double>(Iy, P.PAy - sz * PQy, Q.PAy + se * PQy, ABy, CDy, B10, B01,
+                                                     B00, 1.0);
+        build_2d<La + 1, Lb + 1, Lc + 1, Ld, double>(Iz, P.PAz - sz * PQz, Q.PAz + se * PQz, ABz, CDz, B10, B01,
+                                                     B00, pref * w[r]);
+#pragma unroll
+        for (int a = 0; a < na; ++a)
+#pragma unroll
+          for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+              for (int d = 0; d < nd; ++d) {
+                const int ax = cart_comp(La, a, 0), ay = cart_comp(La, a, 1), az = cart_comp(La, a, 2);
+                const int bx = cart_comp(Lb, b, 0), by = cart_comp(Lb, b, 1), bz = cart_comp(Lb, b, 2);
+                const int cx = cart_comp(Lc, c, 0), cy = cart_comp(Lc, c, 1), cz = cart_comp(Lc, c, 2);
+                const int dx = cart_comp(Ld, d, 0), dy = cart_comp(Ld, d, 1), dz = cart_comp(Ld, d, 2);
+                const double wt = W[a][b][c][d];
+                const double x0 = Ix[ax][bx][cx][dx], y0 = Iy[ay][by][cy][dy], z0 = Iz[az][bz][cz][dz];
+                // d/dA: 2 alpha (a+1) - a (a-1), and likewise for B and C
+                const double dAx = ta2 * Ix[ax + 1][bx][cx][dx] - (ax ? ax * Ix[ax ? ax - 1 : 0][bx][cx][dx] : 0.0);
+                const double dAy = ta2 * Iy[ay + 1][by][cy][dy] - (ay ? ay * Iy[ay ? ay - 1 : 0][by][cy][dy] : 0.0);
+                const double dAz = ta2 * Iz[az + 1][bz][cz][dz] - (az ? az * Iz[az ? az - 1 : 0][bz][cz][dz] : 0.0);
+                const double dBx = tb2 * Ix[ax][bx + 1][cx][dx] - (bx ? bx * Ix[ax][bx ? bx - 1 : 0][cx][dx] : 0.0);
+                const double dBy = tb2 * Iy[ay][by + 1][cy][dy] - (by ? by * Iy[ay][by ? by - 1 : 0][cy][dy] : 0.0);
+                const double dBz = tb2 * Iz[az][bz + 1][cz][dz] - (bz ? bz * Iz[az][bz ? bz - 1 : 0][cz][dz] : 0.0);
+                const double dCx = tc2 * Ix[ax][bx][cx + 1][dx] - (cx ? cx * Ix[ax][bx][cx ? cx - 1 : 0][dx] : 0.0);
+                const double dCy = tc2 * Iy[ay][by][cy + 1][dy] - (cy ? cy * Iy[ay][by][cy ? cy - 1 : 0][dy] : 0.0);
+                const double dCz = tc2 * Iz[az][bz][cz + 1][dz] - (cz ? cz * Iz[az][bz][cz ? cz - 1 : 0][dz] : 0.0);
+                const double yz = wt * y0 * z0, xz = wt * x0 * z0, xy = wt * x0 * y0;
+                f[0] += dAx * yz;
+                f[1] += dAy * xz;
+                f[2] += dAz * xy;
+                f[3] += dBx * yz;
+                f[4] += dBy * xz;
+                f[5] += dBz * xy;
+                f[6] += dCx * yz;
+                f[7] += dCy * xz;
+                f[8] += dCz * xy;
+              }
+      }
+    }
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld>
+__global__ __launch_bounds__(256) void grad_kernel_regs(GradArgs g) {
+  extern __shared__ double gacc[];
+  const int tid = threadIdx.x;
+  const bool glds = g.glds != 0;
+  if (glds) {
+    for (int e = tid; e < 3 * g.n_atoms; e += blockDim.x) gacc[e] = 0.0;
+    __syncthreads();
+  }
+  auto add = [&](int i, double v) {
+    if (glds)
+      lds_add(&gacc[i], v);
+    else
+      atomic_add(&g.grad[i], v);
+  };
+  for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
+    const JkChunk ch = g.chunks[ci];
+    const PairRec bra = g.pairs[ch.bra];
+    const int iA = 3 * g.shell_atom[bra.sa], iB = 3 * g.shell_atom[bra.sb];
+    double fab[6] = {0, 0, 0, 0, 0, 0};
+    const JkSegs* sg = g.segs + ch.seg;
+    int seg = 0;
+    for (int k = ch.k0 + tid; k < ch.k1; k += blockDim.x) {
+      const int kp = chunk_ket(g.kets, sg, k, seg);
+      const PairRec ket = g.pairs[kp];
+      if (quartet_select(g.sel, ch.bra, bra, kp, ket) == 0) continue;
+      double f[9];
+      grad_quartet_regs<La, Lb, Lc, Ld>(g, bra, ket, kp == ch.bra, f);
+#pragma unroll
+      for (int x = 0; x < 6; ++x) fab[x] += f[x];
+      const int iC = 3 * g.shell_atom[ket.sa], iD = 3 * g.shell_atom[ket.sb];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        add(iC + x, f[6 + x]);
+        add(iD + x, -(f[x] + f[3 + x] + f[6 + x]));  // translational invariance
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < 6; ++x) {
+      const double v = wave_sum(fab[x]);
+      if ((tid & 63) == 0 && v != 0.0) add((x < 3 ? iA : iB) + x % 3, v);
+    }
+  }
+  if (glds) {  // this workgroup's forces, once
+    __syncthreads();
+    for (int e = tid; e < 3 * g.n_atoms; e += blockDim.x) {
+      const double v = gacc[e];
+      if (v != 0.0) atomic_add(&g.grad[e], v);
+    }
+  }
+}
+
 template <int La, int Lb, int Lc, int Ld>
 struct GradPolicy {
   static constexpr int NOUT = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld);
   static constexpr int NT = NOUT <= 256 ? 64 : 128;
+  // register shape while the raised tables and outputs stay small
+  static constexpr bool regs = (La + 2) * (Lb + 2) * (Lc + 2) * (Ld + 1) <= 24 && NOUT <= 27;
 };
 
 }  // namespace lai

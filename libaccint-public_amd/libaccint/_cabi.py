@@ -33,6 +33,8 @@ _SIGNATURES = {
     "lai_dev_upload_basis": ([_vp, ctypes.c_int, _ip, _dp, _ip, _dp, _dp, ctypes.POINTER(_vp)], ctypes.c_int),
     "lai_dev_basis_free": ([_vp], ctypes.c_int),
     "lai_dev_basis_nbf": ([_vp], ctypes.c_int),
+    "lai_dev_upload_batch": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lai_dev_batch_layout": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_eri_quartets": ([_vp, _i64, _ip, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
     "lai_dev_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_set_schwarz": ([_vp, _dp], ctypes.c_int),
