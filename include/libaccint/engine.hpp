@@ -281,4 +281,47 @@ private:
     std::vector<Real> wl_dmax_;
 };
 
+// Several independent molecules through one device basis handle: each J/K
+// class kernel covers the quartets of every molecule in one launch, from one
+// worklist ordered by angular-momentum class.  D, J and K are the molecules'
+// nbf_m x nbf_m matrices one after another (matrix_offsets()).  The engine
+// shares ownership of the bases.  No reference counterpart: the reference
+// runs one Engine per BasisSet.
+class BatchEngine {
+public:
+    explicit BatchEngine(std::vector<std::shared_ptr<const BasisSet>> bases);
+    ~BatchEngine();
+    BatchEngine(const BatchEngine&) = delete;
+    BatchEngine& operator=(const BatchEngine&) = delete;
+
+    [[nodiscard]] Size n_molecules() const noexcept { return bases_.size(); }
+    [[nodiscard]] const BasisSet& basis(Size m) const { return *bases_.at(m); }
+    [[nodiscard]] const std::vector<Size>& n_basis_functions() const noexcept { return nbf_; }
+    // offsets of the molecules' blocks in the batched matrices, n_molecules() + 1 entries
+    [[nodiscard]] const std::vector<Size>& matrix_offsets() const noexcept { return moff_; }
+    [[nodiscard]] Size matrix_size() const noexcept { return moff_.back(); }
+
+    void precompute_schwarz_bounds();
+    // D_host: the batched density (matrix_size() doubles) for density-dependent options
+    WorklistInfo prepare_worklist(const JKOptions& opts, const Real* D_host = nullptr);
+    [[nodiscard]] const WorklistInfo& worklist_info() const noexcept { return wl_info_; }
+    // d_J += J[D], d_K += K[D] for every molecule (device pointers, batched layout)
+    void build_jk(const JKOptions& opts, const Real* d_D, Real* d_J, Real* d_K, hipStream_t stream = nullptr,
+                  const Real* D_host = nullptr);
+    void build_jk_host(const JKOptions& opts, const Real* D, Real* J, Real* K);
+    void capture_jk_graph(const JKOptions& opts, const Real* d_D, Real* d_J, Real* d_K, hipStream_t stream);
+    void launch_jk_graph(hipStream_t stream);
+    [[nodiscard]] lai_basis* device_handle();
+
+private:
+    std::vector<Real> shell_dmax(const Real* D) const;
+    std::vector<std::shared_ptr<const BasisSet>> bases_;
+    std::vector<Size> nbf_, moff_;
+    lai_basis* dev_{nullptr};
+    bool have_q_{false};
+    bool wl_valid_{false};
+    JKOptions wl_opts_{};
+    WorklistInfo wl_info_{};
+};
+
 }  // namespace libaccint

@@ -852,15 +852,17 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
   }, 1);
 
   // Chunks: register-shape classes take up to 2048 quartets per workgroup
-  // (8 per lane), LDS-shape classes up to 32; small classes get shorter chunks
-  // so that a launch still has a few workgroups per CU.
+  // (8 per lane), wavefront-per-quartet classes up to 64, the other LDS-shape
+  // classes up to 32; small classes get shorter chunks so that a launch still
+  // has a few workgroups per CU.
   const int n_cu = b->ctx->n_cu;
   std::vector<JkSegs> segs_h;
   std::vector<std::vector<JkChunk>> C(kNumClasses);
   for (int c = 0; c < kNumClasses; ++c) {
     const size_t n = size_t(etot[c]);
-    const int cmax = Ls.info[c].regs ? int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 256, 2048))
-                                     : int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 1, 32));
+    const int cmax = Ls.info[c].regs    ? int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 256, 2048))
+                     : Ls.info[c].waves ? int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 4, 64))
+                                        : int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 1, 32));
     for (const auto& e : E[c]) {
       const int si = int(segs_h.size());
       segs_h.push_back(e.sg);

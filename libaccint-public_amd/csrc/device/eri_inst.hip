@@ -17,6 +17,26 @@ namespace {
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kLdsBudgetB = 160 * 1024;
 
+// static LDS of a kernel as compiled (queried once per instantiation)
+template <typename K>
+size_t static_lds(K kern) {
+  hipFuncAttributes at{};
+  if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(kern)) != hipSuccess) return kLdsBudgetB;
+  return at.sharedSizeBytes;
+}
+
+// Exchange rows in LDS when static + dynamic LDS fit the budget; above 64 KB
+// the kernel is allowed the larger dynamic size first.
+template <typename K>
+int use_rows(K kern, size_t stat, size_t dyn, size_t budget) {
+  if (stat + dyn > budget) return 0;
+  if (stat + dyn > 64 * 1024 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          int(dyn)) != hipSuccess)
+    return 0;
+  return 1;
+}
+
 // MIXED = false: every quartet in FP64 (and the raw-ERI mode); true: the
 // quartets the worklist marks for FP32 run the recursion in float.
 template <int ID, bool MIXED>
@@ -44,20 +64,22 @@ void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
   const size_t nbf = size_t(a.nbf);
   if constexpr (P::regs) {
     const size_t dyn = size_t(P::CHA + nb) * nbf * sizeof(double);
-    a.klds = dyn <= kLdsBudget ? 1 : 0;
+    a.klds = use_rows(jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>, 0, dyn, kLdsBudget);
     int blocks = a.nchunks < max_blocks ? a.nchunks : max_blocks;
     dim3 grid(blocks, P::NCH);
     hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
+  } else if constexpr (P::waves) {
+    auto kern = jk_kernel_waves<La, Lb, Lc, Ld, MIXED, P::NW>;
+    static const size_t stat = static_lds(kern);
+    const size_t dyn = (size_t(na + nb) * nbf + na * nb) * sizeof(double);
+    a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
+    int blocks = a.nchunks < 8 * max_blocks ? a.nchunks : 8 * max_blocks;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * P::NW), a.klds ? dyn : 0, s, a);
   } else {
-    using Sh = LdsShape<La, Lb, Lc, Ld, double>;
-    constexpr size_t stat = size_t(Sh::template npq<P::NT>()) * (Sh::TS + Sh::HSS) * sizeof(double);
-    const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
-    a.klds = stat + dyn <= kLdsBudgetB ? 1 : 0;
     auto kern = jk_kernel_lds<La, Lb, Lc, Ld, MIXED, P::NT>;
-    if (a.klds && stat + dyn > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            int(dyn)) != hipSuccess)
-      a.klds = 0;  // exchange rows straight to HBM
+    static const size_t stat = static_lds(kern);
+    const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
+    a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
     int blocks = a.nchunks < 8 * max_blocks ? a.nchunks : 8 * max_blocks;
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(P::NT), a.klds ? dyn : 0, s, a);
   }
@@ -77,7 +99,8 @@ void fill(ClassLauncher* f64, ClassLauncher* f32, ClassInfo* info) {
     info[ID].Ld = Ld;
     info[ID].regs = P::regs ? 1 : 0;
     info[ID].nchunk = P::regs ? P::NCH : 0;
-    info[ID].nt = P::regs ? 256 : P::NT;
+    info[ID].nt = P::regs ? 256 : (P::waves ? 64 * P::NW : P::NT);
+    info[ID].waves = P::waves ? P::NW : 0;
   }
   if constexpr (ID + 1 < kNumClasses) fill<ID + 1>(f64, f32, info);
 }

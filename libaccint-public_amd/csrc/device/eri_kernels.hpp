@@ -448,18 +448,24 @@ __device__ __forceinline__ void rys_root_one(double T, const double* __restrict_
   }
 }
 
-// Table offsets of this lane's outputs o = tid + NT*j: the three per-axis
-// offsets into a (root, axis) table, 10 bits each.
+// Products are tiled by task = (a, b, c): a lane owns tasks t = tid + NT*j
+// and computes all nd outputs (a, b, c, d) of each from three rows of Ld+1
+// table entries held in registers.  ofs[j] packs the row start of each axis
+// (10 bits each).
+template <int La, int Lb, int Lc, int Ld>
+struct LdsTasks {
+  static constexpr int NTASK = ncart(La) * ncart(Lb) * ncart(Lc);
+};
 template <int La, int Lb, int Lc, int Ld, int NT, int NJ>
 __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
-  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
-  constexpr int NOUT = na * nb * nc * nd;
+  constexpr int nb = ncart(Lb), nc = ncart(Lc);
+  constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int o = tid + NT * j;
-    if (o < NOUT) {
-      const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
-      int ax[3], bx[3], cx[3], dx[3];
+    const int t = tid + NT * j;
+    if (t < NTASK) {
+      const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+      int ax[3], bx[3], cx[3];
       // component exponents by a runtime walk (once per launch)
       int cnt = 0;
       for (int lx = La; lx >= 0; --lx)
@@ -479,16 +485,10 @@ __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
           if (cnt == c) { cx[0] = lx; cx[1] = ly; cx[2] = Lc - lx - ly; }
           ++cnt;
         }
-      cnt = 0;
-      for (int lx = Ld; lx >= 0; --lx)
-        for (int ly = Ld - lx; ly >= 0; --ly) {
-          if (cnt == d) { dx[0] = lx; dx[1] = ly; dx[2] = Ld - lx - ly; }
-          ++cnt;
-        }
       int p = 0;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        const int off = ((ax[x] * (Lb + 1) + bx[x]) * (Lc + 1) + cx[x]) * (Ld + 1) + dx[x];
+        const int off = ((ax[x] * (Lb + 1) + bx[x]) * (Lc + 1) + cx[x]) * (Ld + 1);
         p |= off << (10 * x);
       }
       ofs[j] = p;
@@ -505,13 +505,16 @@ __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
 template <int La, int Lb, int Lc, int Ld, typename R, int NT, int NJ, int NPQ>
 __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket,
                                                 bool all_prims, R* tab, R* hs, const int (&ofs)[NJ],
-                                                R (&acc)[NJ]) {
+                                                R (&acc)[NJ][ncart(Ld)]) {
+  constexpr int nd = ncart(Ld);
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
   constexpr int NR = Sh::NR, S = Sh::S, TS = Sh::TS, HSS = Sh::HSS, TASKS = 3 * NR;
   static_assert(NPQ * TASKS <= NT, "not enough lanes for the table batch");
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) acc[j] = R(0);
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int d = 0; d < nd; ++d) acc[j][d] = R(0);
   const int npp_b = all_prims ? bra.npp : bra.npp_sig, npp_k = all_prims ? ket.npp : ket.npp_sig;
   const int ntot = npp_b * npp_k;
   constexpr bool fp32 = sizeof(R) == 4;
@@ -562,7 +565,16 @@ __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec&
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
           const int p = ofs[j];
-          acc[j] += tx[p & 1023] * tx[S + ((p >> 10) & 1023)] * tx[2 * S + (p >> 20)];
+          R X[Ld + 1], Y[Ld + 1], Z[Ld + 1];
+#pragma unroll
+          for (int e = 0; e <= Ld; ++e) {
+            X[e] = tx[(p & 1023) + e];
+            Y[e] = tx[S + ((p >> 10) & 1023) + e];
+            Z[e] = tx[2 * S + (p >> 20) + e];
+          }
+#pragma unroll
+          for (int d = 0; d < nd; ++d)
+            acc[j][d] += X[cart_comp(Ld, d, 0)] * Y[cart_comp(Ld, d, 1)] * Z[cart_comp(Ld, d, 2)];
         }
       }
     }
@@ -574,9 +586,9 @@ __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec&
 template <int La, int Lb, int Lc, int Ld, typename R, int NT>
 __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-  constexpr int NR = Sh::NR, S = Sh::S, NOUT = Sh::NOUT;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
-  constexpr int NJ = (NOUT + NT - 1) / NT;
+  constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
+  constexpr int NJ = (NTASK + NT - 1) / NT;
   constexpr int NPQ = Sh::template npq<NT>();
   __shared__ R tab[NPQ * Sh::TS];
   __shared__ R hs[NPQ * Sh::HSS];
@@ -588,19 +600,22 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
     const Quartet q = g.q[qi];
     const PairRec bra = g.pairs[q.bra];
     const PairRec ket = g.pairs[q.ket];
-    R acc[NJ];
+    R acc[NJ][nd];
     lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, true, tab, hs, ofs, acc);
     const int perm = g.perm[qi];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-      const int o = tid + NT * j;
-      if (o >= NOUT) continue;
-      const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
-      const double v = double(acc[j]) * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) *
-                       norm_corr(Ld, d);
-      const int idx[4] = {a, b, c, d};
-      const int n4[4] = {na, nb, nc, nd};
-      g.out[g.obase[qi] + orig_offset(perm, idx, n4)] = v;
+      const int t = tid + NT * j;
+      if (t >= NTASK) continue;
+      const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+#pragma unroll
+      for (int d = 0; d < nd; ++d) {
+        const double v = double(acc[j][d]) * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) *
+                         norm_corr(Ld, d);
+        const int idx[4] = {a, b, c, d};
+        const int n4[4] = {na, nb, nc, nd};
+        g.out[g.obase[qi] + orig_offset(perm, idx, n4)] = v;
+      }
     }
   }
 }
@@ -788,14 +803,19 @@ __global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int NT>
 __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   using Sh = LdsShape<La, Lb, Lc, Ld, double>;
-  constexpr int NOUT = Sh::NOUT;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
-  constexpr int NJ = (NOUT + NT - 1) / NT;
+  constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
+  constexpr int NJ = (NTASK + NT - 1) / NT;
   constexpr int ROWS = na + nb;
   constexpr int NPQ = Sh::template npq<NT>();
   // FP64 tables; the FP32 recursion uses the same storage as float
   __shared__ double tab[NPQ * Sh::TS];
   __shared__ double hs[NPQ * Sh::HSS];
+  // without LDS rows: the quartet's six J/K blocks, summed in LDS and added to
+  // HBM once per quartet (Jt[ab], Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd])
+  constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
+                OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
+  __shared__ double blk[NBLK];
   extern __shared__ double dyn[];
   const int tid = threadIdx.x;
   const bool lds = g.klds != 0;
@@ -817,6 +837,8 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
     const JkSegs* sg = g.segs + ch.seg;
     if (lds) {
       for (int e = tid; e < ROWS * n + nc * nd + na * nb; e += NT) dyn[e] = 0.0;
+    } else {
+      for (int e = tid; e < NBLK; e += NT) blk[e] = 0.0;
     }
     int seg = 0;
     for (int k = ch.k0; k < ch.k1; ++k) {
@@ -824,29 +846,35 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
       const PairRec ket = g.pairs[kp];
       const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
       if (mode == 0) continue;  // uniform over the workgroup
-      double v[NJ];
+      double v[NJ][nd];
       if (MIXED && mode == 2) {
-        float acc[NJ];
+        float acc[NJ][nd];
         lds_quartet_acc<La, Lb, Lc, Ld, float, NT, NJ, NPQ>(g, bra, ket, false, reinterpret_cast<float*>(tab),
                                                             reinterpret_cast<float*>(hs), ofs, acc);
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) v[j] = acc[j];
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       } else {
-        double acc[NJ];
+        double acc[NJ][nd];
         lds_quartet_acc<La, Lb, Lc, Ld, double, NT, NJ, NPQ>(g, bra, ket, false, tab, hs, ofs, acc);
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) v[j] = acc[j];
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       }
       const int fc = ket.fa, fd = ket.fb;
       const bool kl = ket.same, bk = (kp == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
       const double fK = bk ? 1.0 : 2.0;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const int o = tid + NT * j;
-        if (o >= NOUT) continue;
-        const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
-        const double x = v[j] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+       for (int d = 0; d < nd; ++d) {
+        const int t = tid + NT * j;
+        if (t >= NTASK) continue;
+        const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+        const double x = v[j][d] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
         const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
         const double jv = fJ * x * D[la * n + si];
         const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
@@ -862,13 +890,31 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
           if (!ij) lds_add(&krow[(na + b) * n + la], k3);
           if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
         } else {
-          atomic_add(&Jt[mu * n + nu], jv);
-          if (!bk) atomic_add(&Jt[la * n + si], jw);
-          atomic_add(&Kt[mu * n + la], k1);
-          if (!kl) atomic_add(&Kt[mu * n + si], k2);
-          if (!ij) atomic_add(&Kt[nu * n + la], k3);
-          if (!ij && !kl) atomic_add(&Kt[nu * n + si], k4);
+          lds_add(&blk[OAB + a * nb + b], jv);
+          if (!bk) lds_add(&blk[OCD + c * nd + d], jw);
+          lds_add(&blk[OAC + a * nc + c], k1);
+          if (!kl) lds_add(&blk[OAD + a * nd + d], k2);
+          if (!ij) lds_add(&blk[OBC + b * nc + c], k3);
+          if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], k4);
         }
+      }
+      if (!lds) {  // the quartet's blocks to HBM, one atomic per element
+        __syncthreads();
+        for (int e = tid; e < NBLK; e += NT) {
+          const double x = blk[e];
+          blk[e] = 0.0;
+          if (x == 0.0) continue;
+          size_t r, q;
+          double* M = Kt;
+          if (e < OCD) { r = fa + (e - OAB) / nb; q = fb + (e - OAB) % nb; M = Jt; }
+          else if (e < OAC) { r = fc + (e - OCD) / nd; q = fd + (e - OCD) % nd; M = Jt; }
+          else if (e < OAD) { r = fa + (e - OAC) / nc; q = fc + (e - OAC) % nc; }
+          else if (e < OBC) { r = fa + (e - OAD) / nd; q = fd + (e - OAD) % nd; }
+          else if (e < OBD) { r = fb + (e - OBC) / nc; q = fc + (e - OBC) % nc; }
+          else { r = fb + (e - OBD) / nd; q = fd + (e - OBD) % nd; }
+          atomic_add(&M[r * n + q], x);
+        }
+        __syncthreads();
       }
       if (lds && !bk) {  // this quartet's ket Coulomb block to HBM
         __syncthreads();
@@ -899,6 +945,255 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
 }
 
 // --------------------------------------------------------------------------
+// Kernel W: NW wavefronts per workgroup, one quartet per wavefront.  The
+// quartets of a chunk share the bra, so the waves share its exchange rows and
+// bra Coulomb block in LDS; each wave builds its own (primitive quartet, root,
+// axis) tables and accumulates its (a, b, c) tasks.  Workgroup barriers stay
+// matched: every step runs the largest primitive count of the NW quartets,
+// waves past their own count build zero tables.
+// --------------------------------------------------------------------------
+template <int La, int Lb, int Lc, int Ld, typename R, int NJ, int NPQ>
+__device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket, int ntot,
+                                                 int ntot_max, R* tab, R* hs, int lane, const int (&ofs)[NJ],
+                                                 R (&acc)[NJ][ncart(Ld)]) {
+  using Sh = LdsShape<La, Lb, Lc, Ld, R>;
+  constexpr int nd = ncart(Ld);
+  constexpr int NR = Sh::NR, S = Sh::S, TS = Sh::TS, HSS = Sh::HSS, TASKS = 3 * NR;
+  static_assert(NPQ * TASKS <= 64, "not enough lanes for the table batch");
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int d = 0; d < nd; ++d) acc[j][d] = R(0);
+  const int npp_k = ket.npp_sig;
+  constexpr bool fp32 = sizeof(R) == 4;
+  const float qcut = fp32 ? float(g.pcut32 * bra.bsum * ket.bsum) : 0.0f;
+  __syncthreads();  // the previous user of tab and hs has finished
+  for (int t0 = 0; t0 < ntot_max; t0 += NPQ) {
+    if (lane < NPQ * TASKS) {
+      const int pq = lane / TASKS, task = lane - pq * TASKS;
+      const int r = task / 3, ax = task - 3 * r;
+      const int t = t0 + pq;
+      R* tb = tab + pq * TS + task * S;
+      bool live = t < ntot;
+      const int ib = live ? t / npp_k : 0, ik = live ? t - ib * npp_k : 0;
+      if (live && fp32 && g.ppb[bra.pp_off + ib] * g.ppb[ket.pp_off + ik] < qcut) live = false;
+      if (live) {
+        const PrimPair P = g.pp[bra.pp_off + ib];
+        const PrimPair Q = g.pp[ket.pp_off + ik];
+        const double zeta = P.zeta, eta = Q.zeta;
+        const double izpe = 1.0 / (zeta + eta);
+        const double rz = eta * izpe, re = zeta * izpe;
+        const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+        const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
+        double ur, wr;
+        rys_root_one<NR>(T, g.rys, r, ur, wr);
+        const double h_z = g.ppih[bra.pp_off + ib], h_e = g.ppih[ket.pp_off + ik];
+        const R B00 = R(0.5 * ur * izpe);
+        const R B10 = R(h_z - h_z * rz * ur);
+        const R B01 = R(h_e - h_e * re * ur);
+        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
+        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
+        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
+        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
+        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+        const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
+        build_2d_lds<La, Lb, Lc, Ld, R>(tb, hs + pq * HSS + task * Sh::HS, R(PA - rz * ur * PQ),
+                                        R(QC + re * ur * PQ), R(AB), R(CD), B10, B01, B00, R(sc));
+      } else {
+        for (int e = 0; e < S; ++e) tb[e] = R(0);  // contributes nothing
+      }
+    }
+    __syncthreads();
+    const int rem = ntot - t0;
+    const int npq = rem < NPQ ? (rem > 0 ? rem : 0) : NPQ;
+#pragma unroll 1
+    for (int pq = 0; pq < npq; ++pq) {
+#pragma unroll 1
+      for (int r = 0; r < NR; ++r) {
+        const R* tx = tab + pq * TS + (r * 3) * S;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int p = ofs[j];
+          R X[Ld + 1], Y[Ld + 1], Z[Ld + 1];
+#pragma unroll
+          for (int e = 0; e <= Ld; ++e) {
+            X[e] = tx[(p & 1023) + e];
+            Y[e] = tx[S + ((p >> 10) & 1023) + e];
+            Z[e] = tx[2 * S + (p >> 20) + e];
+          }
+#pragma unroll
+          for (int d = 0; d < nd; ++d)
+            acc[j][d] += X[cart_comp(Ld, d, 0)] * Y[cart_comp(Ld, d, 1)] * Z[cart_comp(Ld, d, 2)];
+        }
+      }
+    }
+    __syncthreads();  // products finished with tab before the next batch
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld>
+struct WaveShape {
+  static constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
+  static constexpr int NJ = (NTASK + 63) / 64;
+  // primitive quartets per wave and step, within 6 KB of tables per wave
+  static constexpr int NPQ = [] {
+    using Sh = LdsShape<La, Lb, Lc, Ld, double>;
+    constexpr int by_lanes = 64 / (3 * Sh::NR);
+    constexpr int by_lds = int((6 * 1024) / ((Sh::TS + Sh::HSS) * sizeof(double)));
+    constexpr int n = by_lanes < by_lds ? by_lanes : by_lds;
+    return n < 1 ? 1 : n;
+  }();
+};
+
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int NW>
+__global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
+  using Sh = LdsShape<La, Lb, Lc, Ld, double>;
+  using Ws = WaveShape<La, Lb, Lc, Ld>;
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NTASK = Ws::NTASK, NJ = Ws::NJ, NPQ = Ws::NPQ;
+  constexpr int NT = 64 * NW;
+  constexpr int ROWS = na + nb;
+  constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
+                OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
+  __shared__ double tab[NW][NPQ * Sh::TS];
+  __shared__ double hs[NW][NPQ * Sh::HSS];
+  __shared__ double blk[NW][NBLK];  // per wave: the quartet's J/K blocks (jcd only, with LDS rows)
+  __shared__ int kinfo[NW][4];      // per wave: ket functions fc, fd, flags, ntot
+  extern __shared__ double dyn[];   // exchange rows [na + nb][mn], then the bra block [na*nb]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
+  const bool lds = g.klds != 0;
+  int ofs[NJ];
+  lds_component_offsets<La, Lb, Lc, Ld, 64, NJ>(lane, ofs);
+
+  for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
+    const JkChunk ch = g.chunks[ci];
+    const PairRec bra = g.pairs[ch.bra];
+    const int n = bra.mn;
+    const double* __restrict__ D = g.D + bra.moff;
+    double* Jt = g.Jt + bra.moff;
+    double* Kt = g.Kt + bra.moff;
+    double* krow = dyn;
+    double* jab = dyn + ROWS * n;
+    const int fa = bra.fa, fb = bra.fb;
+    const bool ij = bra.same;
+    const JkSegs* sg = g.segs + ch.seg;
+    if (lds)
+      for (int e = tid; e < ROWS * n + na * nb; e += NT) dyn[e] = 0.0;
+    for (int e = tid; e < NW * NBLK; e += NT) (&blk[0][0])[e] = 0.0;
+    int seg = 0;
+    for (int kb = ch.k0; kb < ch.k1; kb += NW) {
+      const int k = kb + wave;
+      int kp = ch.bra, mode = 0;
+      if (k < ch.k1) {
+        kp = chunk_ket(g.kets, sg, k, seg);
+        mode = quartet_select(g.sel, ch.bra, bra, kp, g.pairs[kp]);
+      }
+      const PairRec ket = g.pairs[kp];
+      const int ntot = mode ? bra.npp_sig * ket.npp_sig : 0;
+      if (lane == 0) {
+        kinfo[wave][0] = ket.fa;
+        kinfo[wave][1] = ket.fb;
+        kinfo[wave][2] = (mode ? 1 : 0) | (kp == ch.bra ? 2 : 0);
+        kinfo[wave][3] = ntot;
+      }
+      __syncthreads();
+      int ntot_max = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) ntot_max = max(ntot_max, kinfo[w][3]);
+      double v[NJ][nd];
+      if (MIXED && mode == 2) {
+        float acc[NJ][nd];
+        wave_quartet_acc<La, Lb, Lc, Ld, float, NJ, NPQ>(g, bra, ket, ntot, ntot_max,
+                                                         reinterpret_cast<float*>(tab[wave]),
+                                                         reinterpret_cast<float*>(hs[wave]), lane, ofs, acc);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
+      } else {
+        double acc[NJ][nd];
+        wave_quartet_acc<La, Lb, Lc, Ld, double, NJ, NPQ>(g, bra, ket, ntot, ntot_max, tab[wave], hs[wave], lane,
+                                                          ofs, acc);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
+      }
+      if (mode) {
+        const int fc = ket.fa, fd = ket.fb;
+        const bool kl = ket.same, bk = (kp == ch.bra);
+        const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+        const double fK = bk ? 1.0 : 2.0;
+        double* bw = blk[wave];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) {
+            const int t = lane + 64 * j;
+            if (t >= NTASK) continue;
+            const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+            const double x = v[j][d] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
+            const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+            const double jv = fJ * x * D[la * n + si];
+            const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
+            const double k1 = fK * x * D[nu * n + si];
+            const double k2 = kl ? 0.0 : fK * x * D[nu * n + la];
+            const double k3 = ij ? 0.0 : fK * x * D[mu * n + si];
+            const double k4 = (ij || kl) ? 0.0 : fK * x * D[mu * n + la];
+            if (!bk) lds_add(&bw[OCD + c * nd + d], jw);
+            if (lds) {
+              lds_add(&jab[a * nb + b], jv);
+              lds_add(&krow[a * n + la], k1);
+              if (!kl) lds_add(&krow[a * n + si], k2);
+              if (!ij) lds_add(&krow[(na + b) * n + la], k3);
+              if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
+            } else {
+              lds_add(&bw[OAB + a * nb + b], jv);
+              lds_add(&bw[OAC + a * nc + c], k1);
+              if (!kl) lds_add(&bw[OAD + a * nd + d], k2);
+              if (!ij) lds_add(&bw[OBC + b * nc + c], k3);
+              if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], k4);
+            }
+          }
+      }
+      __syncthreads();
+      // the waves' quartet blocks to HBM, one atomic per element
+      for (int e = tid; e < NW * NBLK; e += NT) {
+        const int w = e / NBLK, o = e - w * NBLK;
+        const double x = blk[w][o];
+        if (x == 0.0) continue;
+        blk[w][o] = 0.0;
+        const size_t fc = size_t(kinfo[w][0]), fd = size_t(kinfo[w][1]);
+        size_t r, q;
+        double* M = Kt;
+        if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; M = Jt; }
+        else if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; M = Jt; }
+        else if (o < OAD) { r = fa + (o - OAC) / nc; q = fc + (o - OAC) % nc; }
+        else if (o < OBC) { r = fa + (o - OAD) / nd; q = fd + (o - OAD) % nd; }
+        else if (o < OBD) { r = fb + (o - OBC) / nc; q = fc + (o - OBC) % nc; }
+        else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
+        atomic_add(&M[r * n + q], x);
+      }
+      __syncthreads();
+    }
+    if (lds) {
+      for (int e = tid; e < ROWS * n; e += NT) {
+        const double x = krow[e];
+        if (x != 0.0) {
+          const int row = e / n, col = e - row * n;
+          atomic_add(&Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], x);
+        }
+      }
+      for (int e = tid; e < na * nb; e += NT) {
+        const double x = jab[e];
+        if (x != 0.0) atomic_add(&Jt[size_t(fa + e / nb) * n + fb + e % nb], x);
+      }
+      __syncthreads();  // dyn is cleared for the next chunk
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // Per-class execution policy
 // --------------------------------------------------------------------------
 template <int La, int Lb, int Lc, int Ld>
@@ -910,6 +1205,12 @@ struct ClassPolicy {
   static constexpr bool regs = rest <= 27 && NCH <= 6;
   static constexpr int NOUT = na * nb * nc * nd;
   static constexpr int NT = NOUT <= 64 * 24 ? 64 : (NOUT <= 128 * 24 ? 128 : 256);
+  // LDS-shape classes of at most 6 (a, b, c) tasks per lane of a wavefront
+  // run one quartet per wavefront, four wavefronts per workgroup
+  static constexpr bool waves = !regs && LdsTasks<La, Lb, Lc, Ld>::NTASK <= 6 * 64;
+  // four wavefronts, two where one wavefront's tables exceed 12 KB
+  static constexpr int NW =
+      (LdsShape<La, Lb, Lc, Ld, double>::TS + LdsShape<La, Lb, Lc, Ld, double>::HSS) * 8 <= 12 * 1024 ? 4 : 2;
 };
 
 }  // namespace lai

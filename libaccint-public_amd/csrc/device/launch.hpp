@@ -16,6 +16,7 @@ struct ClassInfo {
   int regs;    // 1: register shape (Kernel A), 0: LDS shape (Kernel B)
   int nchunk;  // Kernel A chunks along the bra index
   int nt;      // lanes per workgroup
+  int waves;   // LDS shape with one quartet per wavefront: wavefronts per workgroup, else 0
 };
 
 #define LAI_NPARTS_FIXED 8

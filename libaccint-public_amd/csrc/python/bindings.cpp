@@ -480,4 +480,66 @@ PYBIND11_MODULE(_core, m) {
             return "Engine(n_shells=" + std::to_string(e.basis().n_shells()) +
                    ", gpu_available=" + (e.gpu_available() ? std::string("True") : std::string("False")) + ")";
         });
+
+    py::class_<BatchEngine>(m, "BatchEngine")
+        .def(py::init([](const std::vector<std::shared_ptr<BasisSet>>& bases) {
+                 std::vector<std::shared_ptr<const BasisSet>> v(bases.begin(), bases.end());
+                 return std::make_unique<BatchEngine>(std::move(v));
+             }),
+             py::arg("bases"))
+        .def("n_molecules", &BatchEngine::n_molecules)
+        .def("n_basis_functions", [](const BatchEngine& e) { return e.n_basis_functions(); })
+        .def("matrix_offsets", [](const BatchEngine& e) { return e.matrix_offsets(); })
+        .def("matrix_size", &BatchEngine::matrix_size)
+        .def("precompute_schwarz_bounds", &BatchEngine::precompute_schwarz_bounds)
+        .def("prepare_worklist",
+             [](BatchEngine& e, const JKOptions& o, py::object D) {
+                 if (D.is_none()) return info_dict(e.prepare_worklist(o, nullptr));
+                 auto a = py::array_t<Real, py::array::c_style | py::array::forcecast>(D.cast<py::array>());
+                 if (Size(a.size()) != e.matrix_size())
+                     throw InvalidArgumentException("BatchEngine.prepare_worklist: D has the wrong length");
+                 return info_dict(e.prepare_worklist(o, a.data()));
+             },
+             py::arg("options") = JKOptions{}, py::arg("D") = py::none())
+        .def("worklist_info", [](const BatchEngine& e) { return info_dict(e.worklist_info()); })
+        .def("build_jk",
+             [](BatchEngine& e, py::array D, const JKOptions& o) {
+                 auto a = py::array_t<Real, py::array::c_style | py::array::forcecast>(D);
+                 if (Size(a.size()) != e.matrix_size())
+                     throw InvalidArgumentException("BatchEngine.build_jk: D has the wrong length");
+                 std::vector<Real> J(e.matrix_size()), K(e.matrix_size());
+                 e.build_jk_host(o, a.data(), J.data(), K.data());
+                 return py::make_tuple(py::array_t<Real>(J.size(), J.data()), py::array_t<Real>(K.size(), K.data()));
+             },
+             py::arg("D"), py::arg("options") = JKOptions{})
+        .def("build_jk_device",
+             [](BatchEngine& e, std::uintptr_t dD, std::uintptr_t dJ, std::uintptr_t dK, const JKOptions& o,
+                std::uintptr_t stream, py::object Dh) {
+                 const Real* dh = nullptr;
+                 py::array_t<Real, py::array::c_style | py::array::forcecast> keep;
+                 if (!Dh.is_none()) {
+                     keep = py::array_t<Real, py::array::c_style | py::array::forcecast>(Dh.cast<py::array>());
+                     if (Size(keep.size()) != e.matrix_size())
+                         throw InvalidArgumentException("BatchEngine.build_jk_device: D has the wrong length");
+                     dh = keep.data();
+                 }
+                 e.build_jk(o, reinterpret_cast<const Real*>(dD), reinterpret_cast<Real*>(dJ),
+                            reinterpret_cast<Real*>(dK), reinterpret_cast<hipStream_t>(stream), dh);
+             },
+             py::arg("d_D"), py::arg("d_J"), py::arg("d_K"), py::arg("options") = JKOptions{},
+             py::arg("stream") = std::uintptr_t(0), py::arg("D_host") = py::none())
+        .def("capture_jk_graph",
+             [](BatchEngine& e, std::uintptr_t dD, std::uintptr_t dJ, std::uintptr_t dK, const JKOptions& o,
+                std::uintptr_t stream) {
+                 e.capture_jk_graph(o, reinterpret_cast<const Real*>(dD), reinterpret_cast<Real*>(dJ),
+                                    reinterpret_cast<Real*>(dK), reinterpret_cast<hipStream_t>(stream));
+             },
+             py::arg("d_D"), py::arg("d_J"), py::arg("d_K"), py::arg("options"), py::arg("stream"))
+        .def("launch_jk_graph",
+             [](BatchEngine& e, std::uintptr_t s) { e.launch_jk_graph(reinterpret_cast<hipStream_t>(s)); },
+             py::arg("stream"))
+        .def("device_basis_handle", [](BatchEngine& e) { return reinterpret_cast<std::uintptr_t>(e.device_handle()); })
+        .def("__repr__", [](const BatchEngine& e) {
+            return "BatchEngine(n_molecules=" + std::to_string(e.n_molecules()) + ")";
+        });
 }

@@ -364,3 +364,51 @@ def test_rhf_energies_vs_reference_literals():
         r = rhf(atoms, "sto-3g", nel)
         assert r.converged and r.iterations < 50
         assert abs(r.energy - ref) < 1e-10, (r.energy, ref)
+
+
+# ----------------------------------------------------- batches of molecules
+def test_batch_engine_matches_single_engines():
+    """Three different molecules through one BatchEngine: every block equals the
+    single-molecule build, quartet counts and list checksums add up, for FP64,
+    the mixed-precision split and a replayed hipGraph."""
+    import torch
+    mols = [("sto-3g", systems.h2o_sto3g_geometry()),
+            ("cc-pvdz", systems.water_cluster(2, 8.0, seed=3)),
+            ("def2-svp", systems.random_molecule(np.random.default_rng(7), 4, 6))]
+    bases = [lai.basis_set(name, atoms) for name, atoms in mols]
+    be = lai.BatchEngine(bases)
+    nbf, off = be.n_basis_functions(), be.matrix_offsets()
+    assert off[-1] == sum(n * n for n in nbf)
+    Ds = [systems.density(n, max(2, n // 5), 11 + m, 1.0 / np.sqrt(n)) for m, n in enumerate(nbf)]
+    Db = np.concatenate([D.ravel() for D in Ds])
+    for opts in (lai.JKOptions(schwarz_threshold=1e-12),
+                 lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)):
+        Jb, Kb = be.build_jk(Db, opts)
+        info = be.worklist_info()
+        nq, cs = 0, 0
+        for m, ((name, atoms), D) in enumerate(zip(mols, Ds)):
+            b, e = engine_for(name, atoms)
+            J, K = e.build_jk(D, opts)
+            wi = e.worklist_info()
+            nq += wi["n_quartets"]
+            cs = (cs + wi["checksum"]) % (1 << 64)
+            n = nbf[m]
+            assert rel(Jb[off[m]:off[m + 1]].reshape(n, n), J) < 1e-12
+            assert rel(Kb[off[m]:off[m + 1]].reshape(n, n), K) < 1e-12
+        assert info["n_quartets"] == nq and info["checksum"] == cs
+    # graph replays of the batched FP64 build
+    opts = lai.JKOptions(schwarz_threshold=1e-12)
+    Jref, Kref = be.build_jk(Db, opts)
+    s = torch.cuda.Stream()
+    tD = torch.from_numpy(Db).cuda()
+    tJ = torch.zeros_like(tD)
+    tK = torch.zeros_like(tD)
+    torch.cuda.synchronize()
+    be.capture_jk_graph(tD.data_ptr(), tJ.data_ptr(), tK.data_ptr(), opts, s.cuda_stream)
+    for rep in range(3):
+        tJ.zero_()
+        tK.zero_()
+        torch.cuda.synchronize()
+        be.launch_jk_graph(s.cuda_stream)
+        torch.cuda.synchronize()
+        assert rel(tJ.cpu().numpy(), Jref) < 1e-12 and rel(tK.cpu().numpy(), Kref) < 1e-12, rep
