@@ -37,6 +37,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def traffic_of(cls):
+    """HBM bytes per launch of the class kernel from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs (profiles/), or None when that class was not measured."""
+    path = os.path.join(REPO, "profiles", "eri_jk_16water_ccpvdz_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    import re
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch (\(\w\w\|\w\w\)) = ([0-9.eE+-]+)", line.strip())
+        if m and m.group(1) == cls:
+            return float(m.group(2))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +59,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="end with a device fill and copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--config4-full", action="store_true",
+                    help="time all 64 molecules of the batched workload (several minutes per build)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -218,6 +237,76 @@ def main():
             (ms_d, ms_m, ms_d / ms_m, Em - Ed, mixed["mixed_fp32_quartet_fraction"]))
         del meng, tmD, tmJ, tmK
 
+    # -------------------------------------------------------------- batched
+    # 64 seeded molecules (3-60 atoms of H/C/N/O, random-walk bonding) in
+    # cc-pVTZ through one BatchEngine; by default the molecules with at most
+    # 20 atoms (one full build of all 64 takes minutes on one GPU)
+    batch = {}
+    if not args.no_batch:
+        rng = np.random.default_rng(4)
+        mols = [systems.random_molecule(rng) for _ in range(64)]
+        sel = list(range(64)) if args.config4_full else [k for k, m in enumerate(mols) if len(m) <= 20]
+        bases = [lai.basis_set("cc-pvtz", mols[k]) for k in sel]
+        be = lai.BatchEngine(bases)
+        bnbf = be.n_basis_functions()
+        bD = np.concatenate([systems.density(nb_, max(2, nb_ // 5), 200 + k, 1.0 / math.sqrt(nb_)).ravel()
+                             for k, nb_ in zip(sel, bnbf)])
+        t0 = time.perf_counter()
+        bo = lai.JKOptions(schwarz_threshold=1e-12)
+        binfo = be.prepare_worklist(bo)
+        bsetup = time.perf_counter() - t0
+        tbD = torch.from_numpy(bD).to(dev)
+        tbJ = torch.zeros_like(tbD)
+        tbK = torch.zeros_like(tbD)
+
+        def b_timed(o, steps):
+            be.capture_jk_graph(tbD.data_ptr(), tbJ.data_ptr(), tbK.data_ptr(), o, stream.cuda_stream)
+            be.launch_jk_graph(stream.cuda_stream)
+            torch.cuda.synchronize()
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(steps):
+                be.launch_jk_graph(stream.cuda_stream)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1) / steps
+
+        bsteps = 1 if args.config4_full else 3
+        ms_b = b_timed(bo, bsteps)
+        bm = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
+        be.prepare_worklist(bm, bD)
+        bminfo = be.worklist_info()
+        ms_bm = b_timed(bm, bsteps)
+        # the same molecules one engine at a time (no batching), FP64
+        engines = [lai.Engine(bb) for bb in bases]
+        offs = be.matrix_offsets()
+        for e_ in engines:
+            e_.prepare_worklist(bo)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for m_, e_ in enumerate(engines):
+            a_, z_ = offs[m_], offs[m_ + 1]
+            e_.build_jk_device(tbD[a_:z_].data_ptr(), tbJ[a_:z_].data_ptr(), tbK[a_:z_].data_ptr(), bo,
+                               stream.cuda_stream, None)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms_seq = ev0.elapsed_time(ev1)
+        batch = {"batch_workload": "%d of the 64 seeded H/C/N/O molecules in cc-pVTZ (%s), nbf %d..%d, "
+                                   "Schwarz 1e-12, %d quartets, %.3e ERIs" %
+                                   (len(sel), "all" if args.config4_full else "those with <= 20 atoms",
+                                    min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
+                 "batch_setup_seconds": bsetup,
+                 "batch_jk_seconds_fp64": ms_b * 1e-3, "batch_eris_per_s_fp64": binfo["n_eri"] / (ms_b * 1e-3),
+                 "batch_jk_seconds_mixed": ms_bm * 1e-3,
+                 "batch_mixed_fp32_quartet_fraction": bminfo["n_fp32"] / max(bminfo["n_quartets"], 1),
+                 "batch_one_engine_per_molecule_seconds_fp64": ms_seq * 1e-3}
+        log("batch (%d molecules): setup %.2f s, FP64 %.3f ms, mixed %.3f ms, one engine per molecule %.3f ms" %
+            (len(sel), bsetup, ms_b, ms_bm, ms_seq))
+        del be, engines, tbD, tbJ, tbK
+
     # ------------------------------------------------------------- gradient
     grad = {}
     if not args.no_grad:
@@ -282,14 +371,21 @@ def main():
         "flops_executed": exec_total,
         "flops_all_nonzero_primitives": flops_total,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_of(cls_name(dom)),
                      "kernel": "class %s ERI/J-K kernel" % cls_name(dom),
                      "kernel_ms": float(ms_cls[dom])},
         "cpu_baseline": cpu,
     }
     out.update(mixed)
+    out.update(batch)
     out.update(grad)
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mb > 0:  # fill writes N bytes; copy reads N and writes N
+        x = torch.empty(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        x.fill_(1.0)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":
