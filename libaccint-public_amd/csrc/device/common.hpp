@@ -191,17 +191,44 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
       }
     }
   } else {
-    const double it = 1.0 / T;
-    const double isq = sqrt(it);
+    // 1/sqrt(T) by v_rsq_f64 and two Newton steps; 1/T is its square
+    double y = __builtin_amdgcn_rsq(T);
+    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
+    const double it = y * y;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       u[i] = lai_rys_hx2[lai_rys_hoff[N] + i] * it;
-      w[i] = lai_rys_hw[lai_rys_hoff[N] + i] * isq;
+      w[i] = lai_rys_hw[lai_rys_hoff[N] + i] * y;
     }
   }
 }
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// Read-only tables seen through the constant address space: loads with a
+// workgroup-uniform address become scalar loads.
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ const_ptr<T> as_const(const T* p) {
+  return (const_ptr<T>)p;
+}
+// primitive pair i of a read-only table, member by member through the
+// constant address space
+__device__ __forceinline__ PrimPair load_prim(const PrimPair* base, int i) {
+  const_ptr<double> p = as_const(reinterpret_cast<const double*>(base + i));
+  PrimPair r;
+  r.zeta = p[0];
+  r.Px = p[1];
+  r.Py = p[2];
+  r.Pz = p[3];
+  r.PAx = p[4];
+  r.PAy = p[5];
+  r.PAz = p[6];
+  r.Kc = p[7];
+  return r;
+}
 
 // FP64 add to LDS (ds_add_f64)
 __device__ __forceinline__ void lds_add(double* p, double v) {

@@ -219,10 +219,11 @@ __device__ __forceinline__ void quartet_block_regs(
     const double eta = Q.zeta;
     const double h_e = g.ppih[ket.pp_off + ik];  // 1 / (2 eta)
     for (int ib = 0; ib < nb_pp; ++ib) {
-      if (fp32 && g.ppb[bra.pp_off + ib] * bk < qcut) break;
-      const PrimPair P = g.pp[bra.pp_off + ib];
+      if (fp32 && as_const(g.ppb)[bra.pp_off + ib] * bk < qcut) break;
+      // the bra is uniform in the J/K kernels: scalar loads
+      const PrimPair P = load_prim(g.pp, bra.pp_off + ib);
       const double zeta = P.zeta;
-      const double h_z = g.ppih[bra.pp_off + ib];  // 1 / (2 zeta)
+      const double h_z = as_const(g.ppih)[bra.pp_off + ib];  // 1 / (2 zeta)
       // 1/sqrt(zeta + eta) by v_rsq_f64 and two Newton steps; 1/(zeta + eta) = its square
       const double x = zeta + eta;
       double y = __builtin_amdgcn_rsq(x);
