@@ -263,9 +263,11 @@ __device__ __forceinline__ void grad_quartet_regs(const GradArgs& g, const PairR
     const double eta = Q.zeta, h_e = g.ppih[ket.pp_off + ik];
     const double tc2 = 2.0 * ek.x;
     for (int ib = 0; ib < bra.npp_sig; ++ib) {
-      const PrimPair P = g.pp[bra.pp_off + ib];
-      const double2 eb = g.ppab[bra.pp_off + ib];
-      const double zeta = P.zeta, h_z = g.ppih[bra.pp_off + ib];
+      // the bra is uniform over the workgroup: scalar loads
+      const PrimPair P = load_prim(g.pp, bra.pp_off + ib);
+      const double2 eb = make_double2(as_const(reinterpret_cast<const double*>(g.ppab))[2 * (bra.pp_off + ib)],
+                                      as_const(reinterpret_cast<const double*>(g.ppab))[2 * (bra.pp_off + ib) + 1]);
+      const double zeta = P.zeta, h_z = as_const(g.ppih)[bra.pp_off + ib];
       const double ta2 = 2.0 * eb.x, tb2 = 2.0 * eb.y;
       const double x = zeta + eta;
       double y = __builtin_amdgcn_rsq(x);
