@@ -76,6 +76,85 @@ __device__ __forceinline__ int chunk_ket(const int* __restrict__ kets, const JkS
   return kets[sg->start[seg] + (k - sg->cum[seg])];
 }
 
+// Lane schedule of the lane-per-quartet kernels (256-lane workgroups).  The
+// quartets of a chunk that pass the per-quartet selection are counting-sorted
+// in LDS by (FP32 flag, primitive pairs of the ket), heaviest first, so that
+// the 64 lanes of a wavefront run primitive loops of nearly equal length and
+// one precision; quartets removed by the density screen take no lane.
+constexpr int kSortRun = 2048;                 // quartets sorted at a time
+constexpr int kSortBins = 256;                 // key: 128 * FP32 + min(npp_sig, 127)
+constexpr int kSortPer = kSortRun / 256;       // positions per lane
+constexpr unsigned kSortF32 = 0x80000000u;     // flag bit of a sorted entry
+struct SortLds {
+  unsigned kp[kSortRun];  // ket pair, kSortF32 set for the FP32 recursion
+  int cnt[kSortBins];
+  int n;
+};
+
+// Sorts positions [s0, s1) (at most kSortRun) of a chunk's run into s.kp and
+// returns how many were kept.  All 256 lanes call it; it ends on a barrier.
+__device__ __forceinline__ int sort_run(const int* __restrict__ kets, const JkSegs* __restrict__ sg,
+                                        const PairRec* __restrict__ pairs, const QuartetSel& sel, int bp,
+                                        const PairRec& bra, int s0, int s1, SortLds& s) {
+  const int tid = threadIdx.x;
+  s.cnt[tid] = 0;  // kSortBins == 256 lanes
+  __syncthreads();
+  int kp[kSortPer], key[kSortPer], rk[kSortPer];
+  int seg = 0;
+#pragma unroll
+  for (int j = 0; j < kSortPer; ++j) {
+    const int k = s0 + tid + 256 * j;
+    key[j] = -1;
+    kp[j] = 0;
+    rk[j] = 0;
+    if (k < s1) {
+      kp[j] = chunk_ket(kets, sg, k, seg);
+      const PairRec ket = pairs[kp[j]];
+      const int mode = quartet_select(sel, bp, bra, kp[j], ket);
+      if (mode != 0) {
+        key[j] = (mode == 2 ? 128 : 0) + min(ket.npp_sig, 127);
+        rk[j] = atomicAdd(&s.cnt[key[j]], 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < 64) {  // offsets in decreasing key order; lane l owns keys 255-4l .. 252-4l
+    int c[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c[q] = s.cnt[255 - 4 * tid - q];
+      sum += c[q];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off, 64);
+      if (tid >= off) incl += t;
+    }
+    int run = incl - sum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      s.cnt[255 - 4 * tid - q] = run;
+      run += c[q];
+    }
+    if (tid == 63) s.n = incl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortPer; ++j)
+    if (key[j] >= 0) s.kp[s.cnt[key[j]] + rk[j]] = unsigned(kp[j]) | (key[j] >= 128 ? kSortF32 : 0u);
+  __syncthreads();
+  return s.n;
+}
+
+// Sorted position of lane tid in round r: the four wavefronts take the four
+// 64-entry blocks of a round in an order that rotates with r, so that no
+// wavefront always gets the heaviest block.
+__device__ __forceinline__ int sorted_pos(int tid, int r) {
+  const int w = tid >> 6;
+  return 64 * (4 * r + ((w + r) & 3)) + (tid & 63);
+}
+
 struct EriArgs {
   const Quartet* __restrict__ q;
   int nq;
@@ -629,10 +708,11 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
 // per chunk; the bra Coulomb block Jt[ab] is summed over the chunk as well.
 // Only the ket-side Coulomb update Jt[cd] goes to HBM per quartet.  With
 // klds == 0 (rows too long for LDS) the exchange updates go straight to HBM.
-// MIXED: quartets selected for FP32 run the recursion in float.
+// MIXED: quartets selected for FP32 run the recursion in float.  Lanes take
+// the chunk's quartets in the order of sort_run.
 // --------------------------------------------------------------------------
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int A0, int A1>
-__device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow) {
+__device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt) {
   constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NA = A1 - A0;
   constexpr int ROWS = NA + nb;
@@ -663,12 +743,15 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
 #pragma unroll
     for (int b = 0; b < nb; ++b) jab[a][b] = 0.0;
 
-  int seg = 0;
-  for (int k = ch.k0 + tid; k < ch.k1; k += nt) {
-    const int kp = chunk_ket(g.kets, sg, k, seg);
+  for (int s0 = ch.k0; s0 < ch.k1; s0 += kSortRun) {
+  const int nsel = sort_run(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt);
+  for (int r = 0; 256 * r < nsel; ++r) {
+    const int i = sorted_pos(tid, r);
+    if (i >= nsel) continue;
+    const unsigned e = srt.kp[i];
+    const int kp = int(e & ~kSortF32);
+    const int mode = (e & kSortF32) ? 2 : 1;
     const PairRec ket = g.pairs[kp];
-    const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
-    if (mode == 0) continue;
     double gv[NA][nb][nc][nd];
     if (MIXED && mode == 2)
       quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
@@ -755,6 +838,7 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
         }
     }
   }
+  }
   // Jt[ab]: one atomic per wavefront and element
 #pragma unroll
   for (int a = 0; a < NA; ++a)
@@ -777,14 +861,15 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
 }
 
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
-__device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChunk& ch, double* krow, int chunk) {
+__device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
+                                                  int chunk) {
   constexpr int na = ncart(La);
   constexpr int A0 = C * CHA;
   constexpr int A1 = (A0 + CHA < na) ? A0 + CHA : na;
   if (chunk == C) {
-    jk_chunk_regs<La, Lb, Lc, Ld, MIXED, A0, A1>(g, ch, krow);
+    jk_chunk_regs<La, Lb, Lc, Ld, MIXED, A0, A1>(g, ch, krow, srt);
   } else if constexpr (A1 < na) {
-    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, C + 1>(g, ch, krow, chunk);
+    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, C + 1>(g, ch, krow, srt, chunk);
   }
 }
 
@@ -792,9 +877,10 @@ __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChun
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
 __global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
   extern __shared__ double krow[];
+  __shared__ SortLds srt;
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
     const JkChunk ch = g.chunks[c];
-    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, blockIdx.y);
+    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, srt, blockIdx.y);
   }
 }
 

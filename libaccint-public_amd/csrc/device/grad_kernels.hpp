@@ -336,6 +336,7 @@ __device__ __forceinline__ void grad_quartet_regs(const GradArgs& g, const PairR
 template <int La, int Lb, int Lc, int Ld>
 __global__ __launch_bounds__(256) void grad_kernel_regs(GradArgs g) {
   extern __shared__ double gacc[];
+  __shared__ SortLds srt;  // lane schedule of the chunk (sort_run)
   const int tid = threadIdx.x;
   const bool glds = g.glds != 0;
   if (glds) {
@@ -354,11 +355,13 @@ __global__ __launch_bounds__(256) void grad_kernel_regs(GradArgs g) {
     const int iA = 3 * g.shell_atom[bra.sa], iB = 3 * g.shell_atom[bra.sb];
     double fab[6] = {0, 0, 0, 0, 0, 0};
     const JkSegs* sg = g.segs + ch.seg;
-    int seg = 0;
-    for (int k = ch.k0 + tid; k < ch.k1; k += blockDim.x) {
-      const int kp = chunk_ket(g.kets, sg, k, seg);
+    for (int s0 = ch.k0; s0 < ch.k1; s0 += kSortRun) {
+    const int nsel = sort_run(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt);
+    for (int r = 0; 256 * r < nsel; ++r) {
+      const int i = sorted_pos(tid, r);
+      if (i >= nsel) continue;
+      const int kp = int(srt.kp[i] & ~kSortF32);
       const PairRec ket = g.pairs[kp];
-      if (quartet_select(g.sel, ch.bra, bra, kp, ket) == 0) continue;
       double f[9];
       grad_quartet_regs<La, Lb, Lc, Ld>(g, bra, ket, kp == ch.bra, f);
 #pragma unroll
@@ -369,6 +372,7 @@ __global__ __launch_bounds__(256) void grad_kernel_regs(GradArgs g) {
         add(iC + x, f[6 + x]);
         add(iD + x, -(f[x] + f[3 + x] + f[6 + x]));  // translational invariance
       }
+    }
     }
 #pragma unroll
     for (int x = 0; x < 6; ++x) {
