@@ -179,6 +179,48 @@ int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, doubl
                              void* stream);
 int lai_dev_jk_graph_launch(lai_basis* b, void* stream);
 
+/* ---- dispatch table and autotuner ---------------------------------------------
+ * One entry per quartet class (class index as in lai_dev_worklist_stats): the
+ * execution strategy of its J/K kernel -- LAI_STRATEGY_THREAD (a lane per
+ * quartet, register tables), LAI_STRATEGY_WAVEFRONT (a wavefront per quartet)
+ * or LAI_STRATEGY_BLOCK (a workgroup per quartet, LDS tables) -- the quartets
+ * per chunk (a chunk is one bra pair with a run of its kets, taken by one
+ * workgroup; 0 = the strategy's default rule) and the grid cap in workgroups
+ * per CU (0 = the strategy's default).  Mirrors
+ * OptimalDispatchTable / TwoElectronEntry::gpu_strategy
+ * (include/libaccint/kernels/optimal_dispatch_table.hpp:29-33,64-67,83-149),
+ * which the reference fills offline from bench_optimal_dispatch. */
+#define LAI_STRATEGY_THREAD 0
+#define LAI_STRATEGY_WAVEFRONT 1
+#define LAI_STRATEGY_BLOCK 2
+typedef struct lai_class_tune {
+  int cls;
+  int strategy;
+  int chunk;
+  int blocks_per_cu;
+  double ms;         /* lai_dev_autotune: milliseconds per launch of the choice, else 0 */
+  double ms_default; /* lai_dev_autotune: milliseconds per launch of the default, else 0 */
+} lai_class_tune;
+
+/* The strategy compiled as a class's default and its alternative (-1: none). */
+int lai_dev_class_strategies(int cls, int* strategy, int* alternative);
+/* Times candidate launches of every class of the current worklist on d_D --
+ * each compiled strategy with several chunk caps, then grid caps for the best
+ * -- reps launches each after one warm-up, and keeps per class the fastest
+ * (the default unless a candidate is at least 1% faster).  Writes one entry
+ * per class with work (at most cap) and their number to n_out.  Synchronises
+ * the stream; drops a captured graph.  Takes the place of the reference's
+ * offline bench_optimal_dispatch run (tests/benchmark/bench_optimal_dispatch.cpp)
+ * and LIBACCINT_DISPATCH_TABLE loading (src/device/cuda/engine/cuda_engine.cu:201-206). */
+int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int reps, lai_class_tune* table, int cap,
+                     int* n_out, void* stream);
+/* The current choice of the 55 classes (ms fields 0); n_out = 55. */
+int lai_dev_get_tuning(const lai_basis* b, lai_class_tune* table, int cap, int* n_out);
+/* Applies n entries (validated first; a strategy not compiled for the class
+ * is LAI_ERR_UNSUPPORTED); n = 0 restores the defaults of every class.  The
+ * choice holds across later worklist builds; drops a captured graph. */
+int lai_dev_set_tuning(lai_basis* b, const lai_class_tune* table, int n);
+
 /* ---- nuclear gradient (new capability; the reference has no derivative
  * integrals: every DerivOrder is static_assert(== 0), engine.hpp:169-170) ---
  * atom_of_shell[n_shells] maps shells to atoms for the gradient. */

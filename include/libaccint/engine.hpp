@@ -27,6 +27,8 @@
 #include <array>
 #include <memory>
 #include <span>
+#include <string>
+#include <string_view>
 #include <type_traits>
 #include <vector>
 
@@ -50,6 +52,43 @@ struct WorklistInfo {
     Real flops_fp64{0}, flops_fp32{0};
     Index n_eri{0};
     std::array<Index, 55> class_counts{};
+};
+
+// Execution strategy of a quartet class's J/K kernel, as in the reference's
+// dispatch table (include/libaccint/kernels/optimal_dispatch_table.hpp:29-56);
+// a warp is a 64-lane wavefront here.
+enum class GpuExecutionStrategy : int { ThreadPerQuartet = 0, WarpPerQuartet = 1, BlockPerQuartet = 2 };
+[[nodiscard]] std::string_view to_string(GpuExecutionStrategy s) noexcept;
+// throws std::invalid_argument for an unknown name
+[[nodiscard]] GpuExecutionStrategy gpu_execution_strategy_from_string(std::string_view s);
+
+// Quartet class index c = pb (pb + 1) / 2 + pk of canonical angular momenta
+// (la >= lb, lc >= ld, pair class p = la (la + 1) / 2 + lb, pb >= pk; 55
+// classes up to f) and back; -1 for a non-canonical or out-of-range quartet.
+[[nodiscard]] int quartet_class_index(const std::array<int, 4>& am) noexcept;
+[[nodiscard]] std::array<int, 4> quartet_class_am(int cls);
+
+// One quartet class of the dispatch table.
+struct DispatchEntry {
+    int cls{0};  // class index (the 55-entry numbering of WorklistInfo::class_counts)
+    std::array<int, 4> am{};
+    GpuExecutionStrategy strategy{GpuExecutionStrategy::ThreadPerQuartet};
+    int chunk{0};          // quartets per chunk; 0 = the strategy's default rule
+    int blocks_per_cu{0};  // grid cap; 0 = the strategy's default
+    Real ms{0};            // autotune: milliseconds per launch of this choice
+    Real ms_default{0};    // autotune: milliseconds per launch of the default
+};
+
+// The device build's counterpart of OptimalDispatchTable
+// (optimal_dispatch_table.hpp:83-149): per-class kernel choices, produced by
+// Engine::autotune or read from JSON; LIBACCINT_DISPATCH_TABLE names a file
+// applied when an Engine creates its device basis (cuda_engine.cu:199-207).
+struct DispatchTable {
+    std::vector<DispatchEntry> entries;
+    [[nodiscard]] static DispatchTable from_json(const std::string& path);
+    void to_json(const std::string& path) const;
+    [[nodiscard]] static DispatchTable parse_json(const std::string& text);
+    [[nodiscard]] std::string dump_json() const;
 };
 
 class Engine {
@@ -196,6 +235,19 @@ public:
     [[nodiscard]] std::vector<Real> compute_jk_gradient(const Real* D, Real exchange_scale = 1.0,
                                                         int n_atoms = -1);
 
+    // ---- dispatch table and autotuner ------------------------------------------
+    // Times the compiled kernel strategies, chunk caps and grid caps of every
+    // class of the worklist for these options on d_D and keeps the fastest per
+    // class; returns the classes with work.  With
+    // DispatchConfig::enable_auto_tuning, build_jk does this once by itself
+    // (the reference's ProfileOnce mode, dispatch_policy.hpp:75-93).
+    DispatchTable autotune(const JKOptions& opts, const Real* d_D, hipStream_t stream = nullptr, int reps = 3,
+                           const Real* D_host = nullptr);
+    // the current choice of all 55 classes
+    [[nodiscard]] DispatchTable dispatch_table();
+    // applies the entries (an empty table restores the defaults)
+    void set_dispatch_table(const DispatchTable& table);
+
     // backend handle (for advanced callers)
     [[nodiscard]] lai_basis* device_basis();
 
@@ -279,6 +331,7 @@ private:
     JKOptions wl_opts_{};
     WorklistInfo wl_info_{};
     std::vector<Real> wl_dmax_;
+    int tuned_flags_{-1};  // precision flags of the last autotune, -1: none
 };
 
 // Several independent molecules through one device basis handle: each J/K
@@ -311,6 +364,12 @@ public:
     void build_jk_host(const JKOptions& opts, const Real* D, Real* J, Real* K);
     void capture_jk_graph(const JKOptions& opts, const Real* d_D, Real* d_J, Real* d_K, hipStream_t stream);
     void launch_jk_graph(hipStream_t stream);
+    // dispatch table of the batch (as Engine::autotune / dispatch_table /
+    // set_dispatch_table, over the classes of every molecule)
+    DispatchTable autotune(const JKOptions& opts, const Real* d_D, hipStream_t stream = nullptr, int reps = 3,
+                           const Real* D_host = nullptr);
+    [[nodiscard]] DispatchTable dispatch_table();
+    void set_dispatch_table(const DispatchTable& table);
     [[nodiscard]] lai_basis* device_handle();
 
 private:

@@ -93,6 +93,7 @@ def build_host(incremental=False):
     hdrs = []
     for root, _, files in os.walk(os.path.join(INC, "libaccint")):
         hdrs += [os.path.join(root, f) for f in files]
+    hdrs += [os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".hpp")]
     if incremental and not _stale(lib, srcs + hdrs + [os.path.join(INC, "lai_device.h")]):
         return lib
     _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-I" + INC, "-o", lib] + srcs +

@@ -170,6 +170,23 @@ struct lai_ctx {
 struct ClassList {
   DevBuf<JkChunk> chunks;  // bra-anchored chunks over the ket segments
   int64_t nchunks = 0;
+  // one entry per (bra, segment list): k0 = 0, k1 = its length; chunks are
+  // cut from these
+  std::vector<JkChunk> runs;
+  // chunks of the default rule when the tuned ones differ (the gradient
+  // kernels are sized for those)
+  DevBuf<JkChunk> gchunks;
+  int64_t ngchunks = 0;
+};
+
+// Launch choice of one class (autotuner or caller table): alt selects the
+// class's alternative kernel shape, chunk caps the quartets per chunk (0: the
+// shape's default rule), bpc the workgroups per CU (0: the shape's default).
+struct ClassTune {
+  int alt = 0;
+  int chunk = 0;
+  int bpc = 0;
+  bool is_default() const { return alt == 0 && chunk == 0 && bpc == 0; }
 };
 
 struct lai_basis {
@@ -226,6 +243,8 @@ struct lai_basis {
   uint64_t checksum = 0;
   // J/K scratch (unsymmetrised)
   DevBuf<double> Jt, Kt;
+  // per-class launch choice
+  ClassTune tune[kNumClasses];
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -551,7 +570,10 @@ extern "C" int lai_dev_basis_free(lai_basis* b) {
   b->dcoefs.release();
   b->Jt.release();
   b->Kt.release();
-  for (auto& c : b->cls) c.chunks.release();
+  for (auto& c : b->cls) {
+    c.chunks.release();
+    c.gchunks.release();
+  }
   b->segs.release();
   b->kets.release();
   b->sdmax.release();
@@ -646,7 +668,9 @@ int run_eri_batch(lai_basis* b, EriBatch& eb, double* d_out, hipStream_t st) {
     a.perm = dp.p;
     a.obase = dob.p;
     a.nbf = b->max_nbf;
-    Ls.f64[c](a, st, b->ctx->n_cu * 16);
+    LaunchTune t;
+    t.n_cu = b->ctx->n_cu;
+    Ls.f64[c](a, st, t);
     LAI_HIP(hipGetLastError());
     LAI_HIP(hipStreamSynchronize(st));
   }
@@ -746,6 +770,62 @@ extern "C" uint64_t lai_quartet_mix64(int i, int j, int k, int l) {
                        (uint64_t(uint16_t(k)) << 16) | uint64_t(uint16_t(l));
   return splitmix64(key);
 }
+
+namespace {
+// Chunk cap of the default rule for a class run in shape `shape` with n
+// quartets: lane-per-quartet kernels take up to 2048 quartets per workgroup
+// (8 per lane), wavefront-per-quartet kernels up to 64, workgroup-per-quartet
+// kernels up to 32; small classes get shorter chunks so that a launch still
+// has a few workgroups per CU.
+int default_chunk(int shape, size_t n, int n_cu) {
+  const size_t q = n / (4 * size_t(n_cu));
+  if (shape == kShapeRegs) return int(std::clamp<size_t>(q, 256, 2048));
+  if (shape == kShapeWaves) return int(std::clamp<size_t>(q, 4, 64));
+  return int(std::clamp<size_t>(q, 1, 32));
+}
+
+// Cuts each run into near-equal chunks of at most cmax quartets.
+std::vector<JkChunk> cut_runs(const std::vector<JkChunk>& runs, int cmax) {
+  std::vector<JkChunk> C;
+  for (const auto& r : runs) {
+    const int len = r.k1;
+    const int pieces = (len + cmax - 1) / cmax;
+    const int per = (len + pieces - 1) / pieces;
+    for (int k0 = 0; k0 < len; k0 += per) C.push_back(JkChunk{r.bra, r.seg, k0, std::min(len, k0 + per)});
+  }
+  return C;
+}
+
+int class_shape(const ClassInfo& info, const ClassTune& t) {
+  return (t.alt && info.alt != kShapeNone) ? info.alt : info.shape;
+}
+
+// Chunks of class c under its launch choice (and, when that is not the
+// default, the default chunks for the gradient kernels) on the device.
+int upload_chunks(lai_basis* b, int c) {
+  const auto& Ls = launchers();
+  ClassList& C = b->cls[c];
+  size_t n = 0;
+  for (const auto& r : C.runs) n += size_t(r.k1);
+  const ClassTune& t = b->tune[c];
+  const int shape = class_shape(Ls.info[c], t);
+  const int cmax = t.chunk > 0 ? t.chunk : default_chunk(shape, n, b->ctx->n_cu);
+  const std::vector<JkChunk> ch = cut_runs(C.runs, cmax);
+  hipError_t e;
+  if ((e = C.chunks.assign(ch.data(), ch.size())) != hipSuccess)
+    return fail(LAI_ERR_MEMORY, std::string("chunk upload: ") + hipGetErrorString(e));
+  C.nchunks = int64_t(ch.size());
+  C.gchunks.release();
+  C.ngchunks = 0;
+  if (!t.is_default()) {
+    const std::vector<JkChunk> g = cut_runs(C.runs, default_chunk(Ls.info[c].shape, n, b->ctx->n_cu));
+    if ((e = C.gchunks.assign(g.data(), g.size())) != hipSuccess)
+      return fail(LAI_ERR_MEMORY, std::string("chunk upload: ") + hipGetErrorString(e));
+    C.ngchunks = int64_t(g.size());
+  }
+  return LAI_OK;
+}
+}  // namespace
 
 extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* shell_dmax, int density_screen,
                                       double tau, int64_t* n_out, int64_t* n32_out, uint64_t* checksum) {
@@ -851,29 +931,17 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     etot[c] = tot;
   }, 1);
 
-  // Chunks: register-shape classes take up to 2048 quartets per workgroup
-  // (8 per lane), wavefront-per-quartet classes up to 64, the other LDS-shape
-  // classes up to 32; small classes get shorter chunks so that a launch still
-  // has a few workgroups per CU.
-  const int n_cu = b->ctx->n_cu;
+  // Chunks: one run per (bra, segment list), cut by the class's chunk rule
   std::vector<JkSegs> segs_h;
-  std::vector<std::vector<JkChunk>> C(kNumClasses);
   for (int c = 0; c < kNumClasses; ++c) {
-    const size_t n = size_t(etot[c]);
-    const int cmax = Ls.info[c].regs    ? int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 256, 2048))
-                     : Ls.info[c].waves ? int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 4, 64))
-                                        : int(std::clamp<size_t>(n / (4 * size_t(n_cu)), 1, 32));
+    auto& R = b->cls[c].runs;
+    R.clear();
     for (const auto& e : E[c]) {
-      const int si = int(segs_h.size());
+      R.push_back(JkChunk{e.bra, int(segs_h.size()), 0, e.sg.cum[e.sg.n]});
       segs_h.push_back(e.sg);
-      const int len = e.sg.cum[e.sg.n];
-      const int pieces = (len + cmax - 1) / cmax;
-      const int per = (len + pieces - 1) / pieces;
-      for (int k0 = 0; k0 < len; k0 += per) C[c].push_back(JkChunk{e.bra, si, k0, std::min(len, k0 + per)});
     }
     std::vector<Entry>().swap(E[c]);
   }
-
   // shell-pair max|D| per molecule (density screen, FP32 split)
   std::vector<double> sdm;
   if (shell_dmax) sdm.assign(shell_dmax, shell_dmax + b->mol_dmoff.back());
@@ -885,9 +953,8 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
       (e = b->sdmax.assign(sdm.data(), sdm.size())) != hipSuccess)
     return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
   for (int c = 0; c < kNumClasses; ++c) {
-    if ((e = b->cls[c].chunks.assign(C[c].data(), C[c].size())) != hipSuccess)
-      return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
-    b->cls[c].nchunks = int64_t(C[c].size());
+    const int rc = upload_chunks(b, c);
+    if (rc != LAI_OK) return rc;
   }
   b->wl_dscreen = dscreen ? 1 : 0;
   b->wl_mixed = mixed ? 1 : 0;
@@ -923,7 +990,7 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     sa.cls = c;
     sa.chunks = b->cls[c].chunks.p;
     sa.nchunks = int(b->cls[c].nchunks);
-    const int blocks = int(std::min<int64_t>(b->cls[c].nchunks, int64_t(n_cu) * 16));
+    const int blocks = int(std::min<int64_t>(b->cls[c].nchunks, int64_t(b->ctx->n_cu) * 16));
     hipLaunchKernelGGL(worklist_stats_kernel, dim3(blocks), dim3(256), 0, nullptr, sa);
   }
   LAI_HIP(hipGetLastError());
@@ -981,7 +1048,6 @@ extern "C" int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, doubl
 // (4 events per class) a hipEvent pair brackets each of the two parts.
 static void launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
   const auto& Ls = launchers();
-  const int max_blocks = b->ctx->n_cu * 16;
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
@@ -992,7 +1058,11 @@ static void launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st
     if (C.nchunks > 0) {
       a.chunks = C.chunks.p;
       a.nchunks = int(C.nchunks);
-      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, max_blocks);
+      LaunchTune t;
+      t.alt = b->tune[c].alt;
+      t.blocks_per_cu = b->tune[c].bpc;
+      t.n_cu = b->ctx->n_cu;
+      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, t);
     }
     if (ev) (void)hipEventRecord(ev[2 * c + 1], st);
   }
@@ -1199,12 +1269,233 @@ extern "C" int lai_dev_jk_grad(lai_basis* b, const double* d_D, double xscale, d
   a.sel.mixed = 0;  // the gradient runs every quartet in FP64
   const int max_blocks = b->ctx->n_cu * 16;
   for (int c = 0; c < kNumGradClasses; ++c) {
-    if (b->cls[c].nchunks <= 0) continue;
-    a.chunks = b->cls[c].chunks.p;
-    a.nchunks = int(b->cls[c].nchunks);
+    const ClassList& C = b->cls[c];
+    if (C.nchunks <= 0) continue;
+    const bool own = C.ngchunks > 0;  // the J/K chunks are tuned: default ones
+    a.chunks = own ? C.gchunks.p : C.chunks.p;
+    a.nchunks = int(own ? C.ngchunks : C.nchunks);
     G.f[c](a, st, max_blocks);
   }
   LAI_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner and dispatch table
+// ---------------------------------------------------------------------------
+namespace {
+void drop_graph(lai_basis* b) {
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+}
+
+lai_class_tune tune_entry(const lai_basis* b, int c) {
+  lai_class_tune e{};
+  e.cls = c;
+  e.strategy = class_shape(launchers().info[c], b->tune[c]);
+  e.chunk = b->tune[c].chunk;
+  e.blocks_per_cu = b->tune[c].bpc;
+  return e;
+}
+
+// quartets of class c in the current worklist
+size_t class_quartets(const lai_basis* b, int c) {
+  size_t n = 0;
+  for (const auto& r : b->cls[c].runs) n += size_t(r.k1);
+  return n;
+}
+}  // namespace
+
+extern "C" int lai_dev_class_strategies(int cls, int* strategy, int* alternative) {
+  if (cls < 0 || cls >= kNumClasses || !strategy || !alternative)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_class_strategies: invalid arguments");
+  const ClassInfo& info = launchers().info[cls];
+  *strategy = info.shape;
+  *alternative = info.alt;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_get_tuning(const lai_basis* b, lai_class_tune* table, int cap, int* n_out) {
+  if (!b || cap < 0 || (cap > 0 && !table)) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_get_tuning: invalid arguments");
+  const int n = std::min(cap, kNumClasses);
+  for (int c = 0; c < n; ++c) table[c] = tune_entry(b, c);
+  if (n_out) *n_out = kNumClasses;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_set_tuning(lai_basis* b, const lai_class_tune* table, int n) {
+  if (!b || n < 0 || (n > 0 && !table)) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_tuning: invalid arguments");
+  const auto& Ls = launchers();
+  ClassTune next[kNumClasses];
+  bool touched[kNumClasses] = {};
+  if (n == 0) {
+    for (int c = 0; c < kNumClasses; ++c) touched[c] = true;
+  } else {
+    for (int c = 0; c < kNumClasses; ++c) next[c] = b->tune[c];
+  }
+  for (int i = 0; i < n; ++i) {  // validate every entry before applying any
+    const lai_class_tune& e = table[i];
+    if (e.cls < 0 || e.cls >= kNumClasses)
+      return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_tuning: class " + std::to_string(e.cls) + " out of range");
+    const ClassInfo& info = Ls.info[e.cls];
+    if (e.strategy != info.shape && (info.alt == kShapeNone || e.strategy != info.alt))
+      return fail(LAI_ERR_UNSUPPORTED, "lai_dev_set_tuning: strategy " + std::to_string(e.strategy) +
+                                           " is not compiled for class " + std::to_string(e.cls));
+    if (e.chunk < 0 || e.chunk > (1 << 20) || e.blocks_per_cu < 0 || e.blocks_per_cu > 4096)
+      return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_set_tuning: chunk or blocks_per_cu out of range");
+    next[e.cls].alt = e.strategy == info.shape ? 0 : 1;
+    next[e.cls].chunk = e.chunk;
+    next[e.cls].bpc = e.blocks_per_cu;
+    touched[e.cls] = true;
+  }
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (!touched[c]) continue;
+    b->tune[c] = next[c];
+    if (b->have_worklist) {
+      const int rc = upload_chunks(b, c);
+      if (rc != LAI_OK) return rc;
+    }
+  }
+  drop_graph(b);
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int reps, lai_class_tune* table, int cap,
+                                int* n_out, void* stream) {
+  if (!b || !d_D || reps < 1 || reps > 100 || cap < 0 || (cap > 0 && !table))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_autotune: invalid arguments");
+  if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_autotune: worklist not built");
+  if (flags != LAI_JK_FP64 && flags != LAI_JK_MIXED)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_autotune: unknown flags");
+  hipStream_t st = hipStream_t(stream);
+  const auto& Ls = launchers();
+  const int n_cu = b->ctx->n_cu;
+  EriArgs a = jk_args(b, d_D);
+  a.segs = b->segs.p;
+  a.kets = b->kets.p;
+  a.sel = make_sel(b);
+  if (flags != LAI_JK_MIXED) a.sel.mixed = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LAI_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(LAI_ERR_BACKEND, "lai_dev_autotune: hipEventCreate failed");
+  }
+  // the trial launches accumulate into the J/K scratch, which every build clears
+  int rc = LAI_OK;
+  int n_rep = 0;
+  // milliseconds per launch of class c under b->tune[c] (chunks uploaded)
+  auto time_class = [&](int c, double& ms) -> int {
+    const ClassList& C = b->cls[c];
+    a.chunks = C.chunks.p;
+    a.nchunks = int(C.nchunks);
+    LaunchTune t;
+    t.alt = b->tune[c].alt;
+    t.blocks_per_cu = b->tune[c].bpc;
+    t.n_cu = n_cu;
+    const ClassLauncher f = flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c];
+    f(a, st, t);  // warm-up
+    LAI_HIP(hipGetLastError());
+    LAI_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) f(a, st, t);
+    LAI_HIP(hipEventRecord(e1, st));
+    LAI_HIP(hipGetLastError());
+    LAI_HIP(hipEventSynchronize(e1));
+    float t_ms = 0.f;
+    LAI_HIP(hipEventElapsedTime(&t_ms, e0, e1));
+    ms = double(t_ms) / reps;
+    return LAI_OK;
+  };
+  int failed_class = -1;
+  ClassTune failed_saved{};
+  for (int c = kNumClasses - 1; c >= 0 && rc == LAI_OK; --c) {
+    const size_t nq = class_quartets(b, c);
+    if (nq == 0) continue;
+    const ClassInfo& info = Ls.info[c];
+    const ClassTune saved = b->tune[c];
+    auto eval = [&](const ClassTune& t, double& ms) -> int {
+      b->tune[c] = t;
+      const int r = upload_chunks(b, c);
+      if (r != LAI_OK) return r;
+      return time_class(c, ms);
+    };
+    // stage 1: shape x chunk cap at the default grid
+    std::vector<std::pair<int, int>> tried;  // (shape, effective cap)
+    ClassTune best{};
+    double best_ms = 0.0, dflt_ms = 0.0;
+    failed_class = c;
+    failed_saved = saved;
+    if ((rc = eval(best, dflt_ms)) != LAI_OK) break;
+    best_ms = dflt_ms;
+    tried.emplace_back(info.shape, default_chunk(info.shape, nq, n_cu));
+    for (int alt = 0; alt <= (info.alt != kShapeNone ? 1 : 0) && rc == LAI_OK; ++alt) {
+      const int shape = alt ? info.alt : info.shape;
+      std::vector<int> caps;
+      if (shape == kShapeRegs) caps = {512, 1024, 2048};
+      else if (shape == kShapeWaves) caps = {8, 16, 32, 64};
+      else caps = {4, 8, 16, 32};
+      if (alt) caps.insert(caps.begin(), 0);
+      for (int cap_q : caps) {
+        const int eff = cap_q > 0 ? cap_q : default_chunk(shape, nq, n_cu);
+        if (std::find(tried.begin(), tried.end(), std::make_pair(shape, eff)) != tried.end()) continue;
+        tried.emplace_back(shape, eff);
+        ClassTune t{};
+        t.alt = alt;
+        t.chunk = cap_q;
+        double ms = 0.0;
+        if ((rc = eval(t, ms)) != LAI_OK) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+    }
+    if (rc != LAI_OK) break;
+    // stage 2: grid cap of the best
+    const int shape = class_shape(info, best);
+    const std::vector<int> bpcs = shape == kShapeRegs ? std::vector<int>{4, 64} : std::vector<int>{16, 512};
+    const ClassTune base = best;
+    for (int bpc : bpcs) {
+      ClassTune t = base;
+      t.bpc = bpc;
+      double ms = 0.0;
+      if ((rc = eval(t, ms)) != LAI_OK) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    if (rc != LAI_OK) break;
+    if (!(best_ms < 0.99 * dflt_ms)) {  // keep the default unless clearly faster
+      best = ClassTune{};
+      best_ms = dflt_ms;
+    }
+    b->tune[c] = best;
+    if ((rc = upload_chunks(b, c)) != LAI_OK) break;
+    if (n_rep < cap) {
+      lai_class_tune& e = table[n_rep];
+      e = tune_entry(b, c);
+      e.ms = best_ms;
+      e.ms_default = dflt_ms;
+    }
+    ++n_rep;
+    failed_class = -1;
+  }
+  if (rc != LAI_OK && failed_class >= 0) {  // leave the failing class as it was
+    b->tune[failed_class] = failed_saved;
+    (void)upload_chunks(b, failed_class);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  drop_graph(b);
+  if (rc != LAI_OK) return rc;
+  if (n_out) *n_out = n_rep;
   return LAI_OK;
 }
 

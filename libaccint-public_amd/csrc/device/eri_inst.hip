@@ -40,7 +40,7 @@ int use_rows(K kern, size_t stat, size_t dyn, size_t budget) {
 // MIXED = false: every quartet in FP64 (and the raw-ERI mode); true: the
 // quartets the worklist marks for FP32 run the recursion in float.
 template <int ID, bool MIXED>
-void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
+void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
   constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
   using P = ClassPolicy<La, Lb, Lc, Ld>;
@@ -48,6 +48,7 @@ void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
   EriArgs a = a0;
   if (a.mode == 1) {  // raw ERIs, one launch over the quartet list
     if (MIXED || a.nq <= 0) return;
+    const int max_blocks = t.n_cu * kBlocksPerCuRegs;
     if constexpr (P::regs) {
       int blocks = (a.nq + 127) / 128;
       if (blocks > max_blocks) blocks = max_blocks;
@@ -62,26 +63,34 @@ void launch_class(const EriArgs& a0, hipStream_t s, int max_blocks) {
   // J/K digestion over the bra-anchored chunks
   if (a.nchunks <= 0) return;
   const size_t nbf = size_t(a.nbf);
-  if constexpr (P::regs) {
-    const size_t dyn = size_t(P::CHA + nb) * nbf * sizeof(double);
-    a.klds = use_rows(jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>, 0, dyn, kLdsBudget);
-    int blocks = a.nchunks < max_blocks ? a.nchunks : max_blocks;
-    dim3 grid(blocks, P::NCH);
-    hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
-  } else if constexpr (P::waves) {
-    auto kern = jk_kernel_waves<La, Lb, Lc, Ld, MIXED, P::NW>;
-    static const size_t stat = static_lds(kern);
-    const size_t dyn = (size_t(na + nb) * nbf + na * nb) * sizeof(double);
-    a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
-    int blocks = a.nchunks < 8 * max_blocks ? a.nchunks : 8 * max_blocks;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * P::NW), a.klds ? dyn : 0, s, a);
+  const int shape = (t.alt && P::alt >= 0) ? P::alt : P::shape;
+  auto grid_cap = [&](int dflt) {
+    const long long cap = (long long)t.n_cu * (t.blocks_per_cu > 0 ? t.blocks_per_cu : dflt);
+    return int(a.nchunks < cap ? a.nchunks : cap);
+  };
+  if (shape == kShapeRegs) {
+    if constexpr (P::regs) {
+      const size_t dyn = size_t(P::CHA + nb) * nbf * sizeof(double);
+      a.klds = use_rows(jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>, 0, dyn, kLdsBudget);
+      dim3 grid(grid_cap(kBlocksPerCuRegs), P::NCH);
+      hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
+    }
+  } else if (shape == kShapeWaves) {
+    if constexpr (P::waves || P::alt == kShapeWaves) {
+      auto kern = jk_kernel_waves<La, Lb, Lc, Ld, MIXED, P::NW>;
+      static const size_t stat = static_lds(kern);
+      const size_t dyn = (size_t(na + nb) * nbf + na * nb) * sizeof(double);
+      a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
+      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuLds)), dim3(64 * P::NW), a.klds ? dyn : 0, s, a);
+    }
   } else {
-    auto kern = jk_kernel_lds<La, Lb, Lc, Ld, MIXED, P::NT>;
-    static const size_t stat = static_lds(kern);
-    const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
-    a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
-    int blocks = a.nchunks < 8 * max_blocks ? a.nchunks : 8 * max_blocks;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(P::NT), a.klds ? dyn : 0, s, a);
+    if constexpr (!P::regs) {
+      auto kern = jk_kernel_lds<La, Lb, Lc, Ld, MIXED, P::NT>;
+      static const size_t stat = static_lds(kern);
+      const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
+      a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
+      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuLds)), dim3(P::NT), a.klds ? dyn : 0, s, a);
+    }
   }
 }
 
@@ -101,6 +110,8 @@ void fill(ClassLauncher* f64, ClassLauncher* f32, ClassInfo* info) {
     info[ID].nchunk = P::regs ? P::NCH : 0;
     info[ID].nt = P::regs ? 256 : (P::waves ? 64 * P::NW : P::NT);
     info[ID].waves = P::waves ? P::NW : 0;
+    info[ID].shape = P::shape;
+    info[ID].alt = P::alt;
   }
   if constexpr (ID + 1 < kNumClasses) fill<ID + 1>(f64, f32, info);
 }

@@ -1298,6 +1298,13 @@ struct ClassPolicy {
   // four wavefronts, two where one wavefront's tables exceed 12 KB
   static constexpr int NW =
       (LdsShape<La, Lb, Lc, Ld, double>::TS + LdsShape<La, Lb, Lc, Ld, double>::HSS) * 8 <= 12 * 1024 ? 4 : 2;
+  // shape codes: 0 lane per quartet, 1 wavefront per quartet, 2 workgroup per quartet
+  static constexpr int shape = regs ? 0 : (waves ? 1 : 2);
+  // the second shape compiled for the autotuner: register classes that split
+  // the bra over several workgroups may run a wavefront per quartet instead,
+  // wavefront classes a workgroup per quartet
+  static constexpr int alt =
+      (regs && NCH >= 2 && LdsTasks<La, Lb, Lc, Ld>::NTASK <= 6 * 64) ? 1 : (waves ? 2 : -1);
 };
 
 }  // namespace lai

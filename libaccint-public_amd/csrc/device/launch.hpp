@@ -9,7 +9,23 @@ namespace lai {
 struct EriArgs;
 struct GradArgs;
 
-using ClassLauncher = void (*)(const EriArgs&, hipStream_t, int max_blocks);
+// Kernel shapes of the J/K build: a lane per quartet (register tables), a
+// wavefront per quartet, a workgroup per quartet (LDS tables).
+enum : int { kShapeNone = -1, kShapeRegs = 0, kShapeWaves = 1, kShapeLds = 2 };
+
+// Launch parameters of one class: which of its compiled shapes runs and the
+// grid cap in workgroups per CU (0: the shape's default).
+struct LaunchTune {
+  int alt = 0;            // 0: the class's primary shape, 1: its alternative
+  int blocks_per_cu = 0;  // grid = min(chunks, n_cu * blocks_per_cu)
+  int n_cu = 256;
+};
+
+using ClassLauncher = void (*)(const EriArgs&, hipStream_t, const LaunchTune&);
+
+// default grid caps, workgroups per CU
+inline constexpr int kBlocksPerCuRegs = 16;
+inline constexpr int kBlocksPerCuLds = 128;
 
 struct ClassInfo {
   int La, Lb, Lc, Ld;
@@ -17,6 +33,8 @@ struct ClassInfo {
   int nchunk;  // Kernel A chunks along the bra index
   int nt;      // lanes per workgroup
   int waves;   // LDS shape with one quartet per wavefront: wavefronts per workgroup, else 0
+  int shape;   // primary shape (kShape*)
+  int alt;     // alternative shape compiled for the autotuner, kShapeNone if none
 };
 
 #define LAI_NPARTS_FIXED 8

@@ -5,6 +5,7 @@
 #include <cmath>
 
 #include "libaccint/engine.hpp"
+#include "dispatch_detail.hpp"
 
 namespace libaccint {
 
@@ -47,6 +48,7 @@ lai_basis* BatchEngine::device_handle() {
                                        cf.data(), &h),
                   "BatchEngine");
     dev_ = h;
+    detail::apply_dispatch_table_env(dev_);
     return dev_;
 }
 

@@ -1,8 +1,11 @@
 // engine.cpp -- Engine over the gfx950 backend (see engine.hpp).
 #include <libaccint/engine.hpp>
 
+#include "dispatch_detail.hpp"
+
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace libaccint {
@@ -79,6 +82,8 @@ lai_basis* Engine::device_basis() {
                       "Engine: basis upload");
         dev_ = b;
         device::check(lai_dev_set_shell_atoms(dev_, atom.data(), n_atoms), "Engine: shell atoms");
+        // a dispatch table file applies to every engine (cuda_engine.cu:199-207)
+        detail::apply_dispatch_table_env(dev_);
     }
     return dev_;
 }
@@ -303,6 +308,11 @@ WorklistInfo Engine::prepare_worklist(const JKOptions& o, const Real* D_host) {
 
 void Engine::build_jk(const JKOptions& o, const Real* d_D, Real* d_J, Real* d_K, hipStream_t s, const Real* D_host) {
     prepare_worklist(o, D_host);
+    const int flags = o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64;
+    // ProfileOnce: the first build of a precision tunes the dispatch table
+    if (config_.enable_auto_tuning && tuned_flags_ != flags &&
+        static_cast<Size>(wl_info_.n_quartets) >= config_.auto_tune_min_batch)
+        (void)autotune(o, d_D, s, 3, D_host);
     device::check(lai_dev_set_mixed_prim_cut(device_basis(), o.fp32_prim_cut), "build_jk");
     device::check(lai_dev_jk(device_basis(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
                   "build_jk");
@@ -322,6 +332,10 @@ void Engine::build_jk_host(const JKOptions& o, const Real* D, Real* J, Real* K) 
 
 void Engine::capture_jk_graph(const JKOptions& o, const Real* d_D, Real* d_J, Real* d_K, hipStream_t s) {
     prepare_worklist(o, nullptr);
+    const int flags = o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64;
+    if (config_.enable_auto_tuning && tuned_flags_ != flags &&
+        static_cast<Size>(wl_info_.n_quartets) >= config_.auto_tune_min_batch)
+        (void)autotune(o, d_D, s, 3, nullptr);
     device::check(lai_dev_set_mixed_prim_cut(device_basis(), o.fp32_prim_cut), "capture_jk_graph");
     device::check(
         lai_dev_jk_graph_capture(device_basis(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),

@@ -303,7 +303,50 @@ PYBIND11_MODULE(_core, m) {
         .def_readwrite("min_gpu_batch_size", &DispatchConfig::min_gpu_batch_size)
         .def_readwrite("min_gpu_primitives", &DispatchConfig::min_gpu_primitives)
         .def_readwrite("min_gpu_shells", &DispatchConfig::min_gpu_shells)
-        .def_readwrite("n_gpu_slots", &DispatchConfig::n_gpu_slots);
+        .def_readwrite("n_gpu_slots", &DispatchConfig::n_gpu_slots)
+        .def_readwrite("enable_auto_tuning", &DispatchConfig::enable_auto_tuning)
+        .def_readwrite("auto_tune_min_batch", &DispatchConfig::auto_tune_min_batch);
+
+    py::enum_<GpuExecutionStrategy>(m, "GpuExecutionStrategy")
+        .value("ThreadPerQuartet", GpuExecutionStrategy::ThreadPerQuartet)
+        .value("WarpPerQuartet", GpuExecutionStrategy::WarpPerQuartet)
+        .value("BlockPerQuartet", GpuExecutionStrategy::BlockPerQuartet);
+    m.def("gpu_execution_strategy_from_string", [](const std::string& s) { return gpu_execution_strategy_from_string(s); });
+    m.def("strategy_name", [](GpuExecutionStrategy s) { return std::string(to_string(s)); });
+    m.def("quartet_class_index", [](std::array<int, 4> am) { return quartet_class_index(am); }, py::arg("am"));
+    m.def("quartet_class_am", &quartet_class_am, py::arg("cls"));
+
+    py::class_<DispatchEntry>(m, "DispatchEntry")
+        .def(py::init([](int cls, GpuExecutionStrategy st, int chunk, int bpc) {
+                 DispatchEntry e;
+                 e.cls = cls;
+                 e.strategy = st;
+                 e.chunk = chunk;
+                 e.blocks_per_cu = bpc;
+                 return e;
+             }),
+             py::arg("cls"), py::arg("strategy"), py::arg("chunk") = 0, py::arg("blocks_per_cu") = 0)
+        .def_readwrite("cls", &DispatchEntry::cls)
+        .def_property_readonly("am", [](const DispatchEntry& e) { return quartet_class_am(e.cls); })
+        .def_readwrite("strategy", &DispatchEntry::strategy)
+        .def_readwrite("chunk", &DispatchEntry::chunk)
+        .def_readwrite("blocks_per_cu", &DispatchEntry::blocks_per_cu)
+        .def_readonly("ms", &DispatchEntry::ms)
+        .def_readonly("ms_default", &DispatchEntry::ms_default)
+        .def("__repr__", [](const DispatchEntry& e) {
+            return "DispatchEntry(cls=" + std::to_string(e.cls) + ", strategy=" + std::string(to_string(e.strategy)) +
+                   ", chunk=" + std::to_string(e.chunk) + ", blocks_per_cu=" + std::to_string(e.blocks_per_cu) + ")";
+        });
+
+    py::class_<DispatchTable>(m, "DispatchTable")
+        .def(py::init<>())
+        .def(py::init([](std::vector<DispatchEntry> v) { return DispatchTable{std::move(v)}; }), py::arg("entries"))
+        .def_readwrite("entries", &DispatchTable::entries)
+        .def_static("from_json", &DispatchTable::from_json, py::arg("path"))
+        .def("to_json", &DispatchTable::to_json, py::arg("path"))
+        .def_static("parse_json", &DispatchTable::parse_json, py::arg("text"))
+        .def("dump_json", &DispatchTable::dump_json)
+        .def("__len__", [](const DispatchTable& t) { return t.entries.size(); });
 
     py::class_<JKOptions>(m, "JKOptions")
         .def(py::init([](Real thr, bool dw, bool mixed, Real tau, Real pcut) {
@@ -474,6 +517,20 @@ PYBIND11_MODULE(_core, m) {
                  return r;
              },
              py::arg("D"), py::arg("exchange_scale") = 1.0, py::arg("n_atoms") = -1)
+        .def("autotune",
+             [](Engine& e, std::uintptr_t dD, const JKOptions& o, std::uintptr_t stream, int reps, py::object Dh) {
+                 const Real* dh = nullptr;
+                 py::array_t<Real, py::array::c_style> keep;
+                 if (!Dh.is_none()) {
+                     keep = square(Dh.cast<py::array>(), e.basis().n_basis_functions(), "autotune");
+                     dh = keep.data();
+                 }
+                 return e.autotune(o, reinterpret_cast<const Real*>(dD), reinterpret_cast<hipStream_t>(stream), reps, dh);
+             },
+             py::arg("d_D"), py::arg("options") = JKOptions{}, py::arg("stream") = std::uintptr_t(0),
+             py::arg("reps") = 3, py::arg("D_host") = py::none())
+        .def("dispatch_table", &Engine::dispatch_table)
+        .def("set_dispatch_table", &Engine::set_dispatch_table, py::arg("table"))
         .def("worklist_info", [](const Engine& e) { return info_dict(e.worklist_info()); })
         .def("device_basis_handle", [](Engine& e) { return reinterpret_cast<std::uintptr_t>(e.device_basis()); })
         .def("__repr__", [](const Engine& e) {
@@ -538,6 +595,22 @@ PYBIND11_MODULE(_core, m) {
         .def("launch_jk_graph",
              [](BatchEngine& e, std::uintptr_t s) { e.launch_jk_graph(reinterpret_cast<hipStream_t>(s)); },
              py::arg("stream"))
+        .def("autotune",
+             [](BatchEngine& e, std::uintptr_t dD, const JKOptions& o, std::uintptr_t stream, int reps, py::object Dh) {
+                 const Real* dh = nullptr;
+                 py::array_t<Real, py::array::c_style | py::array::forcecast> keep;
+                 if (!Dh.is_none()) {
+                     keep = py::array_t<Real, py::array::c_style | py::array::forcecast>(Dh.cast<py::array>());
+                     if (Size(keep.size()) != e.matrix_size())
+                         throw InvalidArgumentException("BatchEngine.autotune: D has the wrong length");
+                     dh = keep.data();
+                 }
+                 return e.autotune(o, reinterpret_cast<const Real*>(dD), reinterpret_cast<hipStream_t>(stream), reps, dh);
+             },
+             py::arg("d_D"), py::arg("options") = JKOptions{}, py::arg("stream") = std::uintptr_t(0),
+             py::arg("reps") = 3, py::arg("D_host") = py::none())
+        .def("dispatch_table", &BatchEngine::dispatch_table)
+        .def("set_dispatch_table", &BatchEngine::set_dispatch_table, py::arg("table"))
         .def("device_basis_handle", [](BatchEngine& e) { return reinterpret_cast<std::uintptr_t>(e.device_handle()); })
         .def("__repr__", [](const BatchEngine& e) {
             return "BatchEngine(n_molecules=" + std::to_string(e.n_molecules()) + ")";

@@ -19,6 +19,16 @@ _i64 = ctypes.c_int64
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
+
+
+class ClassTune(ctypes.Structure):
+    """lai_class_tune: one dispatch-table entry."""
+    _fields_ = [("cls", ctypes.c_int), ("strategy", ctypes.c_int), ("chunk", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int), ("ms", ctypes.c_double), ("ms_default", ctypes.c_double)]
+
+
+_tp = ctypes.POINTER(ClassTune)
+
 _SIGNATURES = {
     "lai_dev_init": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "lai_dev_fini": ([_vp], ctypes.c_int),
@@ -49,6 +59,10 @@ _SIGNATURES = {
     "lai_dev_jk_graph_capture": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_jk_graph_launch": ([_vp, _vp], ctypes.c_int),
     "lai_dev_set_mixed_prim_cut": ([_vp, ctypes.c_double], ctypes.c_int),
+    "lai_dev_class_strategies": ([ctypes.c_int, _ip, _ip], ctypes.c_int),
+    "lai_dev_autotune": ([_vp, _vp, ctypes.c_int, ctypes.c_int, _tp, ctypes.c_int, _ip, _vp], ctypes.c_int),
+    "lai_dev_get_tuning": ([_vp, _tp, ctypes.c_int, _ip], ctypes.c_int),
+    "lai_dev_set_tuning": ([_vp, _tp, ctypes.c_int], ctypes.c_int),
     "lai_dev_set_shell_atoms": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
     "lai_dev_jk_grad": ([_vp, _vp, ctypes.c_double, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_onee": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp, _vp], ctypes.c_int),
