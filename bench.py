@@ -60,6 +60,11 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="no autotuner run in the setup (the default choices, or the table named by "
+                         "LIBACCINT_DISPATCH_TABLE)")
+    ap.add_argument("--save-table", default="",
+                    help="write the 16-water build's dispatch table (JSON) to this path")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="end with a device fill and copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--config4-full", action="store_true",
@@ -123,6 +128,23 @@ def main():
     # -------------------------------------------------------------- FP64 J+K
     J64, K64 = jk_once(opts)
     E64 = 0.5 * np.sum(D * J64) - 0.25 * np.sum(D * K64)
+    tune = {}
+    if not args.no_tune:
+        # the default kernel choices, then the autotuner (setup, not timed)
+        ms_untuned, _ = timed_graph(opts, max(2, args.steps // 2), 1)
+        t0 = time.perf_counter()
+        table = engine.autotune(tD.data_ptr(), opts, stream.cuda_stream, 3)
+        t_tune = time.perf_counter() - t0
+        Jt_, Kt_ = jk_once(opts)
+        dev_tuned = max(float(np.max(np.abs(Jt_ - J64))) / float(np.max(np.abs(J64))),
+                        float(np.max(np.abs(Kt_ - K64))) / float(np.max(np.abs(K64))))
+        changed = [e_ for e_ in table.entries if e_.ms < e_.ms_default]
+        tune = {"autotune_seconds": t_tune, "jk_seconds_fp64_untuned": ms_untuned * 1e-3,
+                "autotune_classes_changed": len(changed), "autotune_max_rel_dev_jk": dev_tuned}
+        log("autotune: %.2f s, %d of %d classes changed, untuned %.3f ms, tuned J/K rel. deviation %.1e" %
+            (t_tune, len(changed), len(table.entries), ms_untuned, dev_tuned))
+    if args.save_table:
+        engine.dispatch_table().to_json(args.save_table)
     ms64, wall64 = timed_graph(opts, args.steps, args.warmup)
     log("FP64 J+K: %.3f ms/step (events), wall %.3f s for %d steps" % (ms64, wall64, args.steps))
 
@@ -212,10 +234,14 @@ def main():
         msteps = max(2, args.steps // 2)
         Jd, Kd = m_once(o64)
         Ed = 0.5 * np.sum(mD * Jd) - 0.25 * np.sum(mD * Kd)
+        if not args.no_tune:
+            meng.autotune(tmD.data_ptr(), o64, stream.cuda_stream, 2)
         ms_d = m_timed(o64, msteps, 1)
         om = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
         meng.prepare_worklist(om, mD)
         mim = meng.worklist_info()
+        if not args.no_tune:
+            meng.autotune(tmD.data_ptr(), om, stream.cuda_stream, 2, mD)
         Jm, Km = m_once(om)
         Em = 0.5 * np.sum(mD * Jm) - 0.25 * np.sum(mD * Km)
         ms_m = m_timed(om, msteps, 1)
@@ -273,6 +299,11 @@ def main():
             return ev0.elapsed_time(ev1) / steps
 
         bsteps = 1 if args.config4_full else 3
+        bt = 0.0
+        if not args.no_tune and not args.config4_full:
+            t0 = time.perf_counter()
+            be.autotune(tbD.data_ptr(), bo, stream.cuda_stream, 1)
+            bt = time.perf_counter() - t0
         ms_b = b_timed(bo, bsteps)
         bm = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
         be.prepare_worklist(bm, bD)
@@ -298,7 +329,7 @@ def main():
                                    "Schwarz 1e-12, %d quartets, %.3e ERIs" %
                                    (len(sel), "all" if args.config4_full else "those with <= 20 atoms",
                                     min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
-                 "batch_setup_seconds": bsetup,
+                 "batch_setup_seconds": bsetup, "batch_autotune_seconds": bt,
                  "batch_jk_seconds_fp64": ms_b * 1e-3, "batch_eris_per_s_fp64": binfo["n_eri"] / (ms_b * 1e-3),
                  "batch_jk_seconds_mixed": ms_bm * 1e-3,
                  "batch_mixed_fp32_quartet_fraction": bminfo["n_fp32"] / max(bminfo["n_quartets"], 1),
@@ -376,6 +407,7 @@ def main():
                      "kernel_ms": float(ms_cls[dom])},
         "cpu_baseline": cpu,
     }
+    out.update(tune)
     out.update(mixed)
     out.update(batch)
     out.update(grad)

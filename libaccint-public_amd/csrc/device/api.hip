@@ -1459,7 +1459,7 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
     if (rc != LAI_OK) break;
     // stage 2: grid cap of the best
     const int shape = class_shape(info, best);
-    const std::vector<int> bpcs = shape == kShapeRegs ? std::vector<int>{4, 64} : std::vector<int>{16, 512};
+    const std::vector<int> bpcs = shape == kShapeRegs ? std::vector<int>{4, 16} : std::vector<int>{16, 512};
     const ClassTune base = best;
     for (int bpc : bpcs) {
       ClassTune t = base;

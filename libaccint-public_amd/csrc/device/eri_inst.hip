@@ -48,7 +48,7 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
   EriArgs a = a0;
   if (a.mode == 1) {  // raw ERIs, one launch over the quartet list
     if (MIXED || a.nq <= 0) return;
-    const int max_blocks = t.n_cu * kBlocksPerCuRegs;
+    const int max_blocks = t.n_cu * 16;  // grid-stride over the quartet list
     if constexpr (P::regs) {
       int blocks = (a.nq + 127) / 128;
       if (blocks > max_blocks) blocks = max_blocks;

@@ -23,8 +23,10 @@ struct LaunchTune {
 
 using ClassLauncher = void (*)(const EriArgs&, hipStream_t, const LaunchTune&);
 
-// default grid caps, workgroups per CU
-inline constexpr int kBlocksPerCuRegs = 16;
+// default grid caps, workgroups per CU: with up to 64 per CU a lane-per-quartet
+// launch has one workgroup per chunk for all but the largest classes, which
+// leaves the load balance to the hardware dispatcher
+inline constexpr int kBlocksPerCuRegs = 64;
 inline constexpr int kBlocksPerCuLds = 128;
 
 struct ClassInfo {
