@@ -773,13 +773,16 @@ extern "C" uint64_t lai_quartet_mix64(int i, int j, int k, int l) {
 
 namespace {
 // Chunk cap of the default rule for a class run in shape `shape` with n
-// quartets: lane-per-quartet kernels take up to 2048 quartets per workgroup
-// (8 per lane), wavefront-per-quartet kernels up to 64, workgroup-per-quartet
-// kernels up to 32; small classes get shorter chunks so that a launch still
-// has a few workgroups per CU.
-int default_chunk(int shape, size_t n, int n_cu) {
+// quartets: lane-per-quartet kernels take up to 8 quartets per lane of their
+// workgroup (nt lanes; at most 2048), wavefront-per-quartet kernels up to 64,
+// workgroup-per-quartet kernels up to 32; small classes get shorter chunks so
+// that a launch still has a few workgroups per CU.
+int default_chunk(int shape, size_t n, int n_cu, int nt = 256) {
   const size_t q = n / (4 * size_t(n_cu));
-  if (shape == kShapeRegs) return int(std::clamp<size_t>(q, 256, 2048));
+  if (shape == kShapeRegs) {
+    const size_t lo = size_t(nt), hi = std::min<size_t>(8 * size_t(nt), 2048);
+    return int(std::clamp<size_t>(q * size_t(nt) / 256, lo, hi));
+  }
   if (shape == kShapeWaves) return int(std::clamp<size_t>(q, 4, 64));
   return int(std::clamp<size_t>(q, 1, 32));
 }
@@ -809,7 +812,7 @@ int upload_chunks(lai_basis* b, int c) {
   for (const auto& r : C.runs) n += size_t(r.k1);
   const ClassTune& t = b->tune[c];
   const int shape = class_shape(Ls.info[c], t);
-  const int cmax = t.chunk > 0 ? t.chunk : default_chunk(shape, n, b->ctx->n_cu);
+  const int cmax = t.chunk > 0 ? t.chunk : default_chunk(shape, n, b->ctx->n_cu, Ls.info[c].nt);
   const std::vector<JkChunk> ch = cut_runs(C.runs, cmax);
   hipError_t e;
   if ((e = C.chunks.assign(ch.data(), ch.size())) != hipSuccess)
@@ -818,7 +821,7 @@ int upload_chunks(lai_basis* b, int c) {
   C.gchunks.release();
   C.ngchunks = 0;
   if (!t.is_default()) {
-    const std::vector<JkChunk> g = cut_runs(C.runs, default_chunk(Ls.info[c].shape, n, b->ctx->n_cu));
+    const std::vector<JkChunk> g = cut_runs(C.runs, default_chunk(Ls.info[c].shape, n, b->ctx->n_cu, Ls.info[c].nt));
     if ((e = C.gchunks.assign(g.data(), g.size())) != hipSuccess)
       return fail(LAI_ERR_MEMORY, std::string("chunk upload: ") + hipGetErrorString(e));
     C.ngchunks = int64_t(g.size());
@@ -1433,16 +1436,16 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
     failed_saved = saved;
     if ((rc = eval(best, dflt_ms)) != LAI_OK) break;
     best_ms = dflt_ms;
-    tried.emplace_back(info.shape, default_chunk(info.shape, nq, n_cu));
+    tried.emplace_back(info.shape, default_chunk(info.shape, nq, n_cu, info.nt));
     for (int alt = 0; alt <= (info.alt != kShapeNone ? 1 : 0) && rc == LAI_OK; ++alt) {
       const int shape = alt ? info.alt : info.shape;
       std::vector<int> caps;
-      if (shape == kShapeRegs) caps = {512, 1024, 2048};
+      if (shape == kShapeRegs) caps = {std::min(2 * info.nt, 2048), std::min(4 * info.nt, 2048), 2048};
       else if (shape == kShapeWaves) caps = {8, 16, 32, 64};
       else caps = {4, 8, 16, 32};
       if (alt) caps.insert(caps.begin(), 0);
       for (int cap_q : caps) {
-        const int eff = cap_q > 0 ? cap_q : default_chunk(shape, nq, n_cu);
+        const int eff = cap_q > 0 ? cap_q : default_chunk(shape, nq, n_cu, shape == info.shape ? info.nt : 256);
         if (std::find(tried.begin(), tried.end(), std::make_pair(shape, eff)) != tried.end()) continue;
         tried.emplace_back(shape, eff);
         ClassTune t{};

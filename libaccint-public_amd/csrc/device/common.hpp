@@ -134,6 +134,39 @@ __device__ __forceinline__ double boys_step(const double* r, int m, const double
   return s;
 }
 
+// 1/x for positive normal x: v_rcp_f64 and two Newton steps, each of which
+// squares the relative error (about one ulp at the end)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+// F_0(T) and F_1(T): the Boys Taylor table below the one-root cut, the
+// Gauss-Hermite limit of the one-root rule above it (F_0 = w, F_1 = u w)
+__device__ __forceinline__ void boys01(double T, const double* __restrict__ tab, double& F0, double& F1) {
+  if (T < double(lai_rys_tcut[1])) {
+    const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
+    const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
+    double e[7];
+#pragma unroll
+    for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
+    const double* r = tab + LAI_BT_OFF + k * LAI_BT_M;
+    double rr[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) rr[m] = r[m];
+    F0 = boys_step(rr, 0, e);
+    F1 = boys_step(rr, 1, e);
+  } else {
+    double y = __builtin_amdgcn_rsq(T);
+    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
+    F0 = lai_rys_hw[lai_rys_hoff[1]] * y;
+    F1 = F0 * (lai_rys_hx2[lai_rys_hoff[1]] * (y * y));
+  }
+}
+
 template <int N>
 __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tab, double* u, double* w) {
   static_assert(N >= 1 && N <= LAI_RYS_NMAX, "Rys order out of table range");
@@ -152,17 +185,17 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
       for (int m = 0; m < 2 * N + 6; ++m) rr[m] = r[m];
       const double F0 = boys_step(rr, 0, e), F1 = boys_step(rr, 1, e);
       if constexpr (N == 1) {
-        u[0] = F1 / F0;
+        u[0] = F1 * rcp_nr(F0);
         w[0] = F0;
       } else {
         const double F2 = boys_step(rr, 2, e), F3 = boys_step(rr, 3, e);
         // x^2 + p x + q orthogonal to 1 and x under the moments
-        const double idet = 1.0 / fma(F0, F2, -F1 * F1);
+        const double idet = rcp_nr(fma(F0, F2, -F1 * F1));
         const double p = fma(F1, F2, -F0 * F3) * idet;
         const double q = fma(F1, F3, -F2 * F2) * idet;
         const double x2 = 0.5 * (sqrt(fma(p, p, -4.0 * q)) - p);
-        const double x1 = q / x2;
-        const double w1 = fma(-x2, F0, F1) / (x1 - x2);
+        const double x1 = q * rcp_nr(x2);
+        const double w1 = fma(-x2, F0, F1) * rcp_nr(x1 - x2);
         u[0] = x1;
         u[1] = x2;
         w[0] = w1;

@@ -69,7 +69,14 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
     return int(a.nchunks < cap ? a.nchunks : cap);
   };
   if (shape == kShapeRegs) {
-    if constexpr (P::regs) {
+    if constexpr (P::regs && P::NCH > 1) {
+      // bra cut into NCH slices: one wavefront per slice, all rows in LDS
+      auto kern = jk_kernel_merged<La, Lb, Lc, Ld, MIXED, P::CHA>;
+      static const size_t stat = static_lds(kern);
+      const size_t dyn = size_t(na + nb) * nbf * sizeof(double);
+      a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);
+      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuRegs)), dim3(64 * P::NCH), a.klds ? dyn : 0, s, a);
+    } else if constexpr (P::regs) {
       const size_t dyn = size_t(P::CHA + nb) * nbf * sizeof(double);
       a.klds = use_rows(jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>, 0, dyn, kLdsBudget);
       dim3 grid(grid_cap(kBlocksPerCuRegs), P::NCH);
@@ -108,7 +115,7 @@ void fill(ClassLauncher* f64, ClassLauncher* f32, ClassInfo* info) {
     info[ID].Ld = Ld;
     info[ID].regs = P::regs ? 1 : 0;
     info[ID].nchunk = P::regs ? P::NCH : 0;
-    info[ID].nt = P::regs ? 256 : (P::waves ? 64 * P::NW : P::NT);
+    info[ID].nt = P::regs ? (P::NCH > 1 ? 64 * P::NCH : 256) : (P::waves ? 64 * P::NW : P::NT);
     info[ID].waves = P::waves ? P::NW : 0;
     info[ID].shape = P::shape;
     info[ID].alt = P::alt;

@@ -76,14 +76,13 @@ __device__ __forceinline__ int chunk_ket(const int* __restrict__ kets, const JkS
   return kets[sg->start[seg] + (k - sg->cum[seg])];
 }
 
-// Lane schedule of the lane-per-quartet kernels (256-lane workgroups).  The
-// quartets of a chunk that pass the per-quartet selection are counting-sorted
-// in LDS by (FP32 flag, primitive pairs of the ket), heaviest first, so that
-// the 64 lanes of a wavefront run primitive loops of nearly equal length and
-// one precision; quartets removed by the density screen take no lane.
+// Lane schedule of the lane-per-quartet kernels.  The quartets of a chunk
+// that pass the per-quartet selection are counting-sorted in LDS by (FP32
+// flag, primitive pairs of the ket), heaviest first, so that the 64 lanes of a
+// wavefront run primitive loops of nearly equal length and one precision;
+// quartets removed by the density screen take no lane.
 constexpr int kSortRun = 2048;                 // quartets sorted at a time
 constexpr int kSortBins = 256;                 // key: 128 * FP32 + min(npp_sig, 127)
-constexpr int kSortPer = kSortRun / 256;       // positions per lane
 constexpr unsigned kSortF32 = 0x80000000u;     // flag bit of a sorted entry
 struct SortLds {
   unsigned kp[kSortRun];  // ket pair, kSortF32 set for the FP32 recursion
@@ -92,18 +91,21 @@ struct SortLds {
 };
 
 // Sorts positions [s0, s1) (at most kSortRun) of a chunk's run into s.kp and
-// returns how many were kept.  All 256 lanes call it; it ends on a barrier.
+// returns how many were kept.  All NT lanes of the workgroup call it (NT a
+// multiple of 64, at least 64); it ends on a barrier.
+template <int NT = 256>
 __device__ __forceinline__ int sort_run(const int* __restrict__ kets, const JkSegs* __restrict__ sg,
                                         const PairRec* __restrict__ pairs, const QuartetSel& sel, int bp,
                                         const PairRec& bra, int s0, int s1, SortLds& s) {
+  constexpr int kSortPer = (kSortRun + NT - 1) / NT;  // positions per lane
   const int tid = threadIdx.x;
-  s.cnt[tid] = 0;  // kSortBins == 256 lanes
+  for (int i = tid; i < kSortBins; i += NT) s.cnt[i] = 0;
   __syncthreads();
   int kp[kSortPer], key[kSortPer], rk[kSortPer];
   int seg = 0;
 #pragma unroll
   for (int j = 0; j < kSortPer; ++j) {
-    const int k = s0 + tid + 256 * j;
+    const int k = s0 + tid + NT * j;
     key[j] = -1;
     kp[j] = 0;
     rk[j] = 0;
@@ -314,6 +316,23 @@ __device__ __forceinline__ void quartet_block_regs(
       const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
       const double T = zeta * rz * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = P.Kc * Q.Kc * y;
+      if constexpr (La + Lb + Lc + Ld == 1) {
+        // (ps|ss), the one class of total angular momentum 1: with one root,
+        // pref w (PA - (rho/zeta) u PQ) = pref (F0 PA - (rho/zeta) F1 PQ), so
+        // F0 and F1 replace the root and weight
+        double F0, F1;
+        boys01(T, g.rys, F0, F1);
+        const double s0 = pref * F0, s1 = -pref * F1 * rz;
+        const double pa[3] = {P.PAx, P.PAy, P.PAz}, pq[3] = {PQx, PQy, PQz};
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+          if constexpr (sizeof(R) == 8) {
+            acc[a][0][0][0] = fma(s1, pq[A0 + a], fma(s0, pa[A0 + a], acc[a][0][0][0]));
+          } else {
+            acc[a][0][0][0] += R(fma(s1, pq[A0 + a], s0 * pa[A0 + a]));
+          }
+        }
+      } else {
       double u[NR], w[NR];
       rys_roots<NR>(T, g.rys, u, w);
 #pragma unroll
@@ -347,6 +366,7 @@ __device__ __forceinline__ void quartet_block_regs(
                                    Iz[cart_comp(La, ai, 2)][cart_comp(Lb, b, 2)][cart_comp(Lc, c, 2)]
                                      [cart_comp(Ld, d, 2)];
               }
+      }
       }
     }
   }
@@ -881,6 +901,209 @@ __global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
     const JkChunk ch = g.chunks[c];
     dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, srt, blockIdx.y);
+  }
+}
+
+// Kernel A with merged bra slices: for classes whose bra index is cut into
+// NCH slices (to fit the registers), one workgroup of NCH wavefronts takes a
+// chunk, wavefront C computing slice C of the same sorted quartets in step.
+// The exchange rows of A and B, the sort and the chunk's loads are shared; the
+// slices' partial ket Coulomb blocks Jt[cd] meet in LDS and go to HBM with one
+// atomic per element instead of one per slice.
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
+__device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
+                                                double* jpart) {
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NCH = (na + CHA - 1) / CHA;
+  constexpr int NT = 64 * NCH;
+  constexpr int A0 = C * CHA, A1 = (A0 + CHA < na) ? A0 + CHA : na;
+  constexpr int NA = A1 - A0;
+  constexpr int NCD = nc * nd;
+  constexpr int ROWS = na + nb;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const bool lds = g.klds != 0;
+  const PairRec bra = g.pairs[ch.bra];
+  const int n = bra.mn;
+  const double* __restrict__ D = g.D + bra.moff;
+  double* Jt = g.Jt + bra.moff;
+  double* Kt = g.Kt + bra.moff;
+  const int fa = bra.fa, fb = bra.fb;
+  const bool ij = bra.same;
+  const JkSegs* sg = g.segs + ch.seg;
+  if (lds) {
+    for (int e = tid; e < ROWS * n; e += NT) krow[e] = 0.0;
+    __syncthreads();
+  }
+  // row r of the exchange rows: bra function fa + r (r < na) or fb + r - na
+  auto kadd = [&](int row, int col, double v) {
+    if (lds)
+      lds_add(&krow[row * n + col], v);
+    else
+      atomic_add(&Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], v);
+  };
+  double jab[NA][nb];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b) jab[a][b] = 0.0;
+
+  for (int s0 = ch.k0; s0 < ch.k1; s0 += kSortRun) {
+    const int nsel = sort_run<NT>(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt);
+    for (int r = 0; 64 * r < nsel; ++r) {
+      const int i = 64 * r + lane;
+      const bool valid = i < nsel;
+      int fc = 0, fd = 0;
+      bool bk = true;
+      if (valid) {
+        const unsigned e = srt.kp[i];
+        const int kp = int(e & ~kSortF32);
+        const int mode = (e & kSortF32) ? 2 : 1;
+        const PairRec ket = g.pairs[kp];
+        double gv[NA][nb][nc][nd];
+        if (MIXED && mode == 2)
+          quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
+        else
+          quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1>(g, bra, ket, false, gv);
+        fc = ket.fa;
+        fd = ket.fb;
+        const bool kl = ket.same;
+        bk = (kp == ch.bra);
+        const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+        const double fK = bk ? 1.0 : 2.0;
+        // Jt[ab] += fJ sum_cd g D[cd]  (this slice's rows, summed over the chunk)
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < nb; ++b) {
+            double t = 0.0;
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+              for (int d = 0; d < nd; ++d) t += gv[a][b][c][d] * D[size_t(fc + c) * n + fd + d];
+            jab[a][b] += fJ * t;
+          }
+        // this slice's part of Jt[cd] += fJ sum_ab g D[ab], to LDS
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+              for (int b = 0; b < nb; ++b) t += gv[a][b][c][d] * D[size_t(fa + A0 + a) * n + fb + b];
+            jpart[(C * NCD + c * nd + d) * 64 + lane] = fJ * t;
+          }
+        // Kt[ac] += fK sum_bd g D[bd]
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int c = 0; c < nc; ++c) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < nb; ++b)
+#pragma unroll
+              for (int d = 0; d < nd; ++d) t += gv[a][b][c][d] * D[size_t(fb + b) * n + fd + d];
+            kadd(A0 + a, fc + c, fK * t);
+          }
+        if (!kl) {  // Kt[ad] += fK sum_bc g D[bc]
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < nd; ++d) {
+              double t = 0.0;
+#pragma unroll
+              for (int b = 0; b < nb; ++b)
+#pragma unroll
+                for (int c = 0; c < nc; ++c) t += gv[a][b][c][d] * D[size_t(fb + b) * n + fc + c];
+              kadd(A0 + a, fd + d, fK * t);
+            }
+        }
+        if (!ij) {  // Kt[bc] += fK sum_ad g D[ad]  (this slice's a)
+#pragma unroll
+          for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int c = 0; c < nc; ++c) {
+              double t = 0.0;
+#pragma unroll
+              for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int d = 0; d < nd; ++d) t += gv[a][b][c][d] * D[size_t(fa + A0 + a) * n + fd + d];
+              kadd(na + b, fc + c, fK * t);
+            }
+        }
+        if (!ij && !kl) {  // Kt[bd] += fK sum_ac g D[ac]  (this slice's a)
+#pragma unroll
+          for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int d = 0; d < nd; ++d) {
+              double t = 0.0;
+#pragma unroll
+              for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int c = 0; c < nc; ++c) t += gv[a][b][c][d] * D[size_t(fa + A0 + a) * n + fc + c];
+              kadd(na + b, fd + d, fK * t);
+            }
+        }
+      }
+      __syncthreads();  // every slice's part of Jt[cd] is in LDS
+      if (valid && !bk) {
+        // element e of the ket block: wavefront e mod NCH adds the slices' sum
+        for (int e = C; e < NCD; e += NCH) {
+          double t = 0.0;
+#pragma unroll
+          for (int w = 0; w < NCH; ++w) t += jpart[(w * NCD + e) * 64 + lane];
+          atomic_add(&Jt[size_t(fc + e / nd) * n + fd + e % nd], t);
+        }
+      }
+      __syncthreads();  // jpart is rewritten by the next round
+    }
+  }
+  // Jt[ab] of this slice: one atomic per wavefront and element
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+      const double v = wave_sum(jab[a][b]);
+      if (lane == 0 && v != 0.0) atomic_add(&Jt[size_t(fa + A0 + a) * n + fb + b], v);
+    }
+  if (lds) {
+    __syncthreads();
+    for (int e = tid; e < ROWS * n; e += NT) {
+      const double v = krow[e];
+      if (v != 0.0) {
+        const int row = e / n, col = e - row * n;
+        atomic_add(&Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], v);
+      }
+    }
+    __syncthreads();  // krow is reused by the next chunk
+  }
+}
+
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
+__device__ __forceinline__ void dispatch_merged(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
+                                                double* jpart, int w) {
+  constexpr int na = ncart(La);
+  if (w == C) {
+    jk_chunk_merged<La, Lb, Lc, Ld, MIXED, CHA, C>(g, ch, krow, srt, jpart);
+  } else if constexpr ((C + 1) * CHA < na) {
+    dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, C + 1>(g, ch, krow, srt, jpart, w);
+  }
+}
+
+// Kernel A, merged slices: 64 * NCH lanes per chunk, wavefront w = slice w.
+// Dynamic LDS: the exchange rows [na + nb][mn]; static: the sort and the
+// slices' ket Coulomb parts.
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
+__global__ __launch_bounds__(64 * ((ncart(La) + CHA - 1) / CHA)) void jk_kernel_merged(EriArgs g) {
+  constexpr int NCH = (ncart(La) + CHA - 1) / CHA;
+  extern __shared__ double krow[];
+  __shared__ SortLds srt;
+  __shared__ double jpart[NCH * ncart(Lc) * ncart(Ld) * 64];
+  const int w = threadIdx.x >> 6;  // uniform over the wavefront
+  for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
+    const JkChunk ch = g.chunks[c];
+    dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, srt, jpart, w);
   }
 }
 
