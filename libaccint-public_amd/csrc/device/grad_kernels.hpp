@@ -40,20 +40,38 @@ struct GradArgs {
   const double* __restrict__ ppih;  // 1 / (2 zeta) of each primitive pair
 };
 
+// Primitive quartets per step of the LDS-shape gradient kernel: 3 n_r lanes
+// build the (root, axis) tables of one primitive quartet, as many primitive
+// quartets at once as the lanes and 16 KB of tables allow (more would cost
+// workgroups per CU).
+template <int La, int Lb, int Lc, int Ld, int NT>
+struct GradShape {
+  static constexpr int NR = (La + Lb + Lc + Ld + 1) / 2 + 1;
+  static constexpr int A1 = La + 2, B1 = Lb + 2, C1 = Lc + 2, D1 = Ld + 1;
+  static constexpr int S = A1 * B1 * C1 * D1;
+  static constexpr int HS = A1 * B1 * (Lc + 1 + Ld + 1);
+  static constexpr int NPQ = [] {
+    constexpr int by_lanes = NT / (3 * NR);
+    constexpr int by_lds = int((16 * 1024) / (3 * NR * (S + HS) * sizeof(double)));
+    constexpr int n = by_lanes < by_lds ? by_lanes : by_lds;
+    return n < 1 ? 1 : n;
+  }();
+};
+
 template <int La, int Lb, int Lc, int Ld, int NT>
 __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
-  constexpr int NR = (La + Lb + Lc + Ld + 1) / 2 + 1;
-  constexpr int A1 = La + 2, B1 = Lb + 2, C1 = Lc + 2, D1 = Ld + 1;
-  constexpr int S = A1 * B1 * C1 * D1;
-  constexpr int LCD1 = Lc + 1 + Ld;
-  constexpr int HS = A1 * B1 * (LCD1 + 1);
+  using Sh = GradShape<La, Lb, Lc, Ld, NT>;
+  constexpr int NR = Sh::NR, NPQ = Sh::NPQ;
+  constexpr int B1 = Sh::B1, C1 = Sh::C1, D1 = Sh::D1;
+  constexpr int S = Sh::S, HS = Sh::HS;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NOUT = na * nb * nc * nd;
-  constexpr int NJ = (NOUT + NT - 1) / NT;
   constexpr int SA = B1 * C1 * D1, SB = C1 * D1, SC = D1;
-  __shared__ double tab[NR * 3 * S];
-  __shared__ double hs[NR * 3 * HS];
+  __shared__ double tab[NPQ * NR * 3 * S];
+  __shared__ double hs[NPQ * NR * 3 * HS];
   __shared__ double W[NOUT];
+  __shared__ double ex2[NPQ][3];  // 2 alpha_a, 2 alpha_b, 2 alpha_c of each primitive quartet
+  __shared__ int cbase[NOUT], cexpo[NOUT];
   __shared__ double red[NT / 64][9];
   extern __shared__ double gacc[];
   const int tid = threadIdx.x;
@@ -61,25 +79,18 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
   if (glds)
     for (int e = tid; e < 3 * g.n_atoms; e += NT) gacc[e] = 0.0;
 
-  // per-lane component data: base offsets (8 bits per axis) and exponents
-  int base[NJ], expo[NJ];
-#pragma unroll
-  for (int j = 0; j < NJ; ++j) {
-    const int o = tid + NT * j;
-    base[j] = 0;
-    expo[j] = 0;
-    if (o < NOUT) {
-      const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
-      int e = 0, bs = 0;
-      for (int x = 0; x < 3; ++x) {
-        const int ax = cart_comp(La, a, x), bx = cart_comp(Lb, b, x), cx = cart_comp(Lc, c, x),
-                  dx = cart_comp(Ld, d, x);
-        bs |= (((ax * B1 + bx) * C1 + cx) * D1 + dx) << (8 * x);
-        e |= (ax | (bx << 2) | (cx << 4)) << (6 * x);
-      }
-      base[j] = bs;
-      expo[j] = e;
+  // per component: table offsets (8 bits per axis) and exponents
+  for (int o = tid; o < NOUT; o += NT) {
+    const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
+    int e = 0, bs = 0;
+    for (int x = 0; x < 3; ++x) {
+      const int ax = cart_comp(La, a, x), bx = cart_comp(Lb, b, x), cx = cart_comp(Lc, c, x),
+                dx = cart_comp(Ld, d, x);
+      bs |= (((ax * B1 + bx) * C1 + cx) * D1 + dx) << (8 * x);
+      e |= (ax | (bx << 2) | (cx << 4)) << (6 * x);
     }
+    cbase[o] = bs;
+    cexpo[o] = e;
   }
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
@@ -99,7 +110,7 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
     const bool ij = bra.same, kl = ket.same, bk = (kp == ch.bra);
     const double deg = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0) * (bk ? 1.0 : 2.0);
     const size_t n = size_t(bra.mn);
-    __syncthreads();  // W of the previous quartet is no longer read
+    __syncthreads();  // W and the tables of the previous quartet are no longer read
     for (int o = tid; o < NOUT; o += NT) {
       const int d = o % nd, c = (o / nd) % nc, b = (o / (nd * nc)) % nb, a = o / (nd * nc * nb);
       const size_t mu = bra.fa + a, nu = bra.fb + b, la = ket.fa + c, si = ket.fb + d;
@@ -110,15 +121,17 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
     }
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
     const int npp_b = bra.npp_sig, npp_k = ket.npp_sig;
-    for (int t = 0; t < npp_b * npp_k; ++t) {
-      const int ib = t / npp_k, ik = t - ib * npp_k;
-      const PrimPair P = g.pp[bra.pp_off + ib];
-      const PrimPair Q = g.pp[ket.pp_off + ik];
-      const double2 eb = g.ppab[bra.pp_off + ib];
-      const double2 ek = g.ppab[ket.pp_off + ik];
-      __syncthreads();
-      if (tid < 3 * NR) {
-        const int r = tid / 3, ax = tid - 3 * (tid / 3);
+    const int npt = npp_b * npp_k;
+    for (int t0 = 0; t0 < npt; t0 += NPQ) {
+      const int nq = min(NPQ, npt - t0);
+      if (t0 > 0) __syncthreads();  // the previous step's tables are no longer read
+      if (tid < 3 * NR * nq) {
+        const int q = tid / (3 * NR), rr = tid - q * (3 * NR);
+        const int r = rr / 3, ax = rr - 3 * r;
+        const int t = t0 + q;
+        const int ib = t / npp_k, ik = t - ib * npp_k;
+        const PrimPair P = g.pp[bra.pp_off + ib];
+        const PrimPair Q = g.pp[ket.pp_off + ik];
         const double zeta = P.zeta, eta = Q.zeta;
         const double izpe = 1.0 / (zeta + eta);
         const double rz = eta * izpe, re = zeta * izpe;
@@ -134,29 +147,37 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
         const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
         const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
         const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
-        build_2d_lds<La + 1, Lb + 1, Lc + 1, Ld, double>(tab + (r * 3 + ax) * S, hs + (r * 3 + ax) * HS,
-                                                         PA - rz * ur * PQ, QC + re * ur * PQ, AB, CD, B10, B01,
-                                                         B00, sc);
+        build_2d_lds<La + 1, Lb + 1, Lc + 1, Ld, double>(tab + ((q * NR + r) * 3 + ax) * S,
+                                                         hs + ((q * NR + r) * 3 + ax) * HS, PA - rz * ur * PQ,
+                                                         QC + re * ur * PQ, AB, CD, B10, B01, B00, sc);
+        if (rr == 0) {
+          const double2 eb = g.ppab[bra.pp_off + ib];
+          const double2 ek = g.ppab[ket.pp_off + ik];
+          ex2[q][0] = 2.0 * eb.x;
+          ex2[q][1] = 2.0 * eb.y;
+          ex2[q][2] = 2.0 * ek.x;
+        }
       }
       __syncthreads();
-      const double ta2 = 2.0 * eb.x, tb2 = 2.0 * eb.y, tc2 = 2.0 * ek.x;
+      // contraction: tasks (component o, primitive quartet q) over all lanes
+      for (int tk = tid; tk < NOUT * nq; tk += NT) {
+        const int q = tk / NOUT, o = tk - q * NOUT;
+        const double w = W[o];
+        const int bs = cbase[o], ee = cexpo[o];
+        const double ta2 = ex2[q][0], tb2 = ex2[q][1], tc2 = ex2[q][2];
+        int i0[3], ea[3], eb_[3], ec[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          i0[x] = (bs >> (8 * x)) & 255;
+          const int e = (ee >> (6 * x)) & 63;
+          ea[x] = e & 3;
+          eb_[x] = (e >> 2) & 3;
+          ec[x] = (e >> 4) & 3;
+        }
 #pragma unroll 1
-      for (int r = 0; r < NR; ++r) {
-        const double* T3[3] = {tab + (r * 3) * S, tab + (r * 3 + 1) * S, tab + (r * 3 + 2) * S};
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          const int o = tid + NT * j;
-          if (o >= NOUT) continue;
-          const double w = W[o];
-          int i0[3], ea[3], eb_[3], ec[3];
-#pragma unroll
-          for (int x = 0; x < 3; ++x) {
-            i0[x] = (base[j] >> (8 * x)) & 255;
-            const int e = (expo[j] >> (6 * x)) & 63;
-            ea[x] = e & 3;
-            eb_[x] = (e >> 2) & 3;
-            ec[x] = (e >> 4) & 3;
-          }
+        for (int r = 0; r < NR; ++r) {
+          const double* T3[3] = {tab + ((q * NR + r) * 3) * S, tab + ((q * NR + r) * 3 + 1) * S,
+                                 tab + ((q * NR + r) * 3 + 2) * S};
           const double v0[3] = {T3[0][i0[0]], T3[1][i0[1]], T3[2][i0[2]]};
 #pragma unroll
           for (int x = 0; x < 3; ++x) {
@@ -394,7 +415,7 @@ struct GradPolicy {
   static constexpr int NOUT = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld);
   static constexpr int NT = NOUT <= 256 ? 64 : 128;
   // register shape while the raised tables and outputs stay small
-  static constexpr bool regs = (La + 2) * (Lb + 2) * (Lc + 2) * (Ld + 1) <= 24 && NOUT <= 27;
+  static constexpr bool regs = (La + 2) * (Lb + 2) * (Lc + 2) * (Ld + 1) <= 27 && NOUT <= 27;
 };
 
 }  // namespace lai
