@@ -85,6 +85,30 @@ struct DispatchConfig {
     Size n_gpu_slots = 4;
 };
 
+// Kinds of integral work a dispatch decision is asked for
+// (include/libaccint/engine/dispatch_policy.hpp:28-34).
+enum class WorkUnitType { SingleShellPair, SingleShellQuartet, ShellSetPair, ShellSetQuartet, FullBasis };
+
+// The reference's CPU/GPU routing rule (src/host/engine/dispatch_policy.cpp:12-90):
+// force hints first, CPU without a GPU, then per work type thresholds of the
+// config, moved by the Prefer hints.  In this build every integral runs on
+// the device whatever the decision: the rule is kept so that callers that
+// consult it see the reference's answer (HIP where the reference says CUDA).
+class DispatchPolicy {
+public:
+    DispatchPolicy() = default;
+    explicit DispatchPolicy(DispatchConfig config) : config_(config) {}
+    [[nodiscard]] BackendType select_backend(WorkUnitType work_type, Size batch_size, int total_am,
+                                             Size n_primitives, BackendHint hint, bool gpu_available) const;
+    [[nodiscard]] const DispatchConfig& config() const noexcept { return config_; }
+    void set_config(DispatchConfig config) noexcept { config_ = config; }
+    void set_auto_tuning(bool enable) noexcept { config_.enable_auto_tuning = enable; }
+    [[nodiscard]] bool auto_tuning_enabled() const noexcept { return config_.enable_auto_tuning; }
+
+private:
+    DispatchConfig config_;
+};
+
 // Precision vocabulary (include/libaccint/core/precision.hpp:310-352).
 enum class Precision { Float32, Float64 };
 enum class MixedPrecisionMode { FP64Only, FP32Compute, Adaptive };

@@ -55,6 +55,39 @@ std::array<int, 4> quartet_class_am(int cls) {
     return {la, lb, lc, ld};
 }
 
+BackendType DispatchPolicy::select_backend(WorkUnitType work_type, Size batch_size, int total_am, Size n_primitives,
+                                           BackendHint hint, bool gpu_available) const {
+    if (hint == BackendHint::ForceCPU) return BackendType::CPU;
+    if (hint == BackendHint::ForceGPU) return gpu_available ? BackendType::HIP : BackendType::CPU;
+    if (!gpu_available) return BackendType::CPU;
+    const DispatchConfig& c = config_;
+    const bool single = work_type == WorkUnitType::SingleShellPair || work_type == WorkUnitType::SingleShellQuartet;
+    bool gpu = false;
+    switch (work_type) {
+        case WorkUnitType::SingleShellPair:
+        case WorkUnitType::SingleShellQuartet:
+            gpu = total_am >= c.high_am_threshold + 2;
+            break;
+        case WorkUnitType::ShellSetPair:
+            gpu = batch_size >= c.min_gpu_batch_size || n_primitives >= c.min_gpu_primitives ||
+                  total_am >= c.high_am_threshold;
+            break;
+        case WorkUnitType::ShellSetQuartet:
+            gpu = batch_size >= c.min_gpu_batch_size || n_primitives >= c.min_gpu_primitives ||
+                  total_am >= c.high_am_threshold - 1;
+            break;
+        case WorkUnitType::FullBasis:
+            gpu = batch_size >= c.min_gpu_shells;
+            break;
+    }
+    if (hint == BackendHint::PreferCPU) {
+        gpu = !single && gpu && (batch_size >= 2 * c.min_gpu_batch_size || total_am >= c.high_am_threshold + 2);
+    } else if (hint == BackendHint::PreferGPU && !single) {
+        gpu = gpu || batch_size >= c.min_gpu_batch_size / 4 || total_am >= c.high_am_threshold - 1;
+    }
+    return gpu ? BackendType::HIP : BackendType::CPU;
+}
+
 std::string_view to_string(GpuExecutionStrategy s) noexcept {
     switch (s) {
         case GpuExecutionStrategy::ThreadPerQuartet: return "thread_per_quartet";

@@ -307,6 +307,23 @@ PYBIND11_MODULE(_core, m) {
         .def_readwrite("enable_auto_tuning", &DispatchConfig::enable_auto_tuning)
         .def_readwrite("auto_tune_min_batch", &DispatchConfig::auto_tune_min_batch);
 
+    py::enum_<WorkUnitType>(m, "WorkUnitType")
+        .value("SingleShellPair", WorkUnitType::SingleShellPair)
+        .value("SingleShellQuartet", WorkUnitType::SingleShellQuartet)
+        .value("ShellSetPair", WorkUnitType::ShellSetPair)
+        .value("ShellSetQuartet", WorkUnitType::ShellSetQuartet)
+        .value("FullBasis", WorkUnitType::FullBasis);
+
+    py::class_<DispatchPolicy>(m, "DispatchPolicy")
+        .def(py::init<>())
+        .def(py::init<DispatchConfig>(), py::arg("config"))
+        .def("select_backend", &DispatchPolicy::select_backend, py::arg("work_type"), py::arg("batch_size"),
+             py::arg("total_am"), py::arg("n_primitives"), py::arg("hint"), py::arg("gpu_available"))
+        .def("config", &DispatchPolicy::config)
+        .def("set_config", &DispatchPolicy::set_config, py::arg("config"))
+        .def("set_auto_tuning", &DispatchPolicy::set_auto_tuning, py::arg("enable"))
+        .def("auto_tuning_enabled", &DispatchPolicy::auto_tuning_enabled);
+
     py::enum_<GpuExecutionStrategy>(m, "GpuExecutionStrategy")
         .value("ThreadPerQuartet", GpuExecutionStrategy::ThreadPerQuartet)
         .value("WarpPerQuartet", GpuExecutionStrategy::WarpPerQuartet)
