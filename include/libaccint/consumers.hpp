@@ -81,6 +81,46 @@ private:
     std::vector<std::vector<Real>> J_tl_, K_tl_;
 };
 
+// J and K in FP64 from FP32 or FP64 integral blocks
+// (include/libaccint/consumers/mixed_precision_fock_builder.hpp:43-147).  As the
+// consumer of Engine::compute_and_consume it receives the device build with
+// the FP32 Rys recursion for the quartets whose Q_ij Q_kl max|D| is below
+// fp32_threshold() (FP64 roots, weights and accumulation); the statistics
+// then count the FP32 and FP64 quartets.
+class MixedPrecisionFockBuilder {
+public:
+    explicit MixedPrecisionFockBuilder(Size nbf);
+    void set_density(const Real* D, Size nbf);
+    [[nodiscard]] const Real* density() const noexcept { return D_; }
+    void accumulate(const TwoElectronBuffer<0, float>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+                    int nc, int nd);
+    void accumulate(const TwoElectronBuffer<0, double>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+                    int nc, int nd);
+    // complete J and K of a device build and its quartet counts
+    void add_jk(const Real* J, const Real* K, Size n_fp32, Size n_fp64);
+    [[nodiscard]] std::span<const Real> get_coulomb_matrix() const noexcept { return J_; }
+    [[nodiscard]] std::span<const Real> get_exchange_matrix() const noexcept { return K_; }
+    [[nodiscard]] std::vector<Real> get_fock_matrix(std::span<const Real> H_core, Real exchange_fraction = 1.0) const;
+    void reset() noexcept;
+    [[nodiscard]] Size nbf() const noexcept { return nbf_; }
+    [[nodiscard]] MixedPrecisionMode mode() const noexcept { return MixedPrecisionMode::Compute32Accumulate64; }
+    [[nodiscard]] Size n_float32_accumulations() const noexcept { return n32_; }
+    [[nodiscard]] Size n_float64_accumulations() const noexcept { return n64_; }
+    // FP32 selection bound of the device build (default 1e-7)
+    void set_fp32_threshold(Real tau) noexcept { tau_ = tau; }
+    [[nodiscard]] Real fp32_threshold() const noexcept { return tau_; }
+
+private:
+    template <typename R>
+    void accumulate_impl(const TwoElectronBuffer<0, R>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
+                         int nc, int nd);
+    Size nbf_;
+    const Real* D_{nullptr};
+    std::vector<Real> J_, K_;
+    Size n32_{0}, n64_{0};
+    Real tau_{1e-7};
+};
+
 class GpuFockBuilder {
 public:
     explicit GpuFockBuilder(Size nbf, hipStream_t stream = nullptr);

@@ -111,6 +111,15 @@ private:
 
 // Precision vocabulary (include/libaccint/core/precision.hpp:310-352).
 enum class Precision { Float32, Float64 };
-enum class MixedPrecisionMode { FP64Only, FP32Compute, Adaptive };
+enum class MixedPrecisionMode : std::uint8_t { Pure32 = 0, Pure64 = 1, Compute32Accumulate64 = 2, Adaptive = 3 };
+[[nodiscard]] constexpr const char* mixed_precision_to_string(MixedPrecisionMode m) noexcept {
+    switch (m) {
+        case MixedPrecisionMode::Pure32: return "pure32";
+        case MixedPrecisionMode::Pure64: return "pure64";
+        case MixedPrecisionMode::Compute32Accumulate64: return "compute32_accumulate64";
+        case MixedPrecisionMode::Adaptive: return "adaptive";
+    }
+    return "unknown";
+}
 
 }  // namespace libaccint

@@ -163,7 +163,8 @@ public:
         (void)hint;
         require_two_electron(op, "compute_and_consume");
         if (options.density_weighted && !density_matrix_set() && !std::is_same_v<Consumer, consumers::FockBuilder> &&
-            !std::is_same_v<Consumer, consumers::GpuFockBuilder>)
+            !std::is_same_v<Consumer, consumers::GpuFockBuilder> &&
+            !std::is_same_v<Consumer, consumers::MixedPrecisionFockBuilder>)
             throw InvalidArgumentException(
                 "density_weighted screening requires a density matrix; call set_density_matrix() first");
         JKOptions jo;
@@ -172,6 +173,10 @@ public:
         if constexpr (std::is_same_v<Consumer, consumers::GpuFockBuilder>) {
             jk_into(consumer, jo);
         } else if constexpr (std::is_same_v<Consumer, consumers::FockBuilder>) {
+            jk_into(consumer, jo);
+        } else if constexpr (std::is_same_v<Consumer, consumers::MixedPrecisionFockBuilder>) {
+            jo.mixed_precision = true;
+            jo.fp32_threshold = consumer.fp32_threshold();
             jk_into(consumer, jo);
         } else {
             generic_full(consumer, options);
@@ -256,6 +261,7 @@ private:
     static void require_two_electron(const Operator& op, const char* what);
     void jk_into(consumers::FockBuilder& fb, const JKOptions& o);
     void jk_into(consumers::GpuFockBuilder& fb, const JKOptions& o);
+    void jk_into(consumers::MixedPrecisionFockBuilder& fb, const JKOptions& o);
     std::vector<Real> shell_dmax(const Real* D) const;
 
     template <typename Consumer>

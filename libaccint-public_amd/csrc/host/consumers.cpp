@@ -144,6 +144,67 @@ void FockBuilder::accumulate_symmetric(const TwoElectronBuffer<0>& buf, Index fa
     }
 }
 
+// ---- MixedPrecisionFockBuilder -------------------------------------------------
+MixedPrecisionFockBuilder::MixedPrecisionFockBuilder(Size nbf) : nbf_(nbf), J_(nbf * nbf, 0.0), K_(nbf * nbf, 0.0) {}
+
+void MixedPrecisionFockBuilder::set_density(const Real* D, Size nbf) {
+    if (nbf != nbf_)
+        throw InvalidArgumentException("MixedPrecisionFockBuilder::set_density: nbf " + std::to_string(nbf) +
+                                       " does not match " + std::to_string(nbf_));
+    D_ = D;
+}
+
+template <typename R>
+void MixedPrecisionFockBuilder::accumulate_impl(const TwoElectronBuffer<0, R>& buf, Index fa, Index fb, Index fc,
+                                                Index fd, int na, int nb, int nc, int nd) {
+    if (!D_) throw InvalidStateException("MixedPrecisionFockBuilder::accumulate called without density matrix");
+    const Size n = nbf_;
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < nd; ++d) {
+                    const Size mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+                    const double g = static_cast<double>(buf(a, b, c, d));  // promoted before accumulation
+                    J_[mu * n + nu] += g * D_[la * n + si];
+                    K_[mu * n + la] += g * D_[nu * n + si];
+                }
+}
+
+void MixedPrecisionFockBuilder::accumulate(const TwoElectronBuffer<0, float>& buf, Index fa, Index fb, Index fc,
+                                           Index fd, int na, int nb, int nc, int nd) {
+    accumulate_impl(buf, fa, fb, fc, fd, na, nb, nc, nd);
+    ++n32_;
+}
+
+void MixedPrecisionFockBuilder::accumulate(const TwoElectronBuffer<0, double>& buf, Index fa, Index fb, Index fc,
+                                           Index fd, int na, int nb, int nc, int nd) {
+    accumulate_impl(buf, fa, fb, fc, fd, na, nb, nc, nd);
+    ++n64_;
+}
+
+void MixedPrecisionFockBuilder::add_jk(const Real* J, const Real* K, Size n_fp32, Size n_fp64) {
+    for (Size i = 0; i < nbf_ * nbf_; ++i) {
+        J_[i] += J[i];
+        K_[i] += K[i];
+    }
+    n32_ += n_fp32;
+    n64_ += n_fp64;
+}
+
+std::vector<Real> MixedPrecisionFockBuilder::get_fock_matrix(std::span<const Real> H, Real x) const {
+    if (H.size() != nbf_ * nbf_)
+        throw InvalidArgumentException("MixedPrecisionFockBuilder::get_fock_matrix: H_core size mismatch");
+    std::vector<Real> F(nbf_ * nbf_);
+    for (Size i = 0; i < F.size(); ++i) F[i] = H[i] + J_[i] - x * K_[i];
+    return F;
+}
+
+void MixedPrecisionFockBuilder::reset() noexcept {
+    std::fill(J_.begin(), J_.end(), 0.0);
+    std::fill(K_.begin(), K_.end(), 0.0);
+    n32_ = n64_ = 0;
+}
+
 void FockBuilder::add_jk(const Real* J, const Real* K) {
     for (Size i = 0; i < nbf_ * nbf_; ++i) {
         J_[i] += J[i];

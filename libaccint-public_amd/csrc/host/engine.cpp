@@ -354,6 +354,17 @@ void Engine::jk_into(consumers::FockBuilder& fb, const JKOptions& o) {
     fb.add_jk(J.data(), K.data());
 }
 
+void Engine::jk_into(consumers::MixedPrecisionFockBuilder& fb, const JKOptions& o) {
+    if (fb.nbf() != basis_->n_basis_functions())
+        throw InvalidArgumentException("compute_and_consume: MixedPrecisionFockBuilder nbf does not match the basis");
+    if (!fb.density()) throw InvalidStateException("MixedPrecisionFockBuilder: density matrix not set");
+    const Size n = fb.nbf();
+    std::vector<Real> J(n * n), K(n * n);
+    build_jk_host(o, fb.density(), J.data(), K.data());
+    const Size nq = static_cast<Size>(wl_info_.n_quartets), n32 = static_cast<Size>(wl_info_.n_fp32);
+    fb.add_jk(J.data(), K.data(), n32, nq - n32);
+}
+
 void Engine::jk_into(consumers::GpuFockBuilder& fb, const JKOptions& o) {
     if (fb.nbf() != basis_->n_basis_functions())
         throw InvalidArgumentException("compute_and_consume: GpuFockBuilder nbf does not match the basis");

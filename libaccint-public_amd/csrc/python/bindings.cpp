@@ -15,6 +15,7 @@ using namespace libaccint;
 using consumers::FockBuilder;
 using consumers::FockThreadingStrategy;
 using consumers::GpuFockBuilder;
+using consumers::MixedPrecisionFockBuilder;
 using screening::ScreeningOptions;
 using screening::SchwarzBounds;
 
@@ -190,9 +191,41 @@ PYBIND11_MODULE(_core, m) {
     // ---- buffers -----------------------------------------------------------------
     py::class_<TwoElectronBuffer<0>>(m, "TwoElectronBuffer")
         .def(py::init<>())
+        .def(py::init<int, int, int, int>(), py::arg("na"), py::arg("nb"), py::arg("nc"), py::arg("nd"))
         .def("__call__", [](const TwoElectronBuffer<0>& b, int a, int c, int d, int e) { return b(a, c, d, e); })
+        .def("n_integrals", &TwoElectronBuffer<0>::n_integrals)
         .def("to_numpy", [](const TwoElectronBuffer<0>& b) {
             py::array_t<Real> a({b.na(), b.nb(), b.nc(), b.nd()});
+            std::copy(b.data().begin(), b.data().end(), a.mutable_data());
+            return a;
+        })
+        .def("from_numpy", [](TwoElectronBuffer<0>& b, py::array_t<Real, py::array::c_style | py::array::forcecast> x) {
+            if (x.ndim() != 4) throw InvalidArgumentException("TwoElectronBuffer.from_numpy: expected 4 dimensions");
+            b.resize(int(x.shape(0)), int(x.shape(1)), int(x.shape(2)), int(x.shape(3)));
+            std::copy(x.data(), x.data() + x.size(), b.data().begin());
+        });
+    py::class_<TwoElectronBuffer<0, float>>(m, "TwoElectronBufferF32")
+        .def(py::init<>())
+        .def("__call__", [](const TwoElectronBuffer<0, float>& b, int a, int c, int d, int e) { return b(a, c, d, e); })
+        .def("from_numpy",
+             [](TwoElectronBuffer<0, float>& b, py::array_t<float, py::array::c_style | py::array::forcecast> x) {
+                 if (x.ndim() != 4) throw InvalidArgumentException("TwoElectronBufferF32.from_numpy: expected 4 dimensions");
+                 b.resize(int(x.shape(0)), int(x.shape(1)), int(x.shape(2)), int(x.shape(3)));
+                 std::copy(x.data(), x.data() + x.size(), b.data().begin());
+             })
+        .def("to_numpy", [](const TwoElectronBuffer<0, float>& b) {
+            py::array_t<float> a({b.na(), b.nb(), b.nc(), b.nd()});
+            std::copy(b.data().begin(), b.data().end(), a.mutable_data());
+            return a;
+        });
+    py::class_<TwoElectronBuffer<1>>(m, "TwoElectronGradientBuffer")
+        .def(py::init<int, int, int, int>(), py::arg("na"), py::arg("nb"), py::arg("nc"), py::arg("nd"))
+        .def_property_readonly_static("N_DERIV", [](py::object) { return TwoElectronBuffer<1>::N_DERIV; })
+        .def("__call__", [](const TwoElectronBuffer<1>& b, int a, int c, int d, int e, int k) { return b(a, c, d, e, k); })
+        .def("set", [](TwoElectronBuffer<1>& b, int a, int c, int d, int e, int k, Real v) { b(a, c, d, e, k) = v; })
+        .def("size", &TwoElectronBuffer<1>::size)
+        .def("to_numpy", [](const TwoElectronBuffer<1>& b) {
+            py::array_t<Real> a({TwoElectronBuffer<1>::N_DERIV, b.na(), b.nb(), b.nc(), b.nd()});
             std::copy(b.data().begin(), b.data().end(), a.mutable_data());
             return a;
         });
@@ -270,6 +303,49 @@ PYBIND11_MODULE(_core, m) {
         .def("prepare_parallel", &FockBuilder::prepare_parallel, py::arg("n_threads") = 0)
         .def("finalize_parallel", &FockBuilder::finalize_parallel)
         .def("__repr__", [](const FockBuilder& f) { return "FockBuilder(nbf=" + std::to_string(f.nbf()) + ")"; });
+
+    py::enum_<MixedPrecisionMode>(m, "MixedPrecisionMode")
+        .value("Pure32", MixedPrecisionMode::Pure32)
+        .value("Pure64", MixedPrecisionMode::Pure64)
+        .value("Compute32Accumulate64", MixedPrecisionMode::Compute32Accumulate64)
+        .value("Adaptive", MixedPrecisionMode::Adaptive);
+    py::class_<MixedPrecisionFockBuilder>(m, "MixedPrecisionFockBuilder", py::dynamic_attr())
+        .def(py::init<Size>(), py::arg("nbf"))
+        .def("set_density",
+             [](py::object self, py::array D) {
+                 MixedPrecisionFockBuilder& fb = self.cast<MixedPrecisionFockBuilder&>();
+                 auto a = square(D, fb.nbf(), "MixedPrecisionFockBuilder.set_density");
+                 self.attr("_density_ref") = a;
+                 fb.set_density(a.data(), fb.nbf());
+             },
+             py::arg("D"))
+        .def("accumulate",
+             [](MixedPrecisionFockBuilder& f, const TwoElectronBuffer<0, float>& b, Index fa, Index fb, Index fc,
+                Index fd) { f.accumulate(b, fa, fb, fc, fd, b.na(), b.nb(), b.nc(), b.nd()); },
+             py::arg("buffer"), py::arg("fa"), py::arg("fb"), py::arg("fc"), py::arg("fd"))
+        .def("accumulate",
+             [](MixedPrecisionFockBuilder& f, const TwoElectronBuffer<0>& b, Index fa, Index fb, Index fc, Index fd) {
+                 f.accumulate(b, fa, fb, fc, fd, b.na(), b.nb(), b.nc(), b.nd());
+             },
+             py::arg("buffer"), py::arg("fa"), py::arg("fb"), py::arg("fc"), py::arg("fd"))
+        .def("get_coulomb_matrix", [](const MixedPrecisionFockBuilder& f) {
+            return to_matrix(std::vector<Real>(f.get_coulomb_matrix().begin(), f.get_coulomb_matrix().end()), f.nbf());
+        })
+        .def("get_exchange_matrix", [](const MixedPrecisionFockBuilder& f) {
+            return to_matrix(std::vector<Real>(f.get_exchange_matrix().begin(), f.get_exchange_matrix().end()),
+                             f.nbf());
+        })
+        .def("get_fock_matrix",
+             [](const MixedPrecisionFockBuilder& f, py::array_t<Real, py::array::c_style | py::array::forcecast> H,
+                Real x) { return to_matrix(f.get_fock_matrix(std::span<const Real>(H.data(), H.size()), x), f.nbf()); },
+             py::arg("H_core"), py::arg("exchange_fraction") = 1.0)
+        .def("reset", &MixedPrecisionFockBuilder::reset)
+        .def("nbf", &MixedPrecisionFockBuilder::nbf)
+        .def("mode", &MixedPrecisionFockBuilder::mode)
+        .def("n_float32_accumulations", &MixedPrecisionFockBuilder::n_float32_accumulations)
+        .def("n_float64_accumulations", &MixedPrecisionFockBuilder::n_float64_accumulations)
+        .def("set_fp32_threshold", &MixedPrecisionFockBuilder::set_fp32_threshold, py::arg("tau"))
+        .def("fp32_threshold", &MixedPrecisionFockBuilder::fp32_threshold);
 
     py::class_<GpuFockBuilder>(m, "GpuFockBuilder")
         .def(py::init([](Size nbf, std::uintptr_t stream) {
@@ -423,6 +499,16 @@ PYBIND11_MODULE(_core, m) {
         .def("compute_and_consume",
              [](Engine& e, const Operator& op, GpuFockBuilder& fb, BackendHint h) { e.compute_and_consume(op, fb, h); },
              py::arg("op"), py::arg("consumer"), py::arg("hint") = BackendHint::Auto)
+        .def("compute_and_consume",
+             [](Engine& e, const Operator& op, MixedPrecisionFockBuilder& fb, BackendHint h) {
+                 e.compute_and_consume(op, fb, h);
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("hint") = BackendHint::Auto)
+        .def("compute_and_consume_screened",
+             [](Engine& e, const Operator& op, MixedPrecisionFockBuilder& fb, const ScreeningOptions& o) {
+                 e.compute_and_consume(op, fb, o);
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("options"))
         .def("compute_and_consume_parallel",
              [](Engine& e, const Operator& op, FockBuilder& fb, int n, BackendHint h) {
                  e.compute_and_consume_parallel(op, fb, n, h);

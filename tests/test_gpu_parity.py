@@ -221,6 +221,31 @@ def test_mixed_precision_error_bounds():
     assert np.max(np.abs(Jm - J)) < 1e-6 and np.max(np.abs(Km - K)) < 1e-6
 
 
+def test_mixed_precision_fock_builder_route():
+    """Engine::compute_and_consume with a MixedPrecisionFockBuilder runs the
+    device mixed build: the same J/K as build_jk with mixed_precision, the FP32
+    and FP64 quartet counts of the worklist, and the FP64 build at tau = 0."""
+    atoms = systems.water_cluster(6, 11.0, seed=12)
+    b, e = engine_for("cc-pvdz", atoms)
+    n = b.n_basis_functions()
+    D = systems.density(n, 30, 6, 1.0 / np.sqrt(n))
+    Jm, Km = e.build_jk(D, lai.JKOptions(mixed_precision=True, fp32_threshold=1e-7))
+    info = e.worklist_info()
+    mb = lai.MixedPrecisionFockBuilder(n)
+    mb.set_density(D)
+    e.compute_and_consume(lai.Operator.coulomb(), mb)
+    assert rel(mb.get_coulomb_matrix(), Jm) < 1e-12 and rel(mb.get_exchange_matrix(), Km) < 1e-12
+    assert mb.n_float32_accumulations() == info["n_fp32"] > 0
+    assert mb.n_float32_accumulations() + mb.n_float64_accumulations() == info["n_quartets"]
+    J, K = e.build_jk(D, lai.JKOptions())
+    m0 = lai.MixedPrecisionFockBuilder(n)
+    m0.set_fp32_threshold(0.0)
+    m0.set_density(D)
+    e.compute_and_consume(lai.Operator.coulomb(), m0)
+    assert rel(m0.get_coulomb_matrix(), J) < 1e-12 and rel(m0.get_exchange_matrix(), K) < 1e-12
+    assert m0.n_float32_accumulations() == 0
+
+
 # ------------------------------------------------------------------- graphs
 def test_graph_replays_match_eager():
     torch = pytest.importorskip("torch")
