@@ -105,6 +105,15 @@ int lai_dev_batch_layout(const lai_basis* b, int* n_mol, int* nbf, int64_t* mat_
  * kernels::compute_eri (src/host/kernels/eri_kernel.cpp:168-381). */
 int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* shells4, double* out,
                          int64_t* offsets);
+/* The same blocks written to a device buffer d_out (offsets[n] doubles) on
+ * stream; with d_out NULL only offsets is filled (to size the buffer).
+ * Synchronises the stream.  Replaces CudaEngine::compute_eri_batch_device
+ * without its D2H copy, the device-resident batch that
+ * GpuFockBuilder::accumulate_device_eri_batch consumes
+ * (src/device/cuda/engine/cuda_engine.cu:1233-1277,
+ * src/device/cuda/consumers/fock_builder_gpu.cu:600-621). */
+int lai_dev_eri_quartets_device(lai_basis* b, int64_t n, const int* shells4, double* d_out, int64_t* offsets,
+                                void* stream);
 
 /* ---- Schwarz screening -------------------------------------------------------
  * Q_ij = sqrt(max_ab |(ab|ab)|) for every shell pair i <= j, packed at

@@ -678,19 +678,20 @@ int run_eri_batch(lai_basis* b, EriBatch& eb, double* d_out, hipStream_t st) {
 }
 }  // namespace
 
-extern "C" int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* s4, double* out, int64_t* offsets) {
-  if (!b || n < 0 || (n > 0 && (!s4 || !out || !offsets)))
-    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: invalid arguments");
-  EriBatch eb;
+namespace {
+// Sorts n shell quartets into the class batches and fills offsets[n + 1] (the
+// row-major block of quartet t starts at offsets[t]).
+int classify_batch(lai_basis* b, int64_t n, const int* s4, int64_t* offsets, EriBatch& eb, const char* what) {
   int64_t off = 0;
   for (int64_t t = 0; t < n; ++t) {
     const int* s = s4 + 4 * t;
     for (int k = 0; k < 4; ++k)
-      if (s[k] < 0 || s[k] >= b->ns) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: shell index out of range");
+      if (s[k] < 0 || s[k] >= b->ns) return fail(LAI_ERR_INVALID_ARGUMENT, std::string(what) + ": shell index out of range");
     Quartet q;
     int perm;
     const int c = classify(b, s, q, perm);
-    if (c < 0) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: the shells of a pair must belong to one molecule");
+    if (c < 0)
+      return fail(LAI_ERR_INVALID_ARGUMENT, std::string(what) + ": the shells of a pair must belong to one molecule");
     offsets[t] = off;
     eb.q[c].push_back(q);
     eb.perm[c].push_back(perm);
@@ -698,13 +699,37 @@ extern "C" int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* s4, doub
     off += int64_t(ncart(b->L[s[0]])) * ncart(b->L[s[1]]) * ncart(b->L[s[2]]) * ncart(b->L[s[3]]);
   }
   offsets[n] = off;
+  return LAI_OK;
+}
+}  // namespace
+
+extern "C" int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* s4, double* out, int64_t* offsets) {
+  if (!b || n < 0 || (n > 0 && (!s4 || !out || !offsets)))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets: invalid arguments");
+  EriBatch eb;
+  int rc = classify_batch(b, n, s4, offsets, eb, "lai_dev_eri_quartets");
+  if (rc != LAI_OK) return rc;
+  const int64_t off = offsets[n];
   DevBuf<double> d;
   hipError_t e = d.alloc(size_t(off));
   if (e != hipSuccess) return fail(LAI_ERR_MEMORY, "lai_dev_eri_quartets: output allocation");
-  int rc = run_eri_batch(b, eb, d.p, nullptr);
+  rc = run_eri_batch(b, eb, d.p, nullptr);
   if (rc != LAI_OK) return rc;
   LAI_HIP(hipMemcpy(out, d.p, sizeof(double) * off, hipMemcpyDeviceToHost));
   d.release();
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_eri_quartets_device(lai_basis* b, int64_t n, const int* s4, double* d_out, int64_t* offsets,
+                                           void* stream) {
+  if (!b || n < 0 || !offsets || (n > 0 && !s4))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets_device: invalid arguments");
+  EriBatch eb;
+  int rc = classify_batch(b, n, s4, offsets, eb, "lai_dev_eri_quartets_device");
+  if (rc != LAI_OK || !d_out) return rc;  // no output: offsets only
+  rc = run_eri_batch(b, eb, d_out, hipStream_t(stream));
+  if (rc != LAI_OK) return rc;
+  LAI_HIP(hipStreamSynchronize(hipStream_t(stream)));
   return LAI_OK;
 }
 

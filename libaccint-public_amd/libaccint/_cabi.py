@@ -46,6 +46,7 @@ _SIGNATURES = {
     "lai_dev_upload_batch": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_batch_layout": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_eri_quartets": ([_vp, _i64, _ip, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
+    "lai_dev_eri_quartets_device": ([_vp, _i64, _ip, _vp, ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "lai_dev_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_set_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_build_worklist": ([_vp, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,

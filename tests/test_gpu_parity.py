@@ -103,6 +103,34 @@ def test_eri_blocks_all_classes_vs_oracle():
         assert np.max(np.abs(got - ref)) <= tol[nr] * scale, (i, j, k, l)
 
 
+def test_device_resident_eri_blocks_equal_host_blocks():
+    """lai_dev_eri_quartets_device: the size query and the device output equal
+    the host-output call bit for bit (same kernels)."""
+    torch = pytest.importorskip("torch")
+    atoms = systems.water_cluster(2, 8.0, seed=3)
+    b, e = engine_for("cc-pvdz", atoms)
+    L = _cabi.lib()
+    hb = ctypes.c_void_p(e.device_basis_handle())
+    rng = np.random.default_rng(9)
+    q = rng.integers(0, b.n_shells(), size=(200, 4)).astype(np.int32)
+    n = len(q)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    _cabi.check(L.lai_dev_eri_quartets_device(hb, n, q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None,
+                                              offs.ctypes.data_as(P64), None))
+    host = np.zeros(int(offs[-1]))
+    offs_h = np.zeros(n + 1, dtype=np.int64)
+    _cabi.check(L.lai_dev_eri_quartets(hb, n, q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       host.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), offs_h.ctypes.data_as(P64)))
+    assert np.array_equal(offs, offs_h)
+    d = torch.zeros(int(offs[-1]), dtype=torch.float64, device="cuda")
+    s = torch.cuda.Stream()
+    _cabi.check(L.lai_dev_eri_quartets_device(hb, n, q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                              ctypes.c_void_p(d.data_ptr()), offs.ctypes.data_as(P64),
+                                              ctypes.c_void_p(s.cuda_stream)))
+    assert np.array_equal(d.cpu().numpy(), host)
+
+
 def test_aug_cc_pvtz_water_vs_pyscf():
     w = np.load(golden("water_aug_cc_pvtz.npz"))
     atoms = [(8, w["geometry_bohr"][0]), (1, w["geometry_bohr"][1]), (1, w["geometry_bohr"][2])]
