@@ -414,8 +414,11 @@ template <int La, int Lb, int Lc, int Ld>
 struct GradPolicy {
   static constexpr int NOUT = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld);
   static constexpr int NT = NOUT <= 256 ? 64 : 128;
-  // register shape while the raised tables and outputs stay small
-  static constexpr bool regs = (La + 2) * (Lb + 2) * (Lc + 2) * (Ld + 1) <= 27 && NOUT <= 27;
+  // register shape while the raised tables and outputs stay small: up to 36
+  // table entries per axis (one wavefront per SIMD, at most a few bytes of
+  // spill); measured faster than the LDS shape there, whose per-quartet
+  // overhead dominates these light classes
+  static constexpr bool regs = (La + 2) * (Lb + 2) * (Lc + 2) * (Ld + 1) <= 36 && NOUT <= 54;
 };
 
 }  // namespace lai
