@@ -1258,13 +1258,13 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
 // Kernel W: NW wavefronts per workgroup, one quartet per wavefront.  The
 // quartets of a chunk share the bra, so the waves share its exchange rows and
 // bra Coulomb block in LDS; each wave builds its own (primitive quartet, root,
-// axis) tables and accumulates its (a, b, c) tasks.  Workgroup barriers stay
-// matched: every step runs the largest primitive count of the NW quartets,
-// waves past their own count build zero tables.
+// axis) tables and accumulates its (a, b, c) tasks.  The waves run
+// independently (each its own primitive count, wavefront-level ordering of
+// its own LDS); workgroup barriers only bracket a chunk's shared rows.
 // --------------------------------------------------------------------------
 template <int La, int Lb, int Lc, int Ld, typename R, int NJ, int NPQ>
 __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket, int ntot,
-                                                 int ntot_max, R* tab, R* hs, int lane, const int (&ofs)[NJ],
+                                                 R* tab, R* hs, int lane, const int (&ofs)[NJ],
                                                  R (&acc)[NJ][ncart(Ld)]) {
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
   constexpr int nd = ncart(Ld);
@@ -1277,8 +1277,8 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
   const int npp_k = ket.npp_sig;
   constexpr bool fp32 = sizeof(R) == 4;
   const float qcut = fp32 ? float(g.pcut32 * bra.bsum * ket.bsum) : 0.0f;
-  __syncthreads();  // the previous user of tab and hs has finished
-  for (int t0 = 0; t0 < ntot_max; t0 += NPQ) {
+  wave_sync();  // the previous user of tab and hs has finished
+  for (int t0 = 0; t0 < ntot; t0 += NPQ) {
     if (lane < NPQ * TASKS) {
       const int pq = lane / TASKS, task = lane - pq * TASKS;
       const int r = task / 3, ax = task - 3 * r;
@@ -1313,7 +1313,7 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
         for (int e = 0; e < S; ++e) tb[e] = R(0);  // contributes nothing
       }
     }
-    __syncthreads();
+    wave_sync();
     const int rem = ntot - t0;
     const int npq = rem < NPQ ? (rem > 0 ? rem : 0) : NPQ;
 #pragma unroll 1
@@ -1337,7 +1337,7 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
         }
       }
     }
-    __syncthreads();  // products finished with tab before the next batch
+    wave_sync();  // products finished with tab before the next batch
   }
 }
 
@@ -1368,7 +1368,6 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
   __shared__ double tab[NW][NPQ * Sh::TS];
   __shared__ double hs[NW][NPQ * Sh::HSS];
   __shared__ double blk[NW][NBLK];  // per wave: the quartet's J/K blocks (jcd only, with LDS rows)
-  __shared__ int kinfo[NW][4];      // per wave: ket functions fc, fd, flags, ntot
   extern __shared__ double dyn[];   // exchange rows [na + nb][mn], then the bra block [na*nb]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
   const bool lds = g.klds != 0;
@@ -1390,31 +1389,19 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
     if (lds)
       for (int e = tid; e < ROWS * n + na * nb; e += NT) dyn[e] = 0.0;
     for (int e = tid; e < NW * NBLK; e += NT) (&blk[0][0])[e] = 0.0;
+    __syncthreads();  // rows and blocks are clear
+    double* bw = blk[wave];
     int seg = 0;
-    for (int kb = ch.k0; kb < ch.k1; kb += NW) {
-      const int k = kb + wave;
-      int kp = ch.bra, mode = 0;
-      if (k < ch.k1) {
-        kp = chunk_ket(g.kets, sg, k, seg);
-        mode = quartet_select(g.sel, ch.bra, bra, kp, g.pairs[kp]);
-      }
+    for (int k = ch.k0 + wave; k < ch.k1; k += NW) {  // uniform over the wavefront
+      const int kp = chunk_ket(g.kets, sg, k, seg);
       const PairRec ket = g.pairs[kp];
-      const int ntot = mode ? bra.npp_sig * ket.npp_sig : 0;
-      if (lane == 0) {
-        kinfo[wave][0] = ket.fa;
-        kinfo[wave][1] = ket.fb;
-        kinfo[wave][2] = (mode ? 1 : 0) | (kp == ch.bra ? 2 : 0);
-        kinfo[wave][3] = ntot;
-      }
-      __syncthreads();
-      int ntot_max = 0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) ntot_max = max(ntot_max, kinfo[w][3]);
+      const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
+      if (mode == 0) continue;
+      const int ntot = bra.npp_sig * ket.npp_sig;
       double v[NJ][nd];
       if (MIXED && mode == 2) {
         float acc[NJ][nd];
-        wave_quartet_acc<La, Lb, Lc, Ld, float, NJ, NPQ>(g, bra, ket, ntot, ntot_max,
-                                                         reinterpret_cast<float*>(tab[wave]),
+        wave_quartet_acc<La, Lb, Lc, Ld, float, NJ, NPQ>(g, bra, ket, ntot, reinterpret_cast<float*>(tab[wave]),
                                                          reinterpret_cast<float*>(hs[wave]), lane, ofs, acc);
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
@@ -1422,58 +1409,52 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
           for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       } else {
         double acc[NJ][nd];
-        wave_quartet_acc<La, Lb, Lc, Ld, double, NJ, NPQ>(g, bra, ket, ntot, ntot_max, tab[wave], hs[wave], lane,
-                                                          ofs, acc);
+        wave_quartet_acc<La, Lb, Lc, Ld, double, NJ, NPQ>(g, bra, ket, ntot, tab[wave], hs[wave], lane, ofs, acc);
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
           for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       }
-      if (mode) {
-        const int fc = ket.fa, fd = ket.fb;
-        const bool kl = ket.same, bk = (kp == ch.bra);
-        const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
-        const double fK = bk ? 1.0 : 2.0;
-        double* bw = blk[wave];
+      const int fc = ket.fa, fd = ket.fb;
+      const bool kl = ket.same, bk = (kp == ch.bra);
+      const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
+      const double fK = bk ? 1.0 : 2.0;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j)
+      for (int j = 0; j < NJ; ++j)
 #pragma unroll
-          for (int d = 0; d < nd; ++d) {
-            const int t = lane + 64 * j;
-            if (t >= NTASK) continue;
-            const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-            const double x = v[j][d] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
-            const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
-            const double jv = fJ * x * D[la * n + si];
-            const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
-            const double k1 = fK * x * D[nu * n + si];
-            const double k2 = kl ? 0.0 : fK * x * D[nu * n + la];
-            const double k3 = ij ? 0.0 : fK * x * D[mu * n + si];
-            const double k4 = (ij || kl) ? 0.0 : fK * x * D[mu * n + la];
-            if (!bk) lds_add(&bw[OCD + c * nd + d], jw);
-            if (lds) {
-              lds_add(&jab[a * nb + b], jv);
-              lds_add(&krow[a * n + la], k1);
-              if (!kl) lds_add(&krow[a * n + si], k2);
-              if (!ij) lds_add(&krow[(na + b) * n + la], k3);
-              if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
-            } else {
-              lds_add(&bw[OAB + a * nb + b], jv);
-              lds_add(&bw[OAC + a * nc + c], k1);
-              if (!kl) lds_add(&bw[OAD + a * nd + d], k2);
-              if (!ij) lds_add(&bw[OBC + b * nc + c], k3);
-              if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], k4);
-            }
+        for (int d = 0; d < nd; ++d) {
+          const int t = lane + 64 * j;
+          if (t >= NTASK) continue;
+          const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+          const double x = v[j][d] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
+          const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
+          const double jv = fJ * x * D[la * n + si];
+          const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
+          const double k1 = fK * x * D[nu * n + si];
+          const double k2 = kl ? 0.0 : fK * x * D[nu * n + la];
+          const double k3 = ij ? 0.0 : fK * x * D[mu * n + si];
+          const double k4 = (ij || kl) ? 0.0 : fK * x * D[mu * n + la];
+          if (!bk) lds_add(&bw[OCD + c * nd + d], jw);
+          if (lds) {
+            lds_add(&jab[a * nb + b], jv);
+            lds_add(&krow[a * n + la], k1);
+            if (!kl) lds_add(&krow[a * n + si], k2);
+            if (!ij) lds_add(&krow[(na + b) * n + la], k3);
+            if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
+          } else {
+            lds_add(&bw[OAB + a * nb + b], jv);
+            lds_add(&bw[OAC + a * nc + c], k1);
+            if (!kl) lds_add(&bw[OAD + a * nd + d], k2);
+            if (!ij) lds_add(&bw[OBC + b * nc + c], k3);
+            if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], k4);
           }
-      }
-      __syncthreads();
-      // the waves' quartet blocks to HBM, one atomic per element
-      for (int e = tid; e < NW * NBLK; e += NT) {
-        const int w = e / NBLK, o = e - w * NBLK;
-        const double x = blk[w][o];
+        }
+      wave_sync();  // the wave's quartet block is complete
+      // this wave's quartet block to HBM, one atomic per element
+      for (int o = lane; o < NBLK; o += 64) {
+        const double x = bw[o];
         if (x == 0.0) continue;
-        blk[w][o] = 0.0;
-        const size_t fc = size_t(kinfo[w][0]), fd = size_t(kinfo[w][1]);
+        bw[o] = 0.0;
         size_t r, q;
         double* M = Kt;
         if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; M = Jt; }
@@ -1484,8 +1465,9 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
         else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
         atomic_add(&M[r * n + q], x);
       }
-      __syncthreads();
+      wave_sync();  // the block is clear for the wave's next quartet
     }
+    __syncthreads();  // every wave has added its quartets to the rows
     if (lds) {
       for (int e = tid; e < ROWS * n; e += NT) {
         const double x = krow[e];
@@ -1498,8 +1480,8 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
         const double x = jab[e];
         if (x != 0.0) atomic_add(&Jt[size_t(fa + e / nb) * n + fb + e % nb], x);
       }
-      __syncthreads();  // dyn is cleared for the next chunk
     }
+    __syncthreads();  // dyn and the blocks are cleared for the next chunk
   }
 }
 
