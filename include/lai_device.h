@@ -34,6 +34,7 @@ extern "C" {
 #define LAI_ERR_MEMORY (-3)           /* -> MemoryException / std::runtime_error */
 #define LAI_ERR_BACKEND (-4)          /* -> BackendError (HIP runtime failure) */
 #define LAI_ERR_UNSUPPORTED (-5)      /* -> std::invalid_argument (angular momentum) */
+#define LAI_ERR_NUMERIC (-6)          /* -> std::runtime_error (metric not positive definite) */
 
 /* J/K build flags */
 #define LAI_JK_FP64 0  /* every quartet in FP64 */
@@ -43,6 +44,7 @@ extern "C" {
 
 typedef struct lai_ctx lai_ctx;
 typedef struct lai_basis lai_basis;
+typedef struct lai_df lai_df;
 
 /* ---- context --------------------------------------------------------------
  * Replaces CudaEngine construction (src/device/cuda/engine/cuda_engine.cu:183-187:
@@ -251,6 +253,46 @@ int lai_dev_jk_grad(lai_basis* b, const double* d_D, double exchange_scale, doub
  * CpuEngine::compute_1e_parallel_impl (src/host/engine/cpu_engine.cpp:482-593). */
 int lai_dev_onee(lai_basis* b, int kind, int n_charges, const double* Z, const double* xyz,
                  double* d_out, void* stream);
+
+/* ---- density-fitted J/K ------------------------------------------------------
+ * The basis handle b holds, as one molecule, the orbital shells
+ * [0, n_orb_shells), the auxiliary shells [n_orb_shells, n_orb_shells +
+ * n_aux_shells) and one unit shell last (L = 0, one primitive, exponent 0,
+ * coefficient 1: the constant function, so (ab|P u) = (ab|P)).  Coefficients
+ * are normalised by the caller as for any shell.  lai_dev_df_create computes
+ * the metric V_PQ (metric 0: Coulomb (P|Q); 1: overlap; 2, attenuated
+ * Coulomb, is LAI_ERR_INVALID_ARGUMENT as in the reference), its diagonal
+ * condition estimate max/min (returned in *cond_estimate when not NULL;
+ * above cond_hard_limit > 0 the call fails with LAI_ERR_INVALID_ARGUMENT),
+ * the Cholesky factor L (LAI_ERR_NUMERIC when V is not positive definite) and
+ * L^-1, the three-centre integrals (ab|P) of every orbital pair and the
+ * fitted tensor B = L^-1 (ab|P), kept in HBM as B[a][P][b] (n_orb^2 n_aux
+ * doubles).  Replaces DFFockBuilder::initialize, compute_metric and
+ * compute_b_tensor (src/shared/consumers/df_fock_builder.cpp:100-235) with
+ * kernels::compute_two_center_metric / compute_three_center_tensor /
+ * compute_B_tensor (src/host/kernels/two_center_coulomb_kernel.cpp,
+ * three_center_eri_kernel.cpp) and math::cholesky_decompose /
+ * triangular_inverse (include/libaccint/math/cholesky.hpp). */
+int lai_dev_df_create(lai_basis* b, int n_orb_shells, int n_aux_shells, int metric, double cond_hard_limit,
+                      double* cond_estimate, lai_df** out);
+int lai_dev_df_free(lai_df* df);
+/* orbital and auxiliary function counts */
+int lai_dev_df_dims(const lai_df* df, int* n_orb, int* n_aux);
+/* flags: LAI_DF_J and/or LAI_DF_K.  J = sum_P B_P gamma_P with
+ * gamma_P = sum_ab B_P,ab D_ab; K = sum_P B_P D B_P^T (D, J, K device
+ * n_orb x n_orb row-major; J, K overwritten; D need not be symmetric).
+ * Asynchronous on stream.  Replaces DFFockBuilder::compute_j_internal /
+ * compute_k_internal (src/shared/consumers/df_fock_builder.cpp:283-378). */
+#define LAI_DF_J 1
+#define LAI_DF_K 2
+int lai_dev_df_jk(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags, void* stream);
+/* as lai_dev_df_jk, synchronous, with each kernel timed by HIP events on
+ * stream: ms[0] the gamma pass, ms[1] the J pass, ms[2] the X = B D GEMM,
+ * ms[3] the K = X B^T GEMM (split over the summation index), ms[4] the sum of
+ * its slices (entries of kernels not run are 0).  No reference counterpart
+ * (measurement). */
+int lai_dev_df_jk_profile(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags, void* stream,
+                          double* ms);
 
 /* ---- math probes (parity tests) ----------------------------------------------
  * Rys roots/weights for n_roots at n points T (device arrays), as used by

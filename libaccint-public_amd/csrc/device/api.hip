@@ -14,6 +14,7 @@
 #include <thread>
 #include <vector>
 
+#include "api_internal.hpp"
 #include "aux_kernels.hpp"
 #include "eri_kernels.hpp"
 #include "grad_kernels.hpp"
@@ -479,6 +480,9 @@ extern "C" int lai_dev_upload_batch(lai_ctx* ctx, int n_mol, const int* mol_shel
         for (int y = 0; y < nprim[sb]; ++y) {
           const double be = exps[b->prim_off[sb] + y], cb = coefs[b->prim_off[sb] + y];
           if (cb == 0.0) continue;
+          // two zero-exponent (unit) functions have no finite product; the
+          // density-fitting basis pairs its unit function only with others
+          if (al + be == 0.0) continue;
           const double zeta = al + be, iz = 1.0 / zeta, mu = al * be * iz;
           PrimPair q;
           q.zeta = zeta;
@@ -599,6 +603,15 @@ extern "C" int lai_dev_batch_layout(const lai_basis* b, int* n_mol, int* nbf, in
   if (mat_off) mat_off[b->n_mol] = b->mol_moff[b->n_mol];
   return LAI_OK;
 }
+
+namespace lai {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+BasisView basis_view(const lai_basis* b) {
+  return BasisView{b->ctx, b->ns, b->nbf, b->n_mol, b->L.data(), b->nprim.data(), b->prim_off.data(),
+                   b->func_off.data(), b->exps.data(), b->coefs.data()};
+}
+}  // namespace lai
 
 // ---------------------------------------------------------------------------
 // raw ERI blocks

@@ -1,0 +1,653 @@
+// df.hip -- density-fitted J/K on the device.
+//
+// Set-up (lai_dev_df_create): the auxiliary metric through the raw ERI
+// kernels ((P u|Q u) with the unit function u) or the overlap kernel, its
+// Cholesky factor and triangular inverse on the host (n_aux^3 / 3 flops each,
+// once), the three-centre integrals (ab|P u) of every orbital shell pair in
+// bounded batches through the raw ERI kernels, scattered into
+// T[a][P][b], and B[a] = L^-1 T[a] for every orbital function a (one batched
+// FP64 MFMA GEMM).  B stays in HBM in the layout [a][P][b]: row a of B is a
+// contiguous n_aux x n matrix, so both exchange products are single GEMMs.
+//
+// Per build (lai_dev_df_jk): J in two HBM-bound passes over B (gamma_P, then
+// J_ab = sum_P B[a][P][b] gamma_P); K = sum_P B_P D B_P^T as
+// X[a][P][:] = B[a][P][:] D (one GEMM, rows (a, P)) and
+// K[a][c] = sum_(P,b) X[a][P][b] B[c][P][b] (one GEMM over the summation
+// index n_aux n, split into slices that are summed by a last kernel).
+#include "lai_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "api_internal.hpp"
+
+namespace {
+
+using lai::set_error;
+
+#define DF_HIP(call)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return set_error(LAI_ERR_BACKEND, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+constexpr int ncart_of(int L) { return (L + 1) * (L + 2) / 2; }
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM on the matrix cores: C = A B with A row-major M x K and B either
+// row-major K x N (bt = 0) or given as its transpose, row-major N x K
+// (bt = 1).  A workgroup of four wavefronts computes a 64 x 64 tile of C;
+// each wavefront a 32 x 32 quadrant as 2 x 2 blocks of
+// v_mfma_f64_16x16x4f64.  The K loop stages 64 x 16 slices of A and B through
+// LDS, the next slice's global loads in flight while the current one runs.
+// blockIdx.z is a batch index (split = 0: A, B, C offset by sA, sB, sC) or a
+// slice of the summation index (split = 1: columns [z kchunk, (z+1) kchunk)
+// of A and rows of B, written to C + z sC).
+// Lane layout of v_mfma_f64_16x16x4f64 on gfx950 (measured with one
+// wavefront of marked inputs): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[l / 16 + 4 r][l % 16] in accumulator element r.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int GT = 64;  // tile edge of C
+constexpr int KT = 16;  // summation step
+
+struct GemmArgs {
+  const double* A;
+  const double* B;
+  double* C;
+  long long lda, ldb, ldc;
+  long long sA, sB, sC;
+  int M, N, K;
+  int bt, split, kchunk;
+};
+
+__global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs g) {
+  __shared__ double As[GT][KT + 1];
+  __shared__ double Bs[KT][GT + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT, z = blockIdx.z;
+  const double* A = g.A;
+  const double* B = g.B;
+  double* C = g.C + z * g.sC;
+  int kb = 0, ke = g.K;
+  if (g.split) {
+    kb = z * g.kchunk;
+    ke = min(g.K, kb + g.kchunk);
+  } else {
+    A += z * g.sA;
+    B += z * g.sB;
+  }
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + 256 * r;
+      {  // A: 64 rows x 16 columns, 16 lanes per row
+        const int row = e >> 4, col = e & 15;
+        const int gm = m0 + row, gk = k0 + col;
+        ra[r] = (gm < g.M && gk < ke) ? A[(long long)gm * g.lda + gk] : 0.0;
+      }
+      if (g.bt) {  // B^T rows: 64 columns of B x 16 summation entries
+        const int nn = e >> 4, kk = e & 15;
+        const int gn = n0 + nn, gk = k0 + kk;
+        rb[r] = (gn < g.N && gk < ke) ? B[(long long)gn * g.ldb + gk] : 0.0;
+      } else {  // B rows: 16 summation entries x 64 columns
+        const int kk = e >> 6, nn = e & 63;
+        const int gn = n0 + nn, gk = k0 + kk;
+        rb[r] = (gn < g.N && gk < ke) ? B[(long long)gk * g.ldb + gn] : 0.0;
+      }
+    }
+  };
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += KT) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + 256 * r;
+      As[e >> 4][e & 15] = ra[r];
+      if (g.bt)
+        Bs[e & 15][e >> 4] = rb[r];
+      else
+        Bs[e >> 6][e & 63] = rb[r];
+    }
+    __syncthreads();
+    if (k0 + KT < ke) load(k0 + KT);
+#pragma unroll
+    for (int kk = 0; kk < KT; kk += 4) {
+      double a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[wm + 16 * i + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wn + 16 * j + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+        const int col = n0 + wn + 16 * j + (lane & 15);
+        if (row < g.M && col < g.N) C[(long long)row * g.ldc + col] = acc[i][j][r];
+      }
+}
+
+// out[i] = sum over the ns slices part[z slice + i]
+__global__ __launch_bounds__(256) void sum_slices(const double* __restrict__ part, long long slice, int ns,
+                                                  long long n, double* __restrict__ out) {
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += 256ll * gridDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < ns; ++z) s += part[z * slice + i];
+    out[i] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coulomb passes over B[a][P][b] (both HBM-bound: one read of B each).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// part[a][P] = sum_b B[a][P][b] D[a][b]; one workgroup per a, one wavefront
+// per P row at a time (lanes along b, coalesced).
+__global__ __launch_bounds__(256) void df_gamma_part(const double* __restrict__ B, const double* __restrict__ D,
+                                                     int n, int naux, double* __restrict__ part) {
+  const int a = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double* Da = D + (long long)a * n;
+  const double* Ba = B + (long long)a * naux * n;
+  for (int P = wave; P < naux; P += 4) {
+    const double* row = Ba + (long long)P * n;
+    double s = 0.0;
+    for (int b = lane; b < n; b += 64) s = fma(row[b], Da[b], s);
+    s = wave_sum(s);
+    if (lane == 0) part[(long long)a * naux + P] = s;
+  }
+}
+
+// gamma[P] = sum_a part[a][P]
+__global__ __launch_bounds__(256) void df_gamma_sum(const double* __restrict__ part, int n, int naux,
+                                                    double* __restrict__ gamma) {
+  const int P = blockIdx.x * 256 + threadIdx.x;
+  if (P >= naux) return;
+  double s = 0.0;
+  for (int a = 0; a < n; ++a) s += part[(long long)a * naux + P];
+  gamma[P] = s;
+}
+
+// J[a][b] = sum_P B[a][P][b] gamma[P]; workgroup (b block, a), lanes along b.
+__global__ __launch_bounds__(256) void df_coulomb(const double* __restrict__ B, const double* __restrict__ gamma,
+                                                  int n, int naux, double* __restrict__ J) {
+  const int a = blockIdx.y, b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= n) return;
+  const double* Ba = B + (long long)a * naux * n + b;
+  double s = 0.0;
+  for (int P = 0; P < naux; ++P) s = fma(Ba[(long long)P * n], gamma[P], s);
+  J[(long long)a * n + b] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Three-centre scatter: raw blocks of the quartets (a b|P u) of one orbital
+// shell pair, P over all auxiliary shells in order, start at base; the block
+// of auxiliary shell P starts na nb ps(P) further (ps = the shell's first
+// auxiliary function) and is row-major [na][nb][nP].  Writes T[a][P][b] and,
+// for a != b, T[b][P][a].
+// ---------------------------------------------------------------------------
+struct Pair3 {
+  int fa, fb, na, nb;
+  long long base;
+  int same, pad;
+};
+
+__global__ __launch_bounds__(256) void df_scatter3(const double* __restrict__ blk, const Pair3* __restrict__ prs,
+                                                   const int* __restrict__ pstart, const int* __restrict__ pn,
+                                                   int naux, int n, double* __restrict__ T) {
+  const Pair3 q = prs[blockIdx.x];
+  const int tot = q.na * q.nb * naux;
+  for (int e = threadIdx.x; e < tot; e += 256) {
+    const int p = e % naux, ij = e / naux;
+    const int i = ij / q.nb, j = ij - i * q.nb;
+    const int ps = pstart[p], np = pn[p];
+    const double v = blk[q.base + (long long)q.na * q.nb * ps + (long long)(i * q.nb + j) * np + (p - ps)];
+    T[((long long)(q.fa + i) * naux + p) * n + q.fb + j] = v;
+    if (!q.same) T[((long long)(q.fb + j) * naux + p) * n + q.fa + i] = v;
+  }
+}
+
+template <typename T>
+struct Dev {
+  T* p = nullptr;
+  ~Dev() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  hipError_t alloc(size_t n) {
+    release();
+    return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1));
+  }
+  T* take() {
+    T* q = p;
+    p = nullptr;
+    return q;
+  }
+};
+
+int gemm(const GemmArgs& g, int gz, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0 || gz <= 0) return LAI_OK;
+  const dim3 grid((g.M + GT - 1) / GT, (g.N + GT - 1) / GT, gz);
+  if (grid.y > 65535u || grid.z > 65535u)
+    return set_error(LAI_ERR_INVALID_ARGUMENT, "density fitting: GEMM grid beyond the launch limits");
+  hipLaunchKernelGGL(dgemm_mfma, grid, dim3(256), 0, st, g);
+  DF_HIP(hipGetLastError());
+  return LAI_OK;
+}
+
+// Lower Cholesky factor of the row-major SPD matrix A in place (upper part
+// zeroed): the row-oriented algorithm of math::cholesky_decompose
+// (include/libaccint/math/cholesky.hpp), which fails when a pivot is <= 0.
+bool cholesky_lower(std::vector<double>& A, int n) {
+  for (int i = 0; i < n; ++i) {
+    double* Ai = &A[size_t(i) * n];
+    double s = Ai[i];
+    for (int k = 0; k < i; ++k) s -= Ai[k] * Ai[k];
+    if (!(s > 0.0)) return false;
+    Ai[i] = std::sqrt(s);
+    for (int j = i + 1; j < n; ++j) {
+      double* Aj = &A[size_t(j) * n];
+      double t = Aj[i];
+      for (int k = 0; k < i; ++k) t -= Aj[k] * Ai[k];
+      Aj[i] = t / Ai[i];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) A[size_t(i) * n + j] = 0.0;
+  return true;
+}
+
+// X = L^-1 for lower-triangular L (row-major), by forward substitution row by
+// row: X[i][j] = (delta_ij - sum_{k=j}^{i-1} L[i][k] X[k][j]) / L[i][i], the
+// sum taken in increasing k as in math::triangular_inverse.
+void lower_inverse(const std::vector<double>& L, std::vector<double>& X, int n) {
+  X.assign(size_t(n) * n, 0.0);
+  std::vector<double> acc(n);
+  for (int i = 0; i < n; ++i) {
+    const double* Li = &L[size_t(i) * n];
+    std::fill(acc.begin(), acc.begin() + i + 1, 0.0);
+    acc[i] = 1.0;
+    for (int k = 0; k < i; ++k) {
+      const double l = Li[k];
+      const double* Xk = &X[size_t(k) * n];
+      for (int j = 0; j <= k; ++j) acc[j] -= l * Xk[j];
+    }
+    double* Xi = &X[size_t(i) * n];
+    for (int j = 0; j <= i; ++j) Xi[j] = acc[j] / Li[i];
+  }
+}
+
+}  // namespace
+
+struct lai_df {
+  int n = 0, naux = 0;
+  int nsplit = 1, kchunk = 0;
+  double* B = nullptr;     // [n][naux][n]
+  double* X = nullptr;     // [n][naux][n]
+  double* part = nullptr;  // max(n * naux, nsplit * n * n)
+  double* gamma = nullptr; // [naux]
+  hipEvent_t ev[6] = {};
+  bool have_ev = false;
+};
+
+namespace {
+void df_release(lai_df* d) {
+  if (!d) return;
+  if (d->B) (void)hipFree(d->B);
+  if (d->X) (void)hipFree(d->X);
+  if (d->part) (void)hipFree(d->part);
+  if (d->gamma) (void)hipFree(d->gamma);
+  if (d->have_ev)
+    for (auto& e : d->ev) (void)hipEventDestroy(e);
+  delete d;
+}
+}  // namespace
+
+extern "C" int lai_dev_df_free(lai_df* df) {
+  df_release(df);
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_df_dims(const lai_df* df, int* n_orb, int* n_aux) {
+  if (!df) return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_dims: null handle");
+  if (n_orb) *n_orb = df->n;
+  if (n_aux) *n_aux = df->naux;
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_df_create(lai_basis* b, int n_orb_shells, int n_aux_shells, int metric,
+                                 double cond_hard_limit, double* cond_estimate, lai_df** out) {
+  if (!b || !out || n_orb_shells < 1 || n_aux_shells < 1)
+    return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_create: invalid arguments");
+  *out = nullptr;
+  const lai::BasisView v = lai::basis_view(b);
+  if (v.n_mol != 1 || v.ns != n_orb_shells + n_aux_shells + 1)
+    return set_error(LAI_ERR_INVALID_ARGUMENT,
+                     "lai_dev_df_create: the handle must hold one molecule of orbital shells, auxiliary shells "
+                     "and one unit shell");
+  const int u = v.ns - 1;
+  if (v.L[u] != 0 || v.nprim[u] != 1 || v.exps[v.prim_off[u]] != 0.0 || v.coefs[v.prim_off[u]] != 1.0)
+    return set_error(LAI_ERR_INVALID_ARGUMENT,
+                     "lai_dev_df_create: the last shell must be the unit s function (exponent 0, coefficient 1)");
+  if (metric == 2)
+    return set_error(LAI_ERR_INVALID_ARGUMENT,
+                     "AttenuatedCoulomb metric is not yet supported. Use Coulomb or Overlap metric instead.");
+  if (metric != 0 && metric != 1) return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_create: unknown metric");
+  const int n = v.func_off[n_orb_shells];
+  const int naux = v.func_off[u] - n;
+  const int a0 = n_orb_shells;  // first auxiliary shell
+
+  // ---- metric V (host, naux x naux) ----
+  std::vector<double> V(size_t(naux) * naux, 0.0);
+  if (metric == 0) {
+    std::vector<int> s4;
+    for (int P = 0; P < n_aux_shells; ++P)
+      for (int Q = P; Q < n_aux_shells; ++Q) {
+        const int q[4] = {a0 + P, u, a0 + Q, u};
+        s4.insert(s4.end(), q, q + 4);
+      }
+    const int64_t nq = int64_t(s4.size() / 4);
+    std::vector<int64_t> off(nq + 1);
+    int64_t tot = 0;
+    for (int P = 0; P < n_aux_shells; ++P)
+      for (int Q = P; Q < n_aux_shells; ++Q) tot += int64_t(ncart_of(v.L[a0 + P])) * ncart_of(v.L[a0 + Q]);
+    std::vector<double> blk(static_cast<size_t>(tot));
+    int rc = lai_dev_eri_quartets(b, nq, s4.data(), blk.data(), off.data());
+    if (rc != LAI_OK) return rc;
+    int64_t t = 0;
+    for (int P = 0; P < n_aux_shells; ++P)
+      for (int Q = P; Q < n_aux_shells; ++Q, ++t) {
+        const int fP = v.func_off[a0 + P] - n, fQ = v.func_off[a0 + Q] - n;
+        const int nP = ncart_of(v.L[a0 + P]), nQ = ncart_of(v.L[a0 + Q]);
+        for (int i = 0; i < nP; ++i)
+          for (int j = 0; j < nQ; ++j) {
+            const double x = blk[size_t(off[t]) + size_t(i) * nQ + j];
+            V[size_t(fP + i) * naux + fQ + j] = x;
+            V[size_t(fQ + j) * naux + fP + i] = x;
+          }
+      }
+  } else {
+    Dev<double> dS;
+    DF_HIP(dS.alloc(size_t(v.nbf) * v.nbf));
+    int rc = lai_dev_onee(b, 0, 0, nullptr, nullptr, dS.p, nullptr);
+    if (rc != LAI_OK) return rc;
+    DF_HIP(hipDeviceSynchronize());
+    std::vector<double> S(size_t(v.nbf) * v.nbf);
+    DF_HIP(hipMemcpy(S.data(), dS.p, sizeof(double) * S.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < naux; ++i)
+      for (int j = 0; j < naux; ++j) V[size_t(i) * naux + j] = S[size_t(n + i) * v.nbf + n + j];
+  }
+  double dmax = 0.0, dmin = HUGE_VAL;
+  for (int i = 0; i < naux; ++i) {
+    const double d = V[size_t(i) * naux + i];
+    dmax = std::max(dmax, d);
+    if (d > 0.0) dmin = std::min(dmin, d);
+  }
+  const double cond = dmin < HUGE_VAL ? dmax / dmin : HUGE_VAL;
+  if (cond_estimate) *cond_estimate = cond;
+  if (cond_hard_limit > 0.0 && cond > cond_hard_limit)
+    return set_error(LAI_ERR_INVALID_ARGUMENT,
+                     "DF metric matrix is severely ill-conditioned (condition estimate = " + std::to_string(cond) +
+                         ", hard limit = " + std::to_string(cond_hard_limit) + ")");
+  if (!cholesky_lower(V, naux))
+    return set_error(LAI_ERR_NUMERIC, "Cholesky decomposition failed: matrix is not positive definite");
+  std::vector<double> Linv;
+  lower_inverse(V, Linv, naux);
+  V.clear();
+  V.shrink_to_fit();
+
+  // ---- three-centre integrals T[a][P][b] ----
+  std::vector<int> pstart(naux), pn(naux);
+  for (int P = 0; P < n_aux_shells; ++P) {
+    const int f = v.func_off[a0 + P] - n, m = ncart_of(v.L[a0 + P]);
+    for (int i = 0; i < m; ++i) {
+      pstart[f + i] = f;
+      pn[f + i] = m;
+    }
+  }
+  const size_t nB = size_t(n) * naux * n;
+  Dev<double> dT, dLinv, dB;
+  Dev<int> dps, dpn;
+  DF_HIP(dT.alloc(nB));
+  DF_HIP(dps.alloc(naux));
+  DF_HIP(dpn.alloc(naux));
+  DF_HIP(hipMemcpy(dps.p, pstart.data(), sizeof(int) * naux, hipMemcpyHostToDevice));
+  DF_HIP(hipMemcpy(dpn.p, pn.data(), sizeof(int) * naux, hipMemcpyHostToDevice));
+  {
+    // orbital shell pairs in batches of at most ~4M quartets / 64M doubles
+    const int64_t kMaxQ = int64_t(1) << 22, kMaxD = int64_t(1) << 26;
+    std::vector<std::pair<int, int>> pairs;
+    for (int i = 0; i < n_orb_shells; ++i)
+      for (int j = i; j < n_orb_shells; ++j) pairs.emplace_back(i, j);
+    size_t p0 = 0;
+    Dev<double> dblk;
+    Dev<Pair3> dpr;
+    int64_t blk_cap = 0, pr_cap = 0;
+    while (p0 < pairs.size()) {
+      size_t p1 = p0;
+      int64_t nq = 0, nd = 0;
+      while (p1 < pairs.size()) {
+        const int64_t d = int64_t(ncart_of(v.L[pairs[p1].first])) * ncart_of(v.L[pairs[p1].second]) * naux;
+        if (p1 > p0 && (nq + n_aux_shells > kMaxQ || nd + d > kMaxD)) break;
+        nq += n_aux_shells;
+        nd += d;
+        ++p1;
+      }
+      std::vector<int> s4;
+      s4.reserve(size_t(nq) * 4);
+      std::vector<Pair3> prs;
+      int64_t base = 0;
+      for (size_t p = p0; p < p1; ++p) {
+        const int i = pairs[p].first, j = pairs[p].second;
+        Pair3 r{};
+        r.fa = v.func_off[i];
+        r.fb = v.func_off[j];
+        r.na = ncart_of(v.L[i]);
+        r.nb = ncart_of(v.L[j]);
+        r.base = base;
+        r.same = (i == j);
+        prs.push_back(r);
+        base += int64_t(r.na) * r.nb * naux;
+        for (int P = 0; P < n_aux_shells; ++P) {
+          const int q[4] = {i, j, a0 + P, u};
+          s4.insert(s4.end(), q, q + 4);
+        }
+      }
+      if (nd > blk_cap) {
+        DF_HIP(dblk.alloc(size_t(nd)));
+        blk_cap = nd;
+      }
+      if (int64_t(prs.size()) > pr_cap) {
+        DF_HIP(dpr.alloc(prs.size()));
+        pr_cap = int64_t(prs.size());
+      }
+      std::vector<int64_t> off(size_t(nq) + 1);
+      int rc = lai_dev_eri_quartets_device(b, nq, s4.data(), dblk.p, off.data(), nullptr);
+      if (rc != LAI_OK) return rc;
+      if (off[size_t(nq)] != nd)
+        return set_error(LAI_ERR_BACKEND, "lai_dev_df_create: three-centre block layout mismatch");
+      DF_HIP(hipMemcpy(dpr.p, prs.data(), sizeof(Pair3) * prs.size(), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(df_scatter3, dim3(unsigned(prs.size())), dim3(256), 0, nullptr, dblk.p, dpr.p, dps.p, dpn.p,
+                         naux, n, dT.p);
+      DF_HIP(hipGetLastError());
+      DF_HIP(hipDeviceSynchronize());
+      p0 = p1;
+    }
+  }
+
+  // ---- B[a] = L^-1 T[a] ----
+  DF_HIP(dLinv.alloc(size_t(naux) * naux));
+  DF_HIP(hipMemcpy(dLinv.p, Linv.data(), sizeof(double) * Linv.size(), hipMemcpyHostToDevice));
+  DF_HIP(dB.alloc(nB));
+  {
+    GemmArgs g{};
+    g.A = dLinv.p;
+    g.B = dT.p;
+    g.C = dB.p;
+    g.lda = naux;
+    g.ldb = n;
+    g.ldc = n;
+    g.sA = 0;
+    g.sB = (long long)naux * n;
+    g.sC = (long long)naux * n;
+    g.M = naux;
+    g.N = n;
+    g.K = naux;
+    int rc = gemm(g, n, nullptr);
+    if (rc != LAI_OK) return rc;
+    DF_HIP(hipDeviceSynchronize());
+  }
+  dT.release();
+  dLinv.release();
+
+  auto* d = new lai_df;
+  d->n = n;
+  d->naux = naux;
+  // slices of the exchange GEMM: about 8 workgroups per CU over the 64 x 64 tiles
+  const long long tiles = (long long)((n + GT - 1) / GT) * ((n + GT - 1) / GT);
+  const long long ksum = (long long)naux * n;
+  int ns = int(std::max(1ll, std::min(2048ll / tiles, (ksum + 16 * KT - 1) / (16 * KT))));
+  long long kc = (ksum + ns - 1) / ns;
+  kc = (kc + KT - 1) / KT * KT;
+  ns = int((ksum + kc - 1) / kc);
+  d->nsplit = ns;
+  d->kchunk = int(kc);
+  Dev<double> dX, dpart, dg;
+  hipError_t e;
+  if ((e = dX.alloc(nB)) != hipSuccess ||
+      (e = dpart.alloc(std::max(size_t(n) * naux, size_t(ns) * n * n))) != hipSuccess ||
+      (e = dg.alloc(size_t(naux))) != hipSuccess) {
+    delete d;
+    return set_error(LAI_ERR_MEMORY, std::string("lai_dev_df_create: ") + hipGetErrorString(e));
+  }
+  d->B = dB.take();
+  d->X = dX.take();
+  d->part = dpart.take();
+  d->gamma = dg.take();
+  *out = d;
+  return LAI_OK;
+}
+
+namespace {
+int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStream_t st, double* ms) {
+  const int n = d->n, naux = d->naux;
+  auto mark = [&](int i) -> int {
+    if (ms) DF_HIP(hipEventRecord(d->ev[i], st));
+    return LAI_OK;
+  };
+  if (ms && !d->have_ev) {
+    for (auto& e : d->ev) DF_HIP(hipEventCreate(&e));
+    d->have_ev = true;
+  }
+  if (ms)
+    for (int i = 0; i < 5; ++i) ms[i] = 0.0;
+  if (flags & LAI_DF_J) {
+    int rc = mark(0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(df_gamma_part, dim3(n), dim3(256), 0, st, d->B, D, n, naux, d->part);
+    DF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 255) / 256), dim3(256), 0, st, d->part, n, naux, d->gamma);
+    DF_HIP(hipGetLastError());
+    if ((rc = mark(1))) return rc;
+    hipLaunchKernelGGL(df_coulomb, dim3((n + 255) / 256, n), dim3(256), 0, st, d->B, d->gamma, n, naux, J);
+    DF_HIP(hipGetLastError());
+    if ((rc = mark(2))) return rc;
+    if (ms) {
+      float t0, t1;
+      DF_HIP(hipEventSynchronize(d->ev[2]));
+      DF_HIP(hipEventElapsedTime(&t0, d->ev[0], d->ev[1]));
+      DF_HIP(hipEventElapsedTime(&t1, d->ev[1], d->ev[2]));
+      ms[0] = t0;
+      ms[1] = t1;
+    }
+  }
+  if (flags & LAI_DF_K) {
+    int rc = mark(0);
+    if (rc) return rc;
+    GemmArgs g{};
+    g.A = d->B;  // rows (a, P), n columns
+    g.B = D;
+    g.C = d->X;
+    g.lda = n;
+    g.ldb = n;
+    g.ldc = n;
+    g.M = n * naux;
+    g.N = n;
+    g.K = n;
+    if ((rc = gemm(g, 1, st))) return rc;
+    if ((rc = mark(1))) return rc;
+    GemmArgs h{};
+    h.A = d->X;  // rows a, summation (P, b)
+    h.B = d->B;  // given transposed: rows c, summation (P, b)
+    h.C = d->part;
+    h.lda = (long long)naux * n;
+    h.ldb = (long long)naux * n;
+    h.ldc = n;
+    h.sC = (long long)n * n;
+    h.M = n;
+    h.N = n;
+    h.K = naux * n;
+    h.bt = 1;
+    h.split = 1;
+    h.kchunk = d->kchunk;
+    if ((rc = gemm(h, d->nsplit, st))) return rc;
+    if ((rc = mark(2))) return rc;
+    const long long nn = (long long)n * n;
+    hipLaunchKernelGGL(sum_slices, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, d->part, nn,
+                       d->nsplit, nn, K);
+    DF_HIP(hipGetLastError());
+    if ((rc = mark(3))) return rc;
+    if (ms) {
+      float t0, t1, t2;
+      DF_HIP(hipEventSynchronize(d->ev[3]));
+      DF_HIP(hipEventElapsedTime(&t0, d->ev[0], d->ev[1]));
+      DF_HIP(hipEventElapsedTime(&t1, d->ev[1], d->ev[2]));
+      DF_HIP(hipEventElapsedTime(&t2, d->ev[2], d->ev[3]));
+      ms[2] = t0;
+      ms[3] = t1;
+      ms[4] = t2;
+    }
+  }
+  return LAI_OK;
+}
+}  // namespace
+
+extern "C" int lai_dev_df_jk(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
+  if (!df || !d_D || (flags & ~(LAI_DF_J | LAI_DF_K)) || ((flags & LAI_DF_J) && !d_J) || ((flags & LAI_DF_K) && !d_K))
+    return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_jk: invalid arguments");
+  return df_run(df, d_D, d_J, d_K, flags, hipStream_t(stream), nullptr);
+}
+
+extern "C" int lai_dev_df_jk_profile(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags,
+                                     void* stream, double* ms) {
+  if (!df || !d_D || !ms || (flags & ~(LAI_DF_J | LAI_DF_K)) || ((flags & LAI_DF_J) && !d_J) ||
+      ((flags & LAI_DF_K) && !d_K))
+    return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_jk_profile: invalid arguments");
+  return df_run(df, d_D, d_J, d_K, flags, hipStream_t(stream), ms);
+}

@@ -15,6 +15,7 @@ void check(int status, const char* what) {
         case LAI_ERR_INVALID_STATE: throw InvalidStateException(msg);
         case LAI_ERR_MEMORY: throw MemoryException(msg);
         case LAI_ERR_UNSUPPORTED: throw std::invalid_argument(msg);
+        case LAI_ERR_NUMERIC: throw std::runtime_error(msg);
         default: throw BackendError(msg);
     }
 }

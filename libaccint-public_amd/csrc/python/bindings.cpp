@@ -2,6 +2,7 @@
 // Names and argument meaning follow the reference bindings
 // (python/src/{basis,core_types,operator,screening,buffer,consumer,engine}_bindings.cpp).
 #include <libaccint/data.hpp>
+#include <libaccint/df.hpp>
 #include <libaccint/engine.hpp>
 
 #include <pybind11/numpy.h>
@@ -12,6 +13,9 @@
 
 namespace py = pybind11;
 using namespace libaccint;
+using consumers::DFFockBuilder;
+using consumers::DFFockBuilderConfig;
+using consumers::DFMetricType;
 using consumers::FockBuilder;
 using consumers::FockThreadingStrategy;
 using consumers::GpuFockBuilder;
@@ -718,4 +722,146 @@ PYBIND11_MODULE(_core, m) {
         .def("__repr__", [](const BatchEngine& e) {
             return "BatchEngine(n_molecules=" + std::to_string(e.n_molecules()) + ")";
         });
+
+    // ---- density fitting ---------------------------------------------------------------
+    py::enum_<FittingType>(m, "FittingType")
+        .value("RI", FittingType::RI).value("JKFIT", FittingType::JKFIT)
+        .value("JFIT", FittingType::JFIT).value("RIFIT", FittingType::RIFIT);
+    m.def("fitting_type_to_string", [](FittingType t) { return std::string(fitting_type_to_string(t)); });
+
+    py::class_<AuxiliaryBasisSet, std::shared_ptr<AuxiliaryBasisSet>>(m, "AuxiliaryBasisSet")
+        .def(py::init([](std::vector<Shell> shells, FittingType t, const std::string& name) {
+                 return std::make_shared<AuxiliaryBasisSet>(std::move(shells), t, name);
+             }),
+             py::arg("shells"), py::arg("fitting_type") = FittingType::JKFIT, py::arg("name") = "")
+        .def("n_shells", &AuxiliaryBasisSet::n_shells)
+        .def("n_functions", &AuxiliaryBasisSet::n_functions)
+        .def("n_primitives", &AuxiliaryBasisSet::n_primitives)
+        .def("max_angular_momentum", &AuxiliaryBasisSet::max_angular_momentum)
+        .def("shell", &AuxiliaryBasisSet::shell, py::return_value_policy::reference_internal)
+        .def("shells", [](const AuxiliaryBasisSet& b) { return std::vector<Shell>(b.shells().begin(), b.shells().end()); })
+        .def("shell_to_function", &AuxiliaryBasisSet::shell_to_function)
+        .def("function_to_shell", &AuxiliaryBasisSet::function_to_shell)
+        .def("fitting_type", &AuxiliaryBasisSet::fitting_type)
+        .def("name", &AuxiliaryBasisSet::name)
+        .def("__repr__", [](const AuxiliaryBasisSet& b) {
+            return "AuxiliaryBasisSet(name='" + b.name() + "', n_shells=" + std::to_string(b.n_shells()) +
+                   ", n_functions=" + std::to_string(b.n_functions()) + ")";
+        });
+
+    auto atom_lists = [](const std::vector<int>& Z, const std::vector<std::vector<Real>>& xyz) {
+        std::vector<std::array<Real, 3>> c;
+        for (const auto& v : xyz) {
+            if (v.size() != 3) throw std::invalid_argument("centers must have 3 components each");
+            c.push_back({v[0], v[1], v[2]});
+        }
+        return c;
+    };
+    m.def("load_auxiliary_basis",
+          [atom_lists](const std::string& name, const std::vector<int>& Z, const std::vector<std::vector<Real>>& xyz) {
+              const auto c = atom_lists(Z, xyz);
+              return std::make_shared<AuxiliaryBasisSet>(load_auxiliary_basis(name, Z, c));
+          },
+          py::arg("basis_name"), py::arg("atomic_numbers"), py::arg("centers"));
+    m.def("available_auxiliary_bases", &available_auxiliary_bases);
+    m.def("is_auxiliary_basis_available",
+          [](const std::string& name, const std::vector<int>& Z) { return is_auxiliary_basis_available(name, Z); },
+          py::arg("basis_name"), py::arg("atomic_numbers"));
+
+    py::enum_<DFMetricType>(m, "DFMetricType")
+        .value("Coulomb", DFMetricType::Coulomb)
+        .value("Overlap", DFMetricType::Overlap)
+        .value("AttenuatedCoulomb", DFMetricType::AttenuatedCoulomb);
+
+    py::class_<DFFockBuilderConfig>(m, "DFFockBuilderConfig")
+        .def(py::init<>())
+        .def_readwrite("exchange_fraction", &DFFockBuilderConfig::exchange_fraction)
+        .def_readwrite("compute_coulomb", &DFFockBuilderConfig::compute_coulomb)
+        .def_readwrite("compute_exchange", &DFFockBuilderConfig::compute_exchange)
+        .def_readwrite("use_symmetry", &DFFockBuilderConfig::use_symmetry)
+        .def_readwrite("incremental", &DFFockBuilderConfig::incremental)
+        .def_readwrite("memory_limit_mb", &DFFockBuilderConfig::memory_limit_mb)
+        .def_readwrite("n_threads", &DFFockBuilderConfig::n_threads)
+        .def_readwrite("conditioning_threshold", &DFFockBuilderConfig::conditioning_threshold)
+        .def_readwrite("conditioning_hard_limit", &DFFockBuilderConfig::conditioning_hard_limit)
+        .def_readwrite("metric_type", &DFFockBuilderConfig::metric_type)
+        .def_readwrite("range_separation_omega", &DFFockBuilderConfig::range_separation_omega);
+
+    py::class_<DFFockBuilder>(m, "DFFockBuilder")
+        .def(py::init([](std::shared_ptr<BasisSet> orb, std::shared_ptr<AuxiliaryBasisSet> aux, DFFockBuilderConfig c) {
+                 return std::make_unique<DFFockBuilder>(*orb, *aux, c);
+             }),
+             py::arg("orbital"), py::arg("auxiliary"), py::arg("config") = DFFockBuilderConfig{}, py::keep_alive<1, 2>(),
+             py::keep_alive<1, 3>())
+        .def("set_density",
+             [](DFFockBuilder& f, py::array D) {
+                 auto a = square(D, f.n_orb(), "DFFockBuilder.set_density");
+                 f.set_density(std::span<const Real>(a.data(), a.size()));
+             },
+             py::arg("D"))
+        .def("set_density_unrestricted",
+             [](DFFockBuilder& f, py::array Da, py::array Db) {
+                 auto a = square(Da, f.n_orb(), "DFFockBuilder.set_density_unrestricted");
+                 auto b = square(Db, f.n_orb(), "DFFockBuilder.set_density_unrestricted");
+                 f.set_density_unrestricted(std::span<const Real>(a.data(), a.size()),
+                                            std::span<const Real>(b.data(), b.size()));
+             },
+             py::arg("D_alpha"), py::arg("D_beta"))
+        .def("initialize", &DFFockBuilder::initialize)
+        .def("is_initialized", &DFFockBuilder::is_initialized)
+        .def("compute", [](DFFockBuilder& f) { return to_matrix(f.compute(), f.n_orb()); })
+        .def("compute_coulomb", [](DFFockBuilder& f) { return to_matrix(f.compute_coulomb(), f.n_orb()); })
+        .def("compute_exchange", [](DFFockBuilder& f) { return to_matrix(f.compute_exchange(), f.n_orb()); })
+        .def("compute_accumulate",
+             [](DFFockBuilder& f, py::array_t<Real, py::array::c_style> F) {
+                 if (F.size() != py::ssize_t(f.n_orb() * f.n_orb()))
+                     throw InvalidArgumentException("Fock matrix size mismatch");
+                 f.compute_accumulate(std::span<Real>(F.mutable_data(), F.size()));
+             },
+             py::arg("F"))
+        .def("coulomb_matrix", [](const DFFockBuilder& f) {
+            return to_matrix(std::vector<Real>(f.coulomb_matrix().begin(), f.coulomb_matrix().end()), f.n_orb());
+        })
+        .def("exchange_matrix", [](const DFFockBuilder& f) {
+            return to_matrix(std::vector<Real>(f.exchange_matrix().begin(), f.exchange_matrix().end()), f.n_orb());
+        })
+        .def("exchange_matrix_alpha", [](const DFFockBuilder& f) {
+            return to_matrix(std::vector<Real>(f.exchange_matrix_alpha().begin(), f.exchange_matrix_alpha().end()),
+                             f.is_uhf() ? f.n_orb() : 0);
+        })
+        .def("exchange_matrix_beta", [](const DFFockBuilder& f) {
+            return to_matrix(std::vector<Real>(f.exchange_matrix_beta().begin(), f.exchange_matrix_beta().end()),
+                             f.is_uhf() ? f.n_orb() : 0);
+        })
+        .def("is_uhf", &DFFockBuilder::is_uhf)
+        .def("fock_matrix",
+             [](const DFFockBuilder& f, py::object H) {
+                 if (H.is_none()) return to_matrix(f.fock_matrix(), f.n_orb());
+                 auto h = py::array_t<Real, py::array::c_style | py::array::forcecast>::ensure(H);
+                 if (h.size() != py::ssize_t(f.n_orb() * f.n_orb()))
+                     throw InvalidArgumentException("H_core size mismatch");
+                 return to_matrix(f.fock_matrix(std::span<const Real>(h.data(), h.size())), f.n_orb());
+             },
+             py::arg("H_core") = py::none())
+        .def("n_orb", &DFFockBuilder::n_orb)
+        .def("n_aux", &DFFockBuilder::n_aux)
+        .def("b_tensor_memory", &DFFockBuilder::b_tensor_memory)
+        .def("config", &DFFockBuilder::config)
+        .def("metric_condition_estimate", &DFFockBuilder::metric_condition_estimate)
+        .def("metric_ill_conditioned", &DFFockBuilder::metric_ill_conditioned)
+        .def("device_handle", [](const DFFockBuilder& f) { return reinterpret_cast<std::uintptr_t>(f.device_handle()); });
+
+    m.def("make_df_fock_builder",
+          [](std::shared_ptr<BasisSet> orb, const std::string& name) {
+              return consumers::make_df_fock_builder(*orb, name);
+          },
+          py::arg("orbital"), py::arg("aux_basis_name") = "", py::keep_alive<0, 1>());
+    m.def("make_df_fock_builder",
+          [atom_lists](std::shared_ptr<BasisSet> orb, const std::vector<int>& Z, const std::vector<std::vector<Real>>& xyz,
+                       const std::string& name, DFFockBuilderConfig c) {
+              const auto cc = atom_lists(Z, xyz);
+              return consumers::make_df_fock_builder(*orb, Z, cc, name, c);
+          },
+          py::arg("orbital"), py::arg("atomic_numbers"), py::arg("centers"), py::arg("aux_basis_name") = "",
+          py::arg("config") = DFFockBuilderConfig{}, py::keep_alive<0, 1>());
 }

@@ -67,6 +67,12 @@ _SIGNATURES = {
     "lai_dev_set_shell_atoms": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
     "lai_dev_jk_grad": ([_vp, _vp, ctypes.c_double, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_onee": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp, _vp], ctypes.c_int),
+    "lai_dev_df_create": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp,
+                           ctypes.POINTER(_vp)], ctypes.c_int),
+    "lai_dev_df_free": ([_vp], ctypes.c_int),
+    "lai_dev_df_dims": ([_vp, _ip, _ip], ctypes.c_int),
+    "lai_dev_df_jk": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "lai_dev_df_jk_profile": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _dp], ctypes.c_int),
     "lai_dev_rys": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_boys": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
 }

@@ -1,7 +1,8 @@
 """Write the basis-set data files this package ships (H, C, N, O only).
 
 The data are the Basis Set Exchange JSON records (schema "complete") for
-cc-pVDZ, def2-SVP, cc-pVTZ and aug-cc-pVTZ.  This script copies the
+cc-pVDZ, def2-SVP, cc-pVTZ and aug-cc-pVTZ and the fitting sets cc-pVDZ-RIFIT and
+def2-SVP-RIFIT.  This script copies the
 ``electron_shells`` entries of the four elements verbatim (exponents and
 coefficients stay strings, exactly as published) from a directory of BSE JSON
 files, so that the loader sees the same text the Basis Set Exchange serves.
@@ -16,7 +17,7 @@ import os
 import sys
 
 ELEMENTS = ("1", "6", "7", "8")
-NAMES = ("cc-pvdz", "def2-svp", "cc-pvtz", "aug-cc-pvtz")
+NAMES = ("cc-pvdz", "def2-svp", "cc-pvtz", "aug-cc-pvtz", "cc-pvdz-rifit", "def2-svp-rifit")
 
 
 def main(src_dir):

@@ -110,6 +110,23 @@ class RawBasis:
         self.charges = np.array([float(z) for z, _ in atoms])
         self.charge_xyz = np.array([list(map(float, xyz)) for _, xyz in atoms]).reshape(-1, 3)
 
+    @classmethod
+    def from_shells(cls, shells):
+        """A basis of explicit shells [(L, (x, y, z), exponents, raw
+        coefficients), ...] (no atoms: charges empty)."""
+        b = cls.__new__(cls)
+        b.L = np.array([sh[0] for sh in shells], dtype=np.int32)
+        b.centers = np.array([list(map(float, sh[1])) for sh in shells], dtype=np.float64).reshape(-1, 3)
+        b.nprim = np.array([len(sh[2]) for sh in shells], dtype=np.int32)
+        b.exps = np.array([float(x) for sh in shells for x in sh[2]], dtype=np.float64)
+        b.raw_coefs = np.array([float(x) for sh in shells for x in sh[3]], dtype=np.float64)
+        b.atom = np.zeros(len(shells), dtype=np.int32)
+        b.n_shells = len(shells)
+        b.nbf = int(sum((l + 1) * (l + 2) // 2 for l in b.L))
+        b.charges = np.zeros(0)
+        b.charge_xyz = np.zeros((0, 3))
+        return b
+
 
 # --------------------------------------------------------------------------
 # shared-library wrappers
@@ -155,6 +172,10 @@ def ref_lib():
             _ref.ref_basis_nbf.argtypes = [ctypes.c_void_p]
             _ref.ref_boys.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
             _ref.ref_rys.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+            _ref.ref_df_jk.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                       _dp, _dp]
+            _ref.ref_df_metric.argtypes = [ctypes.c_void_p, _dp]
+            _ref.ref_df_three_center.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp]
     return _ref
 
 
@@ -300,3 +321,138 @@ class Oracle:
         K = np.zeros((n, n))
         self._f("jk_full")(self.h, _d(D), _d(J), _d(K), int(n_threads), int(i_begin), int(i_end))
         return J, K
+
+
+# --------------------------------------------------------------------------
+# density fitting
+# --------------------------------------------------------------------------
+
+def df_tensors(orb: RawBasis, aux: RawBasis, metric="coulomb"):
+    """Metric V[P][Q] and three-centre integrals T[a][b][P] from the C
+    restatement.  The orbital and fitting shells (each normalised as in
+    src/shared/basis/shell.cpp:32-113) and a unit s function u (exponent 0,
+    coefficient 1) form one pre-normalised basis, so that (ab|P u) is the
+    three-centre integral of src/host/kernels/three_center_eri_kernel.cpp and
+    (P u|Q u) the two-centre Coulomb metric of two_center_coulomb_kernel.cpp;
+    the overlap metric (df_fock_builder.cpp:141-163) is the fitting basis's
+    overlap matrix."""
+    L_ = lib()
+    L_.oracle_basis_create_normalized.restype = ctypes.c_void_p
+    co = Oracle(orb).normalized()[0]
+    oa = Oracle(aux)
+    ca = oa.normalized()[0]
+    Ls = np.concatenate([orb.L, aux.L, [0]]).astype(np.int32)
+    ctr = np.concatenate([orb.centers.ravel(), aux.centers.ravel(), [0.0, 0.0, 0.0]])
+    npr = np.concatenate([orb.nprim, aux.nprim, [1]]).astype(np.int32)
+    ex = np.concatenate([orb.exps, aux.exps, [0.0]])
+    cf = np.concatenate([co, ca, [1.0]])
+    keep = [np.ascontiguousarray(a) for a in (Ls, ctr, npr, ex, cf)]
+    h = ctypes.c_void_p(L_.oracle_basis_create_normalized(len(Ls), _i(keep[0]), _d(keep[1]), _i(keep[2]),
+                                                          _d(keep[3]), _d(keep[4])))
+    try:
+        nsh = orb.n_shells
+        u = len(Ls) - 1
+        nf = [(l + 1) * (l + 2) // 2 for l in Ls]
+        fo = np.concatenate([[0], np.cumsum(nf)])
+        n, naux = orb.nbf, aux.nbf
+
+        def quartet(i, j, k, l):
+            out = np.zeros(nf[i] * nf[j] * nf[k] * nf[l])
+            L_.oracle_eri_shell_quartet(h, int(i), int(j), int(k), int(l), _d(out))
+            return out.reshape(nf[i], nf[j], nf[k], nf[l])
+
+        T = np.zeros((n, n, naux))
+        for i in range(nsh):
+            for j in range(i, nsh):
+                for P in range(aux.n_shells):
+                    b = quartet(i, j, nsh + P, u)[:, :, :, 0]
+                    p0 = fo[nsh + P] - n
+                    T[fo[i]:fo[i + 1], fo[j]:fo[j + 1], p0:p0 + nf[nsh + P]] = b
+                    T[fo[j]:fo[j + 1], fo[i]:fo[i + 1], p0:p0 + nf[nsh + P]] = b.transpose(1, 0, 2)
+        if metric == "coulomb":
+            V = np.zeros((naux, naux))
+            for P in range(aux.n_shells):
+                for Q in range(P, aux.n_shells):
+                    b = quartet(nsh + P, u, nsh + Q, u)[:, 0, :, 0]
+                    p0, q0 = fo[nsh + P] - n, fo[nsh + Q] - n
+                    V[p0:p0 + b.shape[0], q0:q0 + b.shape[1]] = b
+                    V[q0:q0 + b.shape[1], p0:p0 + b.shape[0]] = b.T
+        elif metric == "overlap":
+            V = oa.overlap()
+        else:
+            raise ValueError("metric must be 'coulomb' or 'overlap'")
+    finally:
+        L_.oracle_basis_free(h)
+    return V, T
+
+
+def df_jk_from_tensors(V, T, D, D_beta=None, b_side="left"):
+    """J and K from a metric V[P][Q] and three-centre integrals T[a][b][P]
+    with the contractions of src/shared/consumers/df_fock_builder.cpp:
+    Cholesky factor L of V and its inverse (include/libaccint/math/cholesky.hpp),
+    gamma = B^T D and J = B gamma (:283-326), K = sum_P (B_P D) B_P^T
+    (:328-378); UHF (D_beta given): J from D + D_beta, K = K_alpha + K_beta,
+    returned as (J, K, K_alpha, K_beta).
+    b_side "left": B_P = sum_Q [L^-1]_PQ T_Q, so that B^T B = T V^-1 T^T, the
+    fit the header df_fock_builder.hpp:9-13 states; "right": B_P = sum_Q T_Q
+    [L^-1]_QP, what kernels::compute_B_tensor (three_center_eri_kernel.cpp:364-387)
+    computes (used only to check these contractions against the compiled
+    reference)."""
+    import scipy.linalg
+    Lc = np.linalg.cholesky(V)
+    Linv = scipy.linalg.solve_triangular(Lc, np.eye(len(V)), lower=True)
+    if b_side == "left":
+        B = np.einsum("PQ,abQ->Pab", Linv, T)
+    else:
+        B = np.einsum("abQ,QP->Pab", T, Linv)
+
+    def k_of(Dm):
+        return np.einsum("Pab,Pcb->ac", B @ Dm, B)
+
+    Da = np.asarray(D, dtype=np.float64)
+    Dt = Da if D_beta is None else Da + np.asarray(D_beta, dtype=np.float64)
+    J = np.einsum("Pab,P->ab", B, np.einsum("Pab,ab->P", B, Dt))
+    if D_beta is None:
+        return J, k_of(Da)
+    Ka, Kb = k_of(Da), k_of(np.asarray(D_beta, dtype=np.float64))
+    return J, Ka + Kb, Ka, Kb
+
+
+def df_jk(orb: RawBasis, aux: RawBasis, D, metric="coulomb", D_beta=None):
+    """Density-fitted J and K of the C restatement's integrals (df_tensors)
+    through df_jk_from_tensors with B = L^-1 T."""
+    V, T = df_tensors(orb, aux, metric)
+    return df_jk_from_tensors(V, T, D, D_beta)
+
+
+def ref_df_tensors(orb: RawBasis, aux: RawBasis):
+    """The compiled reference's metric (kernels::compute_two_center_metric)
+    and three-centre tensor (kernels::compute_three_center_tensor, abP)."""
+    r = ref_lib()
+    if r is None:
+        raise RuntimeError("oracle/_ref/libref.so is not built")
+    o, a = Oracle(orb, use_ref=True), Oracle(aux, use_ref=True)
+    V = np.zeros((aux.nbf, aux.nbf))
+    T = np.zeros((orb.nbf, orb.nbf, aux.nbf))
+    r.ref_df_metric(a.h, _d(V))
+    r.ref_df_three_center(o.h, a.h, _d(T))
+    return V, T
+
+
+def ref_df_jk(orb: RawBasis, aux: RawBasis, D, metric="coulomb", D_beta=None):
+    """The reference's own DFFockBuilder (compiled into oracle/_ref) on the
+    same shells: (J, K), or (J, K, K_alpha, K_beta) with D_beta."""
+    r = ref_lib()
+    if r is None:
+        raise RuntimeError("oracle/_ref/libref.so is not built")
+    o, a = Oracle(orb, use_ref=True), Oracle(aux, use_ref=True)
+    n = orb.nbf
+    Da = np.ascontiguousarray(D, dtype=np.float64)
+    Db = None if D_beta is None else np.ascontiguousarray(D_beta, dtype=np.float64)
+    J, K, Ka, Kb = (np.zeros((n, n)) for _ in range(4))
+    rc = r.ref_df_jk(o.h, a.h, 0 if metric == "coulomb" else 1, _d(Da), _d(Db) if Db is not None else None,
+                     _d(J), _d(K), _d(Ka), _d(Kb))
+    if rc != 0:
+        r.ref_df_last_error.restype = ctypes.c_char_p
+        raise RuntimeError("reference DFFockBuilder: " + r.ref_df_last_error().decode())
+    return (J, K) if Db is None else (J, K, Ka, Kb)

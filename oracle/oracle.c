@@ -447,6 +447,17 @@ oracle_basis *oracle_basis_create(int n_shells, const int *L, const double *cent
     return b;
 }
 
+oracle_basis *oracle_basis_create_normalized(int n_shells, const int *L, const double *centers,
+                                             const int *nprim, const double *exps,
+                                             const double *coefs)
+{
+    oracle_basis *b = oracle_basis_create(n_shells, L, centers, nprim, exps, coefs);
+    int np = 0;
+    for (int s = 0; s < n_shells; ++s) np += nprim[s];
+    memcpy(b->coefs, coefs, sizeof(double) * np);
+    return b;
+}
+
 void oracle_basis_free(oracle_basis *b)
 {
     if (!b) return;

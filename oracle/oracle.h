@@ -42,6 +42,14 @@ void oracle_rys(int n_roots, double T, double *roots, double *weights);
 oracle_basis *oracle_basis_create(int n_shells, const int *L, const double *centers,
                                   const int *nprim, const double *exps,
                                   const double *raw_coefs);
+/* As oracle_basis_create with the coefficients taken as given (already
+ * normalised), so that a zero-exponent unit s function can stand in as the
+ * fourth shell of a three- or two-centre integral: (ab|P u) = (ab|P), the
+ * integral of src/host/kernels/three_center_eri_kernel.cpp and
+ * two_center_coulomb_kernel.cpp (density fitting). */
+oracle_basis *oracle_basis_create_normalized(int n_shells, const int *L, const double *centers,
+                                             const int *nprim, const double *exps,
+                                             const double *coefs);
 void oracle_basis_free(oracle_basis *b);
 int oracle_basis_nbf(const oracle_basis *b);
 /* copy out normalised coefficients and function offsets */

@@ -8,12 +8,16 @@
 // (include/libaccint/engine/engine.hpp:1815-1861 canonical loop,
 // include/libaccint/engine/cpu_engine.hpp:553-622 parallel N^4 loop); every
 // integral and every J/K update is the reference's own code.
+#include <libaccint/basis/auxiliary_basis_set.hpp>
 #include <libaccint/basis/basis_set.hpp>
 #include <libaccint/basis/shell.hpp>
 #include <libaccint/buffers/one_electron_buffer.hpp>
 #include <libaccint/buffers/two_electron_buffer.hpp>
+#include <libaccint/consumers/df_fock_builder.hpp>
 #include <libaccint/consumers/fock_builder.hpp>
 #include <libaccint/kernels/eri_kernel.hpp>
+#include <libaccint/kernels/three_center_eri_kernel.hpp>
+#include <libaccint/kernels/two_center_coulomb_kernel.hpp>
 #include <libaccint/kernels/kinetic_kernel.hpp>
 #include <libaccint/kernels/nuclear_kernel.hpp>
 #include <libaccint/kernels/overlap_kernel.hpp>
@@ -24,6 +28,7 @@
 #include <omp.h>
 
 #include <cstring>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -33,6 +38,7 @@ namespace {
 struct RefBasis {
     std::unique_ptr<BasisSet> basis;
 };
+std::string g_df_error;
 }  // namespace
 
 extern "C" {
@@ -274,6 +280,57 @@ void ref_jk_full(void* h, const double* D, double* J, double* K, int n_threads, 
     auto Km = fb.get_exchange_matrix();
     std::memcpy(J, Jm.data(), sizeof(double) * nbf * nbf);
     std::memcpy(K, Km.data(), sizeof(double) * nbf * nbf);
+}
+
+// Density-fitted J and K of the reference DFFockBuilder
+// (src/shared/consumers/df_fock_builder.cpp) for the orbital basis orb and the
+// fitting basis aux (a basis handle of this file; its shells become the
+// AuxiliaryBasisSet).  metric 0 = Coulomb, 1 = overlap.  Returns 0, or -1
+// when the builder throws.  With Db != NULL: the UHF path
+// (set_density_unrestricted), K = Ka + Kb and Ka, Kb returned as well.
+int ref_df_jk(void* orb, void* aux, int metric, const double* Da, const double* Db, double* J, double* K,
+              double* Ka, double* Kb) {
+    const auto& ob = *static_cast<RefBasis*>(orb)->basis;
+    const auto& ab = *static_cast<RefBasis*>(aux)->basis;
+    try {
+        AuxiliaryBasisSet as(ab.shells(), FittingType::JKFIT, "aux");
+        consumers::DFFockBuilderConfig cfg;
+        cfg.metric_type = metric == 0 ? consumers::DFMetricType::Coulomb : consumers::DFMetricType::Overlap;
+        consumers::DFFockBuilder f(ob, as, cfg);
+        const Size n = ob.n_basis_functions();
+        if (Db)
+            f.set_density_unrestricted(std::span<const Real>(Da, n * n), std::span<const Real>(Db, n * n));
+        else
+            f.set_density(std::span<const Real>(Da, n * n));
+        const auto j = f.compute_coulomb();
+        const auto k = f.compute_exchange();
+        std::memcpy(J, j.data(), sizeof(double) * n * n);
+        std::memcpy(K, k.data(), sizeof(double) * n * n);
+        if (Db) {
+            std::memcpy(Ka, f.exchange_matrix_alpha().data(), sizeof(double) * n * n);
+            std::memcpy(Kb, f.exchange_matrix_beta().data(), sizeof(double) * n * n);
+        }
+    } catch (const std::exception& e) {
+        g_df_error = e.what();
+        return -1;
+    }
+    return 0;
+}
+
+const char* ref_df_last_error() { return g_df_error.c_str(); }
+
+// The reference's two-centre metric (P|Q) of the shells of aux (n_aux^2,
+// row-major) and three-centre tensor (ab|P) in the abP layout.
+void ref_df_metric(void* aux, double* V) {
+    const auto& ab = *static_cast<RefBasis*>(aux)->basis;
+    kernels::compute_two_center_metric(ab.shells(), V, ab.n_basis_functions());
+}
+
+void ref_df_three_center(void* orb, void* aux, double* T) {
+    const auto& ob = *static_cast<RefBasis*>(orb)->basis;
+    const auto& ab = *static_cast<RefBasis*>(aux)->basis;
+    kernels::compute_three_center_tensor(ob.shells(), ab.shells(), T, ob.n_basis_functions(), ab.n_basis_functions(),
+                                         kernels::ThreeCenterStorageFormat::abP);
 }
 
 }  // extern "C"
