@@ -41,10 +41,13 @@ constexpr int ncart_of(int L) { return (L + 1) * (L + 2) / 2; }
 // ---------------------------------------------------------------------------
 // FP64 GEMM on the matrix cores: C = A B with A row-major M x K and B either
 // row-major K x N (bt = 0) or given as its transpose, row-major N x K
-// (bt = 1).  A workgroup of four wavefronts computes a 64 x 64 tile of C;
-// each wavefront a 32 x 32 quadrant as 2 x 2 blocks of
-// v_mfma_f64_16x16x4f64.  The K loop stages 64 x 16 slices of A and B through
-// LDS, the next slice's global loads in flight while the current one runs.
+// (bt = 1).  A workgroup of four wavefronts computes a (64 BM) x (16 BN) tile
+// of C, the wavefronts stacked along M, each a (16 BM) x (16 BN) strip as
+// BM x BN blocks of v_mfma_f64_16x16x4f64.  The summation runs in steps of
+// 16 through two LDS buffers: the next step's global loads are in flight
+// (in registers) while the current step multiplies, then land in the other
+// buffer; one workgroup barrier per step.  BN = 5 makes the 80-column tiles
+// that divide the 400-function orbital dimension of the exchange products.
 // blockIdx.z is a batch index (split = 0: A, B, C offset by sA, sB, sC) or a
 // slice of the summation index (split = 1: columns [z kchunk, (z+1) kchunk)
 // of A and rows of B, written to C + z sC).
@@ -53,7 +56,6 @@ constexpr int ncart_of(int L) { return (L + 1) * (L + 2) / 2; }
 // B[l / 16][l % 16] and holds D[l / 16 + 4 r][l % 16] in accumulator element r.
 // ---------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int GT = 64;  // tile edge of C
 constexpr int KT = 16;  // summation step
 
 struct GemmArgs {
@@ -66,11 +68,15 @@ struct GemmArgs {
   int bt, split, kchunk;
 };
 
+template <int BM, int BN>
 __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs g) {
-  __shared__ double As[GT][KT + 1];
-  __shared__ double Bs[KT][GT + 1];
+  constexpr int TM = 64 * BM, TN = 16 * BN;
+  constexpr int NA = TM * KT / 256;              // A entries per thread per step
+  constexpr int NB = (KT * TN + 255) / 256;      // B entries per thread per step
+  __shared__ double As[2][TM][KT + 1];
+  __shared__ double Bs[2][KT][TN + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT, z = blockIdx.z;
+  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN, z = blockIdx.z;
   const double* A = g.A;
   const double* B = g.B;
   double* C = g.C + z * g.sC;
@@ -82,68 +88,86 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs g) {
     A += z * g.sA;
     B += z * g.sB;
   }
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  v4d acc[2][2];
+  const int wm = wave * 16 * BM;
+  v4d acc[BM][BN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-  double ra[4], rb[4];
+    for (int j = 0; j < BN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  double ra[NA], rb[NB];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NA; ++r) {  // 16 lanes per row of A
       const int e = tid + 256 * r;
-      {  // A: 64 rows x 16 columns, 16 lanes per row
-        const int row = e >> 4, col = e & 15;
-        const int gm = m0 + row, gk = k0 + col;
-        ra[r] = (gm < g.M && gk < ke) ? A[(long long)gm * g.lda + gk] : 0.0;
+      const int gm = m0 + e / KT, gk = k0 + e % KT;
+      ra[r] = (gm < g.M && gk < ke) ? A[(long long)gm * g.lda + gk] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int e = tid + 256 * r;
+      double x = 0.0;
+      if (e < KT * TN) {
+        if (g.bt) {  // rows of B^T: contiguous along the summation index
+          const int gn = n0 + e / KT, gk = k0 + e % KT;
+          if (gn < g.N && gk < ke) x = B[(long long)gn * g.ldb + gk];
+        } else {  // rows of B: contiguous along N
+          const int gk = k0 + e / TN, gn = n0 + e % TN;
+          if (gn < g.N && gk < ke) x = B[(long long)gk * g.ldb + gn];
+        }
       }
-      if (g.bt) {  // B^T rows: 64 columns of B x 16 summation entries
-        const int nn = e >> 4, kk = e & 15;
-        const int gn = n0 + nn, gk = k0 + kk;
-        rb[r] = (gn < g.N && gk < ke) ? B[(long long)gn * g.ldb + gk] : 0.0;
-      } else {  // B rows: 16 summation entries x 64 columns
-        const int kk = e >> 6, nn = e & 63;
-        const int gn = n0 + nn, gk = k0 + kk;
-        rb[r] = (gn < g.N && gk < ke) ? B[(long long)gk * g.ldb + gn] : 0.0;
+      rb[r] = x;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      const int e = tid + 256 * r;
+      As[buf][e / KT][e % KT] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int e = tid + 256 * r;
+      if (e < KT * TN) {
+        if (g.bt)
+          Bs[buf][e % KT][e / KT] = rb[r];
+        else
+          Bs[buf][e / TN][e % TN] = rb[r];
       }
     }
   };
-  if (kb < ke) load(kb);
+  if (kb < ke) {
+    load(kb);
+    store(0);
+  }
+  __syncthreads();
+  int cur = 0;
   for (int k0 = kb; k0 < ke; k0 += KT) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int e = tid + 256 * r;
-      As[e >> 4][e & 15] = ra[r];
-      if (g.bt)
-        Bs[e & 15][e >> 4] = rb[r];
-      else
-        Bs[e >> 6][e & 63] = rb[r];
-    }
-    __syncthreads();
-    if (k0 + KT < ke) load(k0 + KT);
+    const bool more = k0 + KT < ke;
+    if (more) load(k0 + KT);
 #pragma unroll
     for (int kk = 0; kk < KT; kk += 4) {
-      double a[2], b[2];
+      double a[BM], b[BN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[wm + 16 * i + (lane & 15)][kk + (lane >> 4)];
+      for (int i = 0; i < BM; ++i) a[i] = As[cur][wm + 16 * i + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wn + 16 * j + (lane & 15)];
+      for (int j = 0; j < BN; ++j) b[j] = Bs[cur][kk + (lane >> 4)][16 * j + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < BM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < BN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (more) store(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
-        const int col = n0 + wn + 16 * j + (lane & 15);
+        const int col = n0 + 16 * j + (lane & 15);
         if (row < g.M && col < g.N) C[(long long)row * g.ldc + col] = acc[i][j][r];
       }
 }
@@ -167,19 +191,33 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// part[a][P] = sum_b B[a][P][b] D[a][b]; one workgroup per a, one wavefront
-// per P row at a time (lanes along b, coalesced).
+// part[a][P] = sum_b B[a][P][b] D[a][b]: workgroup (a, 32 rows of P), each
+// wavefront 8 rows at a time (eight loads in flight per lane, lanes along b),
+// then eight wavefront sums.
+constexpr int kGammaRows = 32;
 __global__ __launch_bounds__(256) void df_gamma_part(const double* __restrict__ B, const double* __restrict__ D,
                                                      int n, int naux, double* __restrict__ part) {
   const int a = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int P0 = blockIdx.y * kGammaRows + wave * 8;
   const double* Da = D + (long long)a * n;
-  const double* Ba = B + (long long)a * naux * n;
-  for (int P = wave; P < naux; P += 4) {
-    const double* row = Ba + (long long)P * n;
-    double s = 0.0;
-    for (int b = lane; b < n; b += 64) s = fma(row[b], Da[b], s);
-    s = wave_sum(s);
-    if (lane == 0) part[(long long)a * naux + P] = s;
+  const double* Ba = B + ((long long)a * naux + P0) * n;
+  double s[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) s[r] = 0.0;
+  for (int b = lane; b < n; b += 64) {
+    const double d = Da[b];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (P0 + r < naux) s[r] = fma(Ba[(long long)r * n + b], d, s[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) s[r] = wave_sum(s[r]);
+  if (lane < 8 && P0 + lane < naux) {
+    double v = s[0];
+#pragma unroll
+    for (int r = 1; r < 8; ++r)
+      if (lane == r) v = s[r];
+    part[(long long)a * naux + P0 + lane] = v;
   }
 }
 
@@ -193,15 +231,27 @@ __global__ __launch_bounds__(256) void df_gamma_sum(const double* __restrict__ p
   gamma[P] = s;
 }
 
-// J[a][b] = sum_P B[a][P][b] gamma[P]; workgroup (b block, a), lanes along b.
+// J[a][b] = sum_P B[a][P][b] gamma[P]: workgroup (64 columns b, a), the four
+// wavefronts take every fourth P (lanes along b, two accumulators each) and
+// meet in LDS.
 __global__ __launch_bounds__(256) void df_coulomb(const double* __restrict__ B, const double* __restrict__ gamma,
                                                   int n, int naux, double* __restrict__ J) {
-  const int a = blockIdx.y, b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= n) return;
-  const double* Ba = B + (long long)a * naux * n + b;
-  double s = 0.0;
-  for (int P = 0; P < naux; ++P) s = fma(Ba[(long long)P * n], gamma[P], s);
-  J[(long long)a * n + b] = s;
+  __shared__ double red[4][64];
+  const int a = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 64 + lane;
+  double s0 = 0.0, s1 = 0.0;
+  if (b < n) {
+    const double* Ba = B + (long long)a * naux * n + b;
+    int P = wave;
+    for (; P + 4 < naux; P += 8) {
+      s0 = fma(Ba[(long long)P * n], gamma[P], s0);
+      s1 = fma(Ba[(long long)(P + 4) * n], gamma[P + 4], s1);
+    }
+    if (P < naux) s0 = fma(Ba[(long long)P * n], gamma[P], s0);
+  }
+  red[wave][lane] = s0 + s1;
+  __syncthreads();
+  if (wave == 0 && b < n) J[(long long)a * n + b] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,12 +301,19 @@ struct Dev {
   }
 };
 
+// C tiles of 128 x 80 (BM = 2) for tall products, 64 x 80 (BM = 1) when M is
+// small (the exchange product, M = n_orb).
 int gemm(const GemmArgs& g, int gz, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0 || gz <= 0) return LAI_OK;
-  const dim3 grid((g.M + GT - 1) / GT, (g.N + GT - 1) / GT, gz);
+  const bool tall = g.M >= 1024;
+  const int TM = tall ? 128 : 64, TN = 80;
+  const dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, gz);
   if (grid.y > 65535u || grid.z > 65535u)
     return set_error(LAI_ERR_INVALID_ARGUMENT, "density fitting: GEMM grid beyond the launch limits");
-  hipLaunchKernelGGL(dgemm_mfma, grid, dim3(256), 0, st, g);
+  if (tall)
+    hipLaunchKernelGGL((dgemm_mfma<2, 5>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((dgemm_mfma<1, 5>), grid, dim3(256), 0, st, g);
   DF_HIP(hipGetLastError());
   return LAI_OK;
 }
@@ -529,8 +586,8 @@ extern "C" int lai_dev_df_create(lai_basis* b, int n_orb_shells, int n_aux_shell
   auto* d = new lai_df;
   d->n = n;
   d->naux = naux;
-  // slices of the exchange GEMM: about 8 workgroups per CU over the 64 x 64 tiles
-  const long long tiles = (long long)((n + GT - 1) / GT) * ((n + GT - 1) / GT);
+  // slices of the exchange GEMM: about 8 workgroups per CU over its tiles
+  const long long tiles = (long long)((n + 63) / 64) * ((n + 79) / 80);  // 64 x 80 tiles (gemm)
   const long long ksum = (long long)naux * n;
   int ns = int(std::max(1ll, std::min(2048ll / tiles, (ksum + 16 * KT - 1) / (16 * KT))));
   long long kc = (ksum + ns - 1) / ns;
@@ -570,12 +627,13 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
   if (flags & LAI_DF_J) {
     int rc = mark(0);
     if (rc) return rc;
-    hipLaunchKernelGGL(df_gamma_part, dim3(n), dim3(256), 0, st, d->B, D, n, naux, d->part);
+    hipLaunchKernelGGL(df_gamma_part, dim3(n, (naux + kGammaRows - 1) / kGammaRows), dim3(256), 0, st, d->B, D, n, naux,
+                       d->part);
     DF_HIP(hipGetLastError());
     hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 255) / 256), dim3(256), 0, st, d->part, n, naux, d->gamma);
     DF_HIP(hipGetLastError());
     if ((rc = mark(1))) return rc;
-    hipLaunchKernelGGL(df_coulomb, dim3((n + 255) / 256, n), dim3(256), 0, st, d->B, d->gamma, n, naux, J);
+    hipLaunchKernelGGL(df_coulomb, dim3((n + 63) / 64, n), dim3(256), 0, st, d->B, d->gamma, n, naux, J);
     DF_HIP(hipGetLastError());
     if ((rc = mark(2))) return rc;
     if (ms) {
