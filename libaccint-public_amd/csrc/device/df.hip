@@ -221,14 +221,24 @@ __global__ __launch_bounds__(256) void df_gamma_part(const double* __restrict__ 
   }
 }
 
-// gamma[P] = sum_a part[a][P]
-__global__ __launch_bounds__(256) void df_gamma_sum(const double* __restrict__ part, int n, int naux,
-                                                    double* __restrict__ gamma) {
-  const int P = blockIdx.x * 256 + threadIdx.x;
-  if (P >= naux) return;
+// gamma[P] = sum_a part[a][P]: workgroup of 64 P (lanes) and 16 wavefronts,
+// each summing every 16th a, then a fixed-order sum of the 16 partials.
+__global__ __launch_bounds__(1024) void df_gamma_sum(const double* __restrict__ part, int n, int naux,
+                                                     double* __restrict__ gamma) {
+  __shared__ double red[16][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int P = blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int a = 0; a < n; ++a) s += part[(long long)a * naux + P];
-  gamma[P] = s;
+  if (P < naux)
+    for (int a = wave; a < n; a += 16) s += part[(long long)a * naux + P];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && P < naux) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t += red[w][lane];
+    gamma[P] = t;
+  }
 }
 
 // J[a][b] = sum_P B[a][P][b] gamma[P]: workgroup (64 columns b, a), the four
@@ -630,7 +640,7 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     hipLaunchKernelGGL(df_gamma_part, dim3(n, (naux + kGammaRows - 1) / kGammaRows), dim3(256), 0, st, d->B, D, n, naux,
                        d->part);
     DF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 255) / 256), dim3(256), 0, st, d->part, n, naux, d->gamma);
+    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 63) / 64), dim3(1024), 0, st, d->part, n, naux, d->gamma);
     DF_HIP(hipGetLastError());
     if ((rc = mark(1))) return rc;
     hipLaunchKernelGGL(df_coulomb, dim3((n + 63) / 64, n), dim3(256), 0, st, d->B, d->gamma, n, naux, J);
