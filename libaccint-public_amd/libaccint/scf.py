@@ -8,9 +8,11 @@ eigenvalues of S at or below 1e-10 dropped; converged when the energy moves by
 less than 1e-12 and the density by less than 1e-10 (max abs).
 
 D, J, K, F, S and X stay in HBM as torch tensors on one stream; J and K come
-from Engine.build_jk_device (the screened, fused gfx950 J/K kernels), the
-eigenproblems from torch.linalg.eigh on the device.  Only the DIIS
-coefficients (a system of at most 7 equations) are solved on the host.
+from Engine.build_jk_device (the screened, fused gfx950 J/K kernels) or, with
+density_fitting set to a fitting-basis name, from the device DFFockBuilder
+(lai_dev_df_jk on the same stream), the eigenproblems from torch.linalg.eigh
+on the device.  Only the DIIS coefficients (a system of at most 7 equations)
+are solved on the host.
 """
 import math
 from dataclasses import dataclass
@@ -18,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _core
+from . import _cabi, _core
 from .convenience import atoms_from_list, basis_set
 
 
@@ -74,8 +76,10 @@ class _DIIS:
 
 
 def rhf(atoms, basis_name, n_electrons, max_iter=100, e_tol=1e-12, d_tol=1e-10, options=None,
-        device="cuda"):
-    """RHF energy of [(Z, (x, y, z)), ...] (bohr) in basis_name with n_electrons (even)."""
+        device="cuda", density_fitting=None):
+    """RHF energy of [(Z, (x, y, z)), ...] (bohr) in basis_name with n_electrons (even).
+    density_fitting: a fitting-basis name ("cc-pVDZ-RI", "def2-SVP-RI") for
+    density-fitted J and K, else the exact four-centre build."""
     if n_electrons % 2:
         raise ValueError("rhf: closed shells only (even electron count)")
     n_occ = n_electrons // 2
@@ -106,11 +110,22 @@ def rhf(atoms, basis_name, n_electrons, max_iter=100, e_tol=1e-12, d_tol=1e-10, 
     J = torch.zeros((n, n), dtype=torch.float64, device=dev)
     K = torch.zeros_like(J)
 
+    dfb = None
+    if density_fitting:
+        dfb = _core.make_df_fock_builder(basis, [int(z) for z, _ in atoms], [list(map(float, p)) for _, p in atoms],
+                                         density_fitting)
+        dfb.initialize()
+        dfh = dfb.device_handle()
+
     def fock(D):
-        J.zero_()
-        K.zero_()
         Dc = D.contiguous()
-        engine.build_jk_device(Dc.data_ptr(), J.data_ptr(), K.data_ptr(), opts, stream.cuda_stream, None)
+        if dfb is not None:  # J and K overwritten
+            _cabi.check(_cabi.lib().lai_dev_df_jk(dfh, Dc.data_ptr(), J.data_ptr(), K.data_ptr(), 3,
+                                                  stream.cuda_stream))
+        else:
+            J.zero_()
+            K.zero_()
+            engine.build_jk_device(Dc.data_ptr(), J.data_ptr(), K.data_ptr(), opts, stream.cuda_stream, None)
         return H + J - 0.5 * K
 
     eps, C = solve(H)
