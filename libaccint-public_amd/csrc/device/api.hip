@@ -190,6 +190,9 @@ struct ClassTune {
   bool is_default() const { return alt == 0 && chunk == 0 && bpc == 0; }
 };
 
+// class kernels of one J/K build run on the caller's stream and this many more
+constexpr int kSideStreams = 3;
+
 struct lai_basis {
   lai_ctx* ctx = nullptr;
   int ns = 0, nbf = 0;  // shells and basis functions of all molecules
@@ -249,6 +252,11 @@ struct lai_basis {
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // side streams of the concurrent class launches (created before the first
+  // J/K build) and the fork / join events that tie them to the caller's stream
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSideStreams] = {};
 };
 
 // pair index of molecule-local shells il <= jl of molecule m
@@ -559,6 +567,11 @@ extern "C" int lai_dev_basis_free(lai_basis* b) {
   if (!b) return LAI_OK;
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+    if (b->join[i]) (void)hipEventDestroy(b->join[i]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   b->pairs.release();
   b->pp.release();
   b->ppab.release();
@@ -1085,16 +1098,50 @@ extern "C" int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, doubl
 // ---------------------------------------------------------------------------
 // J/K build
 // ---------------------------------------------------------------------------
+// Side streams and fork/join events of a basis handle (before any capture).
+static int ensure_side_streams(lai_basis* b) {
+  if (b->fork) return LAI_OK;
+  for (int i = 0; i < kSideStreams; ++i) {
+    LAI_HIP(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+    LAI_HIP(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+  }
+  LAI_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  return LAI_OK;
+}
+
 // Class kernels over the bra-anchored chunks, heavy classes first.  With ev
-// (4 events per class) a hipEvent pair brackets each of the two parts.
-static void launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
+// (4 events per class) a hipEvent pair brackets each of the two parts, one
+// class after another on st.  Without: the classes are spread over st and
+// the side streams (each class to the stream with the least work so far, by
+// the classes' executed flops), forked from and joined back into st, so
+// that the last workgroups of one class overlap the next classes (every
+// class adds into Jt and Kt with atomics only, so the order is free).
+static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
   const auto& Ls = launchers();
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
   if (flags != LAI_JK_MIXED) a.sel.mixed = 0;
+  const bool fan = ev == nullptr && b->fork != nullptr;
+  hipStream_t streams[kSideStreams + 1] = {st};
+  double load[kSideStreams + 1] = {};
+  if (fan) {
+    LAI_HIP(hipEventRecord(b->fork, st));
+    for (int i = 0; i < kSideStreams; ++i) {
+      streams[i + 1] = b->side[i];
+      LAI_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
+    }
+  }
   for (int c = kNumClasses - 1; c >= 0; --c) {
     const ClassList& C = b->cls[c];
+    hipStream_t s = st;
+    if (fan && C.nchunks > 0) {
+      int k = 0;
+      for (int i = 1; i <= kSideStreams; ++i)
+        if (load[i] < load[k]) k = i;
+      load[k] += (flags == LAI_JK_MIXED ? b->cls_f64[c] + b->cls_f32[c] : b->cls_f64[c]) + 1e6 * double(C.nchunks);
+      s = streams[k];
+    }
     if (ev) (void)hipEventRecord(ev[2 * c], st);
     if (C.nchunks > 0) {
       a.chunks = C.chunks.p;
@@ -1103,10 +1150,16 @@ static void launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st
       t.alt = b->tune[c].alt;
       t.blocks_per_cu = b->tune[c].bpc;
       t.n_cu = b->ctx->n_cu;
-      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, st, t);
+      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, s, t);
     }
     if (ev) (void)hipEventRecord(ev[2 * c + 1], st);
   }
+  if (fan)
+    for (int i = 0; i < kSideStreams; ++i) {
+      LAI_HIP(hipEventRecord(b->join[i], b->side[i]));
+      LAI_HIP(hipStreamWaitEvent(st, b->join[i], 0));
+    }
+  return LAI_OK;
 }
 
 // arguments of a J/K build into the scratch Jt, Kt
@@ -1143,9 +1196,14 @@ extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* 
   hipStream_t st = hipStream_t(stream);
   const size_t nn = size_t(b->mat_size);
   int rc;
+  if (!b->fork) {  // side streams are created outside any stream capture
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LAI_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs == hipStreamCaptureStatusNone && (rc = ensure_side_streams(b)) != LAI_OK) return rc;
+  }
   if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
   if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
-  launch_jk_classes(b, jk_args(b, d_D), flags, st, nullptr);
+  if ((rc = launch_jk_classes(b, jk_args(b, d_D), flags, st, nullptr)) != LAI_OK) return rc;
   LAI_HIP(hipGetLastError());
   return symmetrize(b, d_J, d_K, st);
 }
@@ -1163,8 +1221,10 @@ extern "C" int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double*
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
+  int rc = ensure_side_streams(b);  // not inside the capture
+  if (rc != LAI_OK) return rc;
   LAI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  int rc = lai_dev_jk(b, d_D, d_J, d_K, flags, stream);
+  rc = lai_dev_jk(b, d_D, d_J, d_K, flags, stream);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (rc != LAI_OK) {
