@@ -1369,14 +1369,43 @@ extern "C" int lai_dev_jk_grad(lai_basis* b, const double* d_D, double xscale, d
   a.sel = make_sel(b);
   a.sel.mixed = 0;  // the gradient runs every quartet in FP64
   const int max_blocks = b->ctx->n_cu * 16;
-  for (int c = 0; c < kNumGradClasses; ++c) {
+  // the classes fan out over the side streams as in the J/K build (every
+  // class adds into d_grad with atomics only)
+  if (!b->fork) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LAI_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+      const int rc = ensure_side_streams(b);
+      if (rc != LAI_OK) return rc;
+    }
+  }
+  const bool fan = b->fork != nullptr;
+  hipStream_t streams[kSideStreams + 1] = {st};
+  double load[kSideStreams + 1] = {};
+  if (fan) {
+    LAI_HIP(hipEventRecord(b->fork, st));
+    for (int i = 0; i < kSideStreams; ++i) {
+      streams[i + 1] = b->side[i];
+      LAI_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
+    }
+  }
+  for (int c = kNumGradClasses - 1; c >= 0; --c) {
     const ClassList& C = b->cls[c];
     if (C.nchunks <= 0) continue;
     const bool own = C.ngchunks > 0;  // the J/K chunks are tuned: default ones
     a.chunks = own ? C.gchunks.p : C.chunks.p;
     a.nchunks = int(own ? C.ngchunks : C.nchunks);
-    G.f[c](a, st, max_blocks);
+    int k = 0;
+    for (int i = 1; fan && i <= kSideStreams; ++i)
+      if (load[i] < load[k]) k = i;
+    load[k] += b->cls_f64[c] + 1e6 * double(a.nchunks);
+    G.f[c](a, streams[k], max_blocks);
   }
+  if (fan)
+    for (int i = 0; i < kSideStreams; ++i) {
+      LAI_HIP(hipEventRecord(b->join[i], b->side[i]));
+      LAI_HIP(hipStreamWaitEvent(st, b->join[i], 0));
+    }
   LAI_HIP(hipGetLastError());
   return LAI_OK;
 }
