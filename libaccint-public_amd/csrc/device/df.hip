@@ -377,10 +377,14 @@ struct lai_df {
   int nsplit = 1, kchunk = 0;
   double* B = nullptr;     // [n][naux][n]
   double* X = nullptr;     // [n][naux][n]
-  double* part = nullptr;  // max(n * naux, nsplit * n * n)
-  double* gamma = nullptr; // [naux]
+  double* part = nullptr;   // nsplit * n * n: slices of the exchange product
+  double* gpart = nullptr;  // [n][naux]: partial gamma
+  double* gamma = nullptr;  // [naux]
   hipEvent_t ev[6] = {};
   bool have_ev = false;
+  // the Coulomb passes run beside the exchange GEMMs on this stream
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -389,7 +393,11 @@ void df_release(lai_df* d) {
   if (d->B) (void)hipFree(d->B);
   if (d->X) (void)hipFree(d->X);
   if (d->part) (void)hipFree(d->part);
+  if (d->gpart) (void)hipFree(d->gpart);
   if (d->gamma) (void)hipFree(d->gamma);
+  if (d->side) (void)hipStreamDestroy(d->side);
+  if (d->fork) (void)hipEventDestroy(d->fork);
+  if (d->join) (void)hipEventDestroy(d->join);
   if (d->have_ev)
     for (auto& e : d->ev) (void)hipEventDestroy(e);
   delete d;
@@ -605,18 +613,24 @@ extern "C" int lai_dev_df_create(lai_basis* b, int n_orb_shells, int n_aux_shell
   ns = int((ksum + kc - 1) / kc);
   d->nsplit = ns;
   d->kchunk = int(kc);
-  Dev<double> dX, dpart, dg;
+  Dev<double> dX, dpart, dgp, dg;
   hipError_t e;
-  if ((e = dX.alloc(nB)) != hipSuccess ||
-      (e = dpart.alloc(std::max(size_t(n) * naux, size_t(ns) * n * n))) != hipSuccess ||
-      (e = dg.alloc(size_t(naux))) != hipSuccess) {
+  if ((e = dX.alloc(nB)) != hipSuccess || (e = dpart.alloc(size_t(ns) * n * n)) != hipSuccess ||
+      (e = dgp.alloc(size_t(n) * naux)) != hipSuccess || (e = dg.alloc(size_t(naux))) != hipSuccess) {
     delete d;
     return set_error(LAI_ERR_MEMORY, std::string("lai_dev_df_create: ") + hipGetErrorString(e));
   }
   d->B = dB.take();
   d->X = dX.take();
   d->part = dpart.take();
+  d->gpart = dgp.take();
   d->gamma = dg.take();
+  if ((e = hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&d->fork, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&d->join, hipEventDisableTiming)) != hipSuccess) {
+    df_release(d);
+    return set_error(LAI_ERR_BACKEND, std::string("lai_dev_df_create: ") + hipGetErrorString(e));
+  }
   *out = d;
   return LAI_OK;
 }
@@ -634,16 +648,24 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
   }
   if (ms)
     for (int i = 0; i < 5; ++i) ms[i] = 0.0;
+  // J and K together (not profiled): the Coulomb passes (HBM-bound) on the
+  // side stream beside the exchange GEMMs (matrix cores), joined back into st
+  const bool overlap = !ms && (flags & LAI_DF_J) && (flags & LAI_DF_K) && d->side;
+  const hipStream_t sj = overlap ? d->side : st;
+  if (overlap) {
+    DF_HIP(hipEventRecord(d->fork, st));
+    DF_HIP(hipStreamWaitEvent(d->side, d->fork, 0));
+  }
   if (flags & LAI_DF_J) {
     int rc = mark(0);
     if (rc) return rc;
-    hipLaunchKernelGGL(df_gamma_part, dim3(n, (naux + kGammaRows - 1) / kGammaRows), dim3(256), 0, st, d->B, D, n, naux,
-                       d->part);
+    hipLaunchKernelGGL(df_gamma_part, dim3(n, (naux + kGammaRows - 1) / kGammaRows), dim3(256), 0, sj, d->B, D, n,
+                       naux, d->gpart);
     DF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 63) / 64), dim3(1024), 0, st, d->part, n, naux, d->gamma);
+    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 63) / 64), dim3(1024), 0, sj, d->gpart, n, naux, d->gamma);
     DF_HIP(hipGetLastError());
     if ((rc = mark(1))) return rc;
-    hipLaunchKernelGGL(df_coulomb, dim3((n + 63) / 64, n), dim3(256), 0, st, d->B, d->gamma, n, naux, J);
+    hipLaunchKernelGGL(df_coulomb, dim3((n + 63) / 64, n), dim3(256), 0, sj, d->B, d->gamma, n, naux, J);
     DF_HIP(hipGetLastError());
     if ((rc = mark(2))) return rc;
     if (ms) {
@@ -701,6 +723,10 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
       ms[3] = t1;
       ms[4] = t2;
     }
+  }
+  if (overlap) {
+    DF_HIP(hipEventRecord(d->join, d->side));
+    DF_HIP(hipStreamWaitEvent(st, d->join, 0));
   }
   return LAI_OK;
 }
