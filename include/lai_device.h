@@ -285,6 +285,10 @@ int lai_dev_df_dims(const lai_df* df, int* n_orb, int* n_aux);
  * compute_k_internal (src/shared/consumers/df_fock_builder.cpp:283-378). */
 #define LAI_DF_J 1
 #define LAI_DF_K 2
+/* with LAI_DF_K: the caller states that D is symmetric, so K is symmetric;
+ * the exchange product then skips the workgroup tiles below the diagonal and
+ * copies the lower triangle of K from the upper one */
+#define LAI_DF_SYMMETRIC 4
 int lai_dev_df_jk(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags, void* stream);
 /* as lai_dev_df_jk, synchronous, with each kernel timed by HIP events on
  * stream: ms[0] the gamma pass, ms[1] the J pass, ms[2] the X = B D GEMM,

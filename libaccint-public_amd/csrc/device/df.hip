@@ -66,6 +66,7 @@ struct GemmArgs {
   long long sA, sB, sC;
   int M, N, K;
   int bt, split, kchunk;
+  int skip_lower;  // 1: tiles whose every row is below every column of C are not computed
 };
 
 template <int BM, int BN>
@@ -77,6 +78,7 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs g) {
   __shared__ double Bs[2][KT][TN + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN, z = blockIdx.z;
+  if (g.skip_lower && m0 > n0 + TN - 1) return;  // uniform over the workgroup, before any barrier
   const double* A = g.A;
   const double* B = g.B;
   double* C = g.C + z * g.sC;
@@ -170,6 +172,15 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs g) {
         const int col = n0 + 16 * j + (lane & 15);
         if (row < g.M && col < g.N) C[(long long)row * g.ldc + col] = acc[i][j][r];
       }
+}
+
+// K[r][c] = K[c][r] for r > c (n x n, row-major)
+__global__ __launch_bounds__(256) void mirror_lower(double* __restrict__ K, int n) {
+  const long long nn = (long long)n * n;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < nn; i += 256ll * gridDim.x) {
+    const int r = int(i / n), c = int(i - (long long)r * n);
+    if (r > c) K[i] = K[(long long)c * n + r];
+  }
 }
 
 // out[i] = sum over the ns slices part[z slice + i]
@@ -706,12 +717,17 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     h.bt = 1;
     h.split = 1;
     h.kchunk = d->kchunk;
+    h.skip_lower = (flags & LAI_DF_SYMMETRIC) ? 1 : 0;
     if ((rc = gemm(h, d->nsplit, st))) return rc;
     if ((rc = mark(2))) return rc;
     const long long nn = (long long)n * n;
     hipLaunchKernelGGL(sum_slices, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, d->part, nn,
                        d->nsplit, nn, K);
     DF_HIP(hipGetLastError());
+    if (flags & LAI_DF_SYMMETRIC) {  // the skipped tiles hold stale slices: take the upper triangle
+      hipLaunchKernelGGL(mirror_lower, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, K, n);
+      DF_HIP(hipGetLastError());
+    }
     if ((rc = mark(3))) return rc;
     if (ms) {
       float t0, t1, t2;
@@ -733,14 +749,15 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
 }  // namespace
 
 extern "C" int lai_dev_df_jk(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
-  if (!df || !d_D || (flags & ~(LAI_DF_J | LAI_DF_K)) || ((flags & LAI_DF_J) && !d_J) || ((flags & LAI_DF_K) && !d_K))
+  if (!df || !d_D || (flags & ~(LAI_DF_J | LAI_DF_K | LAI_DF_SYMMETRIC)) || ((flags & LAI_DF_J) && !d_J) ||
+      ((flags & LAI_DF_K) && !d_K))
     return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_jk: invalid arguments");
   return df_run(df, d_D, d_J, d_K, flags, hipStream_t(stream), nullptr);
 }
 
 extern "C" int lai_dev_df_jk_profile(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags,
                                      void* stream, double* ms) {
-  if (!df || !d_D || !ms || (flags & ~(LAI_DF_J | LAI_DF_K)) || ((flags & LAI_DF_J) && !d_J) ||
+  if (!df || !d_D || !ms || (flags & ~(LAI_DF_J | LAI_DF_K | LAI_DF_SYMMETRIC)) || ((flags & LAI_DF_J) && !d_J) ||
       ((flags & LAI_DF_K) && !d_K))
     return set_error(LAI_ERR_INVALID_ARGUMENT, "lai_dev_df_jk_profile: invalid arguments");
   return df_run(df, d_D, d_J, d_K, flags, hipStream_t(stream), ms);

@@ -253,13 +253,24 @@ void DFFockBuilder::run(int flags, const std::vector<Real>& D, std::vector<Real>
 
 void DFFockBuilder::compute_j_internal() { run(LAI_DF_J, D_, &J_, nullptr); }
 
+namespace {
+// exact symmetry of a row-major n x n matrix (then K is symmetric too)
+bool symmetric(const std::vector<Real>& M, Size n) {
+    for (Size i = 0; i < n; ++i)
+        for (Size j = i + 1; j < n; ++j)
+            if (M[i * n + j] != M[j * n + i]) return false;
+    return true;
+}
+}  // namespace
+
 void DFFockBuilder::compute_k_internal() {
+    auto kflags = [&](const std::vector<Real>& D) { return LAI_DF_K | (symmetric(D, n_orb_) ? LAI_DF_SYMMETRIC : 0); };
     if (uhf_mode_) {
-        run(LAI_DF_K, D_alpha_, nullptr, &K_alpha_);
-        run(LAI_DF_K, D_beta_, nullptr, &K_beta_);
+        run(kflags(D_alpha_), D_alpha_, nullptr, &K_alpha_);
+        run(kflags(D_beta_), D_beta_, nullptr, &K_beta_);
         for (Size i = 0; i < n_orb_ * n_orb_; ++i) K_[i] = K_alpha_[i] + K_beta_[i];
     } else {
-        run(LAI_DF_K, D_, nullptr, &K_);
+        run(kflags(D_), D_, nullptr, &K_);
     }
 }
 

@@ -120,7 +120,8 @@ def rhf(atoms, basis_name, n_electrons, max_iter=100, e_tol=1e-12, d_tol=1e-10, 
     def fock(D):
         Dc = D.contiguous()
         if dfb is not None:  # J and K overwritten
-            _cabi.check(_cabi.lib().lai_dev_df_jk(dfh, Dc.data_ptr(), J.data_ptr(), K.data_ptr(), 3,
+            # D = 2 C_occ C_occ^T is symmetric: LAI_DF_SYMMETRIC (4) computes K's upper part
+            _cabi.check(_cabi.lib().lai_dev_df_jk(dfh, Dc.data_ptr(), J.data_ptr(), K.data_ptr(), 3 | 4,
                                                   stream.cuda_stream))
         else:
             J.zero_()

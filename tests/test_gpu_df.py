@@ -213,3 +213,26 @@ def test_df_rhf_energy_matches_oracle_scf():
     E0 = _numpy_rhf(H, S, jk, 5, e_nuc)
     assert abs(res.energy - E0) < 1e-8
     assert abs(res.energy - exact.energy) < 2e-3
+
+
+def test_symmetric_and_general_density_paths():
+    """An exactly symmetric D takes the exchange path that computes K's upper
+    part and mirrors it; a non-symmetric D the full product.  Both against
+    the oracle."""
+    print("df symmetric / general D", flush=True)
+    atoms = systems.water_cluster(2, 8.0, seed=3)
+    orb = oracle.RawBasis("cc-pvdz", atoms)
+    aux = oracle.RawBasis("cc-pvdz-rifit", atoms)
+    D = density(orb.nbf, 10, 8)
+    Ds = 0.5 * (D + D.T)  # exactly symmetric
+    Dg = D + 0.1 * np.random.default_rng(9).standard_normal(D.shape) / orb.nbf  # not symmetric
+    f = builder(orb, aux)
+    for Dm in (Ds, Dg):
+        J0, K0 = oracle.df_jk(orb, aux, Dm)
+        f.set_density(np.ascontiguousarray(Dm))
+        assert rel(f.compute_coulomb(), J0) < TOL and rel(f.compute_exchange(), K0) < TOL
+    K = f.exchange_matrix()
+    assert not np.allclose(K, K.T)  # the general density's K is not symmetric
+    f.set_density(Ds)
+    K = f.compute_exchange()
+    assert np.array_equal(K, K.T)  # mirrored exactly
