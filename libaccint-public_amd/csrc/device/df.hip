@@ -385,7 +385,8 @@ void lower_inverse(const std::vector<double>& L, std::vector<double>& X, int n) 
 
 struct lai_df {
   int n = 0, naux = 0;
-  int nsplit = 1, kchunk = 0;
+  int nsplit = 1, kchunk = 0;          // slices of the exchange product
+  int nsplit_sym = 1, kchunk_sym = 0;  // the same over the tiles a symmetric D needs
   double* B = nullptr;     // [n][naux][n]
   double* X = nullptr;     // [n][naux][n]
   double* part = nullptr;   // nsplit * n * n: slices of the exchange product
@@ -615,15 +616,26 @@ extern "C" int lai_dev_df_create(lai_basis* b, int n_orb_shells, int n_aux_shell
   auto* d = new lai_df;
   d->n = n;
   d->naux = naux;
-  // slices of the exchange GEMM: about 8 workgroups per CU over its tiles
-  const long long tiles = (long long)((n + 63) / 64) * ((n + 79) / 80);  // 64 x 80 tiles (gemm)
+  // slices of the exchange GEMM: about 2048 workgroups (8 per CU) over its
+  // tiles, all of them or those a symmetric D needs (gemm's tile rule)
+  const int TMk = n >= 1024 ? 128 : 64, TNk = 80;
+  long long tiles = 0, tiles_sym = 0;
+  for (int mi = 0; mi < (n + TMk - 1) / TMk; ++mi)
+    for (int ni = 0; ni < (n + TNk - 1) / TNk; ++ni) {
+      ++tiles;
+      if (!(mi * TMk > ni * TNk + TNk - 1)) ++tiles_sym;
+    }
   const long long ksum = (long long)naux * n;
-  int ns = int(std::max(1ll, std::min(2048ll / tiles, (ksum + 16 * KT - 1) / (16 * KT))));
-  long long kc = (ksum + ns - 1) / ns;
-  kc = (kc + KT - 1) / KT * KT;
-  ns = int((ksum + kc - 1) / kc);
-  d->nsplit = ns;
-  d->kchunk = int(kc);
+  auto slices = [&](long long t, int& ns_out, int& kc_out) {
+    int ns = int(std::max(1ll, std::min(std::max(1ll, 2048ll / t), (ksum + 16 * KT - 1) / (16 * KT))));
+    long long kc = (ksum + ns - 1) / ns;
+    kc = (kc + KT - 1) / KT * KT;
+    ns_out = int((ksum + kc - 1) / kc);
+    kc_out = int(kc);
+  };
+  slices(tiles, d->nsplit, d->kchunk);
+  slices(tiles_sym, d->nsplit_sym, d->kchunk_sym);
+  const int ns = std::max(d->nsplit, d->nsplit_sym);
   Dev<double> dX, dpart, dgp, dg;
   hipError_t e;
   if ((e = dX.alloc(nB)) != hipSuccess || (e = dpart.alloc(size_t(ns) * n * n)) != hipSuccess ||
@@ -716,13 +728,14 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     h.K = naux * n;
     h.bt = 1;
     h.split = 1;
-    h.kchunk = d->kchunk;
     h.skip_lower = (flags & LAI_DF_SYMMETRIC) ? 1 : 0;
-    if ((rc = gemm(h, d->nsplit, st))) return rc;
+    h.kchunk = h.skip_lower ? d->kchunk_sym : d->kchunk;
+    const int nsl = h.skip_lower ? d->nsplit_sym : d->nsplit;
+    if ((rc = gemm(h, nsl, st))) return rc;
     if ((rc = mark(2))) return rc;
     const long long nn = (long long)n * n;
     hipLaunchKernelGGL(sum_slices, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, d->part, nn,
-                       d->nsplit, nn, K);
+                       nsl, nn, K);
     DF_HIP(hipGetLastError());
     if (flags & LAI_DF_SYMMETRIC) {  // the skipped tiles hold stale slices: take the upper triangle
       hipLaunchKernelGGL(mirror_lower, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, K, n);
