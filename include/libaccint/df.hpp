@@ -157,6 +157,7 @@ private:
     Real cond_{0.0};
     lai_basis* basis_{nullptr};
     lai_df* df_{nullptr};
+    device::DeviceArray dD_, dJ_, dK_;  // n_orb^2 each, allocated by initialize()
 };
 
 /// df_fock_builder.cpp:476-509: without atom data, one s shell of four fixed

@@ -143,7 +143,10 @@ DFFockBuilder::DFFockBuilder(DFFockBuilder&& o) noexcept
       K_beta_(std::move(o.K_beta_)),
       cond_(o.cond_),
       basis_(o.basis_),
-      df_(o.df_) {
+      df_(o.df_),
+      dD_(std::move(o.dD_)),
+      dJ_(std::move(o.dJ_)),
+      dK_(std::move(o.dK_)) {
     o.basis_ = nullptr;
     o.df_ = nullptr;
 }
@@ -168,6 +171,9 @@ DFFockBuilder& DFFockBuilder::operator=(DFFockBuilder&& o) noexcept {
         cond_ = o.cond_;
         basis_ = o.basis_;
         df_ = o.df_;
+        dD_ = std::move(o.dD_);
+        dJ_ = std::move(o.dJ_);
+        dK_ = std::move(o.dK_);
         o.basis_ = nullptr;
         o.df_ = nullptr;
     }
@@ -239,15 +245,18 @@ void DFFockBuilder::initialize() {
         device::check(rc, "DFFockBuilder::initialize");
     }
     df_ = d;
+    const Size nn = n_orb_ * n_orb_;
+    dD_ = device::DeviceArray(nn);
+    dJ_ = device::DeviceArray(nn);
+    dK_ = device::DeviceArray(nn);
 }
 
 void DFFockBuilder::run(int flags, const std::vector<Real>& D, std::vector<Real>* J, std::vector<Real>* K) {
     const Size nn = n_orb_ * n_orb_;
-    device::DeviceArray dD(nn), dJ(nn), dK(nn);
-    dD.upload(D.data(), nn);
-    device::check(lai_dev_df_jk(df_, dD.get(), dJ.get(), dK.get(), flags, nullptr), "DFFockBuilder::compute");
-    if (J) dJ.download(J->data(), nn);
-    if (K) dK.download(K->data(), nn);
+    dD_.upload(D.data(), nn);
+    device::check(lai_dev_df_jk(df_, dD_.get(), dJ_.get(), dK_.get(), flags, nullptr), "DFFockBuilder::compute");
+    if (J) dJ_.download(J->data(), nn);
+    if (K) dK_.download(K->data(), nn);
     device::check(lai_dev_stream_sync(nullptr), "DFFockBuilder::compute");
 }
 
