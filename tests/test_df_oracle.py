@@ -218,3 +218,20 @@ def test_builder_config_and_argument_checks():
     assert g.n_aux() == 3
     h = lai.make_df_fock_builder(orb, [8, 1, 1], [list(p) for _, p in WATER])
     assert h.n_aux() == 96
+
+
+def test_df_cabi_rejects_invalid_arguments():
+    """The density-fitting C ABI checks its arguments before any device work."""
+    import ctypes
+    from libaccint import _cabi
+    L = _cabi.lib()
+    out = ctypes.c_void_p()
+    cond = ctypes.c_double()
+    assert L.lai_dev_df_create(None, 3, 2, 0, 1e14, ctypes.byref(cond), ctypes.byref(out)) == -1
+    assert b"invalid arguments" in L.lai_dev_last_error()
+    assert L.lai_dev_df_jk(None, None, None, None, 3, None) == -1
+    assert b"lai_dev_df_jk" in L.lai_dev_last_error()
+    assert L.lai_dev_df_jk_profile(None, None, None, None, 3, None, None) == -1
+    n = ctypes.c_int()
+    assert L.lai_dev_df_dims(None, ctypes.byref(n), ctypes.byref(n)) == -1
+    assert L.lai_dev_df_free(None) == 0
