@@ -127,3 +127,39 @@ def test_batch_engine_autotune_keeps_results():
     assert len(tab) > 0
     J1, K1 = be.build_jk(Db, opts)
     assert rel(J1, J0) < 1e-12 and rel(K1, K0) < 1e-12
+
+
+def test_concurrent_classes_equal_serial_build():
+    """lai_dev_jk spreads the class kernels over four streams (fork/join);
+    lai_dev_jk_profile runs them one after another on one stream.  Same J/K
+    to the order of the FP64 atomic sums, and the four-stream build is
+    repeatable on the caller's stream."""
+    torch = pytest.importorskip("torch")
+    atoms = systems.water_cluster(3, 9.0, seed=8)
+    b = lai.basis_set("cc-pvdz", atoms)
+    n = b.n_basis_functions()
+    D = systems.density(n, 15, 21, 1.0 / np.sqrt(n))
+    e = lai.Engine(b)
+    e.prepare_worklist(lai.JKOptions(schwarz_threshold=1e-12))
+    L = _cabi.lib()
+    h = ctypes.c_void_p(e.device_basis_handle())
+    s = torch.cuda.Stream()
+    tD = torch.from_numpy(D).cuda()
+    P = ctypes.POINTER(ctypes.c_double)
+    out = []
+    for prof in (False, True, False):
+        tJ = torch.zeros_like(tD)
+        tK = torch.zeros_like(tD)
+        if prof:
+            m64, m32 = np.zeros(55), np.zeros(55)
+            _cabi.check(L.lai_dev_jk_profile(h, ctypes.c_void_p(tD.data_ptr()), ctypes.c_void_p(tJ.data_ptr()),
+                                             ctypes.c_void_p(tK.data_ptr()), 0, ctypes.c_void_p(s.cuda_stream),
+                                             m64.ctypes.data_as(P), m32.ctypes.data_as(P)))
+        else:
+            _cabi.check(L.lai_dev_jk(h, ctypes.c_void_p(tD.data_ptr()), ctypes.c_void_p(tJ.data_ptr()),
+                                     ctypes.c_void_p(tK.data_ptr()), 0, ctypes.c_void_p(s.cuda_stream)))
+        s.synchronize()
+        out.append((tJ.cpu().numpy(), tK.cpu().numpy()))
+    (J0, K0), (J1, K1), (J2, K2) = out
+    assert rel(J0, J1) < 1e-12 and rel(K0, K1) < 1e-12
+    assert rel(J0, J2) < 1e-12 and rel(K0, K2) < 1e-12
