@@ -67,6 +67,13 @@ struct GemmArgs {
   int M, N, K;
   int bt, split, kchunk;
   int skip_lower;  // 1: tiles whose every row is below every column of C are not computed
+  // 1: one-dimensional grid in an XCD-aware order.  Workgroups go to the 8
+  // XCDs (each with its own L2) round-robin by block id, so block b runs on
+  // XCD b % 8: split = 1 puts every tile of slice z on XCD z % 8 (the slice's
+  // rows of A and B are read into one L2), split = 0 every column tile of row
+  // tile m on XCD m % 8 (the row tile of A is read once).  mt, nt: tiles
+  // along M and N; nz: slices.
+  int xcd, mt, nt, nz;
 };
 
 template <int BM, int BN>
@@ -77,7 +84,26 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs g) {
   __shared__ double As[2][TM][KT + 1];
   __shared__ double Bs[2][KT][TN + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN, z = blockIdx.z;
+  int mi = blockIdx.x, ni = blockIdx.y, z = blockIdx.z;
+  if (g.xcd) {  // every return below is uniform over the workgroup, before any barrier
+    const int b = blockIdx.x;
+    if (g.split) {
+      const int T = g.mt * g.nt;
+      const int grp = b / (8 * T), w = b - grp * 8 * T;
+      const int t = w / 8;
+      z = grp * 8 + (w - t * 8);
+      if (z >= g.nz) return;
+      mi = t / g.nt;
+      ni = t - mi * g.nt;
+    } else {
+      const int grp = b / (8 * g.nt), w = b - grp * 8 * g.nt;
+      ni = w / 8;
+      mi = grp * 8 + (w - ni * 8);
+      if (mi >= g.mt) return;
+      z = 0;
+    }
+  }
+  const int m0 = mi * TM, n0 = ni * TN;
   if (g.skip_lower && m0 > n0 + TN - 1) return;  // uniform over the workgroup, before any barrier
   const double* A = g.A;
   const double* B = g.B;
@@ -324,11 +350,24 @@ struct Dev {
 
 // C tiles of 128 x 80 (BM = 2) for tall products, 64 x 80 (BM = 1) when M is
 // small (the exchange product, M = n_orb).
-int gemm(const GemmArgs& g, int gz, hipStream_t st) {
+int gemm(const GemmArgs& g0, int gz, hipStream_t st) {
+  GemmArgs g = g0;
   if (g.M <= 0 || g.N <= 0 || gz <= 0) return LAI_OK;
   const bool tall = g.M >= 1024;
   const int TM = tall ? 128 : 64, TN = 80;
-  const dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, gz);
+  dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, gz);
+  if (g.xcd) {
+    if (!g.split && gz != 1)
+      return set_error(LAI_ERR_INVALID_ARGUMENT, "density fitting: XCD order needs one batch or a split product");
+    g.mt = int(grid.x);
+    g.nt = int(grid.y);
+    g.nz = gz;
+    const long long blocks = g.split ? (long long)(gz + 7) / 8 * 8 * g.mt * g.nt
+                                     : (long long)(g.mt + 7) / 8 * 8 * g.nt;
+    if (blocks > 0x7fffffffll)
+      return set_error(LAI_ERR_INVALID_ARGUMENT, "density fitting: GEMM grid beyond the launch limits");
+    grid = dim3(unsigned(blocks), 1, 1);
+  }
   if (grid.y > 65535u || grid.z > 65535u)
     return set_error(LAI_ERR_INVALID_ARGUMENT, "density fitting: GEMM grid beyond the launch limits");
   if (tall)
@@ -713,6 +752,7 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     g.M = n * naux;
     g.N = n;
     g.K = n;
+    g.xcd = 1;
     if ((rc = gemm(g, 1, st))) return rc;
     if ((rc = mark(1))) return rc;
     GemmArgs h{};
@@ -729,6 +769,7 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     h.bt = 1;
     h.split = 1;
     h.skip_lower = (flags & LAI_DF_SYMMETRIC) ? 1 : 0;
+    h.xcd = 1;
     h.kchunk = h.skip_lower ? d->kchunk_sym : d->kchunk;
     const int nsl = h.skip_lower ? d->nsplit_sym : d->nsplit;
     if ((rc = gemm(h, nsl, st))) return rc;
