@@ -123,7 +123,11 @@ struct JkChunk {
 // LAI_BT_OFF (128-byte aligned) the Boys Taylor rows (lai_boys_taylor_h).
 // ---------------------------------------------------------------------------
 #define LAI_BT_OFF (((LAI_RYS_TABLE_SIZE + 15) / 16) * 16)
-#define LAI_DEVICE_TABLE_SIZE (LAI_BT_OFF + LAI_BT_ROWS * LAI_BT_M)
+// From LAI_B8_OFF: F_0..F_7 of every Boys Taylor row again, 8 doubles (64 B) a
+// row, for the one-root evaluations (boys01, rys_roots<1>), which need no
+// higher order: half the cache footprint of the 128-byte rows.
+#define LAI_B8_OFF (LAI_BT_OFF + LAI_BT_ROWS * LAI_BT_M)
+#define LAI_DEVICE_TABLE_SIZE (LAI_B8_OFF + LAI_BT_ROWS * 8)
 
 // Taylor step from the grid row r (F_m(T_k)) to F_m(T_k + d):
 // F_m + e1 (F_{m+1} + e2 (F_{m+2} + ... + e6 F_{m+6})) with e_j = -d / j.
@@ -152,7 +156,7 @@ __device__ __forceinline__ void boys01(double T, const double* __restrict__ tab,
     double e[7];
 #pragma unroll
     for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
-    const double* r = tab + LAI_BT_OFF + k * LAI_BT_M;
+    const double* r = tab + LAI_B8_OFF + k * 8;
     double rr[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) rr[m] = r[m];
@@ -179,7 +183,7 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
       double e[7];
 #pragma unroll
       for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
-      const double* r = tab + LAI_BT_OFF + k * LAI_BT_M;
+      const double* r = N == 1 ? tab + LAI_B8_OFF + k * 8 : tab + LAI_BT_OFF + k * LAI_BT_M;
       double rr[2 * N + 6];  // F_0 .. F_{2N+5}: orders up to 2N-1 plus six Taylor terms
 #pragma unroll
       for (int m = 0; m < 2 * N + 6; ++m) rr[m] = r[m];
