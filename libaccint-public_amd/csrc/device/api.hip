@@ -320,8 +320,10 @@ extern "C" int lai_dev_init(int device, lai_ctx** out) {
     tab[LAI_RYS_HW_OFF + i] = lai_rys_hw[i];
   }
   std::memcpy(tab.data() + LAI_BT_OFF, lai_boys_taylor_h, sizeof(double) * LAI_BT_ROWS * LAI_BT_M);
-  for (int k = 0; k < LAI_BT_ROWS; ++k)
+  for (int k = 0; k < LAI_BT_ROWS; ++k) {
     std::memcpy(tab.data() + LAI_B8_OFF + size_t(k) * 8, lai_boys_taylor_h + size_t(k) * LAI_BT_M, sizeof(double) * 8);
+    std::memcpy(tab.data() + LAI_B10_OFF + size_t(k) * 10, lai_boys_taylor_h + size_t(k) * LAI_BT_M, sizeof(double) * 10);
+  }
   hipError_t e = ctx->rys.assign(tab.data(), tab.size());
   if (e != hipSuccess) {
     delete ctx;

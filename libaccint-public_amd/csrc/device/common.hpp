@@ -127,7 +127,10 @@ struct JkChunk {
 // row, for the one-root evaluations (boys01, rys_roots<1>), which need no
 // higher order: half the cache footprint of the 128-byte rows.
 #define LAI_B8_OFF (LAI_BT_OFF + LAI_BT_ROWS * LAI_BT_M)
-#define LAI_DEVICE_TABLE_SIZE (LAI_B8_OFF + LAI_BT_ROWS * 8)
+// From LAI_B10_OFF: F_0..F_9 of every row, 10 doubles (80 B) a row, for
+// rys_roots<2>.
+#define LAI_B10_OFF (LAI_B8_OFF + LAI_BT_ROWS * 8)
+#define LAI_DEVICE_TABLE_SIZE (LAI_B10_OFF + LAI_BT_ROWS * 10)
 
 // Taylor step from the grid row r (F_m(T_k)) to F_m(T_k + d):
 // F_m + e1 (F_{m+1} + e2 (F_{m+2} + ... + e6 F_{m+6})) with e_j = -d / j.
@@ -183,7 +186,7 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
       double e[7];
 #pragma unroll
       for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
-      const double* r = N == 1 ? tab + LAI_B8_OFF + k * 8 : tab + LAI_BT_OFF + k * LAI_BT_M;
+      const double* r = N == 1 ? tab + LAI_B8_OFF + k * 8 : tab + LAI_B10_OFF + k * 10;
       double rr[2 * N + 6];  // F_0 .. F_{2N+5}: orders up to 2N-1 plus six Taylor terms
 #pragma unroll
       for (int m = 0; m < 2 * N + 6; ++m) rr[m] = r[m];
