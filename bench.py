@@ -357,7 +357,7 @@ def main():
         grad = {"gradient_seconds": gs, "gradient_workload": "32-water def2-SVP (800 Cartesian bf), Schwarz 1e-12, "
                 "%d quartets, dE/dR for E = 1/2 tr(DJ) - 1/4 tr(DK)" % ginfo["n_quartets"],
                 "gradient_translational_residual": float(np.max(np.abs(g.sum(axis=0))))}
-        log("gradient (config 5): %.3f s" % gs)
+        log("gradient (32-water def2-SVP): %.3f s" % gs)
 
     # ------------------------------------------------------ density fitting
     df = {}
