@@ -68,7 +68,7 @@ def main():
                     help="write the 16-water build's dispatch table (JSON) to this path")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="end with a device fill and copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--config4-full", action="store_true",
+    ap.add_argument("--batch-full", action="store_true",
                     help="time all 64 molecules of the batched workload (several minutes per build)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -272,7 +272,7 @@ def main():
     if not args.no_batch:
         rng = np.random.default_rng(4)
         mols = [systems.random_molecule(rng) for _ in range(64)]
-        sel = list(range(64)) if args.config4_full else [k for k, m in enumerate(mols) if len(m) <= 20]
+        sel = list(range(64)) if args.batch_full else [k for k, m in enumerate(mols) if len(m) <= 20]
         bases = [lai.basis_set("cc-pvtz", mols[k]) for k in sel]
         be = lai.BatchEngine(bases)
         bnbf = be.n_basis_functions()
@@ -299,9 +299,9 @@ def main():
             torch.cuda.synchronize()
             return ev0.elapsed_time(ev1) / steps
 
-        bsteps = 1 if args.config4_full else 3
+        bsteps = 1 if args.batch_full else 3
         bt = 0.0
-        if not args.no_tune and not args.config4_full:
+        if not args.no_tune and not args.batch_full:
             t0 = time.perf_counter()
             be.autotune(tbD.data_ptr(), bo, stream.cuda_stream, 1)
             bt = time.perf_counter() - t0
@@ -328,7 +328,7 @@ def main():
         ms_seq = ev0.elapsed_time(ev1)
         batch = {"batch_workload": "%d of the 64 seeded H/C/N/O molecules in cc-pVTZ (%s), nbf %d..%d, "
                                    "Schwarz 1e-12, %d quartets, %.3e ERIs" %
-                                   (len(sel), "all" if args.config4_full else "those with <= 20 atoms",
+                                   (len(sel), "all" if args.batch_full else "those with <= 20 atoms",
                                     min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
                  "batch_setup_seconds": bsetup, "batch_autotune_seconds": bt,
                  "batch_jk_seconds_fp64": ms_b * 1e-3, "batch_eris_per_s_fp64": binfo["n_eri"] / (ms_b * 1e-3),

@@ -2,7 +2,7 @@
 the reference tree; the outputs are committed, the reference is not needed at
 test time).
 
-    python tests/golden/make_fixtures.py [--config2]
+    python tests/golden/make_fixtures.py [--sixteen-waters]
 
 Writes into tests/golden/:
   reference_literals.json  PySCF values quoted as literals by the reference's
@@ -17,10 +17,10 @@ Writes into tests/golden/:
   water_aug_cc_pvtz.npz    tests/data/reference/water_1_aug_cc_pVTZ.json
                            (PySCF, Cartesian, LibAccInt ordering): S, T, V, D,
                            J, K, sampled ERIs, geometry
-  ref_config1.npz          outputs of the compiled reference CPU path
+  ref_h2o_sto3g.npz        outputs of the compiled reference CPU path
                            (oracle/_ref) for H2O/STO-3G: 7^4 ERIs, S, T, V, J, K
                            for the seeded density
-  ref_config2.npz          (--config2) the compiled reference for the seeded
+  ref_16water_ccpvdz.npz   (--sixteen-waters) the compiled reference for the seeded
                            16-water cc-pVDZ cluster: Schwarz table, screened
                            list count/checksum at 1e-12, J, K
 """
@@ -111,7 +111,7 @@ def data_files():
     )
 
 
-def ref_config1():
+def ref_h2o_sto3g():
     import oracle
     from libaccint import systems
     rb = oracle.RawBasis("sto-3g", systems.h2o_sto3g_geometry())
@@ -120,11 +120,11 @@ def ref_config1():
     C = rng.standard_normal((7, 5)) / np.sqrt(7)
     D = C @ C.T
     J, K = r.jk_full(D, n_threads=1)
-    np.savez_compressed(os.path.join(HERE, "ref_config1.npz"), eri=r.eri_tensor(), S=r.overlap(), T=r.kinetic(),
+    np.savez_compressed(os.path.join(HERE, "ref_h2o_sto3g.npz"), eri=r.eri_tensor(), S=r.overlap(), T=r.kinetic(),
                         V=r.nuclear(), D=D, J=J, K=K)
 
 
-def ref_config2():
+def ref_16water_ccpvdz():
     import oracle
     from libaccint import systems
     atoms = systems.water_cluster(16, 14.0, seed=2)
@@ -134,17 +134,17 @@ def ref_config2():
     Q = r.schwarz()
     n, cs = oracle.Oracle(rb).screened_list(Q, 1e-12)
     J, K = r.jk_canonical(D, Q, 1e-12, n_threads=0)
-    np.savez_compressed(os.path.join(HERE, "ref_config2.npz"), Q=Q, n_quartets=n, checksum=np.uint64(cs), J=J, K=K,
+    np.savez_compressed(os.path.join(HERE, "ref_16water_ccpvdz.npz"), Q=Q, n_quartets=n, checksum=np.uint64(cs), J=J, K=K,
                         atoms_xyz=np.array([p for _, p in atoms]), atoms_Z=np.array([z for z, _ in atoms]))
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config2", action="store_true")
+    ap.add_argument("--sixteen-waters", action="store_true")
     a = ap.parse_args()
-    if a.config2:
-        ref_config2()
+    if a.sixteen_waters:
+        ref_16water_ccpvdz()
     else:
         literals()
         data_files()
-        ref_config1()
+        ref_h2o_sto3g()

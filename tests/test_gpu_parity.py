@@ -35,9 +35,9 @@ def engine_for(name, atoms):
     return b, lai.Engine(b)
 
 
-# ----------------------------------------------------------------- config 1
-def test_config1_eri_tensor_and_literals():
-    ref = np.load(golden("ref_config1.npz"))
+# ------------------------------------------------------ H2O STO-3G
+def test_h2o_sto3g_eri_tensor_and_literals():
+    ref = np.load(golden("ref_h2o_sto3g.npz"))
     b, e = engine_for("sto-3g", systems.h2o_sto3g_geometry())
     E = e.compute_eri_tensor()
     assert np.max(np.abs(E - ref["eri"])) < 1e-12
@@ -45,8 +45,8 @@ def test_config1_eri_tensor_and_literals():
         assert abs(E[i, j, k, l] - v) < 1e-12
 
 
-def test_config1_one_electron():
-    ref = np.load(golden("ref_config1.npz"))
+def test_h2o_sto3g_one_electron():
+    ref = np.load(golden("ref_h2o_sto3g.npz"))
     atoms = systems.h2o_sto3g_geometry()
     b, e = engine_for("sto-3g", atoms)
     A = lai.atoms_from_list(atoms)
@@ -58,8 +58,8 @@ def test_config1_one_electron():
     assert np.max(np.abs(H - (T + V))) < 1e-13
 
 
-def test_config1_fock_builders():
-    ref = np.load(golden("ref_config1.npz"))
+def test_h2o_sto3g_fock_builders():
+    ref = np.load(golden("ref_h2o_sto3g.npz"))
     b, e = engine_for("sto-3g", systems.h2o_sto3g_geometry())
     D = np.ascontiguousarray(ref["D"])
     fb = lai.FockBuilder(7)
@@ -155,7 +155,7 @@ def test_jk_vs_oracle_two_waters_cc_pvdz():
     assert np.allclose(J, J.T, atol=1e-13) and np.allclose(K, K.T, atol=1e-13)
 
 
-# ---------------------------------------------------- screening, config 2
+# ------------------------------- screening, 16 waters cc-pVDZ
 def test_schwarz_vs_oracle():
     atoms = systems.water_cluster(3, 9.0, seed=8)
     b, e = engine_for("def2-svp", atoms)
@@ -171,9 +171,9 @@ def test_schwarz_vs_oracle():
     assert info["n_quartets"] == n and info["checksum"] == cs
 
 
-@pytest.mark.skipif(not os.path.exists(golden("ref_config2.npz")), reason="config-2 fixture not generated")
-def test_config2_against_reference_run():
-    ref = np.load(golden("ref_config2.npz"))
+@pytest.mark.skipif(not os.path.exists(golden("ref_16water_ccpvdz.npz")), reason="16-water fixture not generated")
+def test_sixteen_waters_against_reference_run():
+    ref = np.load(golden("ref_16water_ccpvdz.npz"))
     atoms = systems.water_cluster(16, 14.0, seed=2)
     assert np.allclose(np.array([p for _, p in atoms]), ref["atoms_xyz"])
     b, e = engine_for("cc-pvdz", atoms)

@@ -201,8 +201,8 @@ def test_screened_list_counts():
 
 
 # --------------------------------------------------- fixtures of the reference
-def test_oracle_vs_reference_run_config1(density7):
-    ref = np.load(golden("ref_config1.npz"))
+def test_oracle_vs_reference_run_h2o_sto3g(density7):
+    ref = np.load(golden("ref_h2o_sto3g.npz"))
     rb, o = _h2o()
     assert np.max(np.abs(o.eri_tensor() - ref["eri"])) < 1e-14
     for name, M in (("S", o.overlap()), ("T", o.kinetic()), ("V", o.nuclear())):
