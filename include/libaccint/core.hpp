@@ -64,7 +64,7 @@ public:
 };
 
 // ---- backends -------------------------------------------------------------
-// The device backend of this build is HIP on gfx950; there is no host
+// The device backend of this library is HIP on gfx950; there is no host
 // integral backend (the CPU value is kept for API compatibility and is never
 // selected for integral work).
 enum class BackendType { CPU, HIP };
@@ -91,7 +91,7 @@ enum class WorkUnitType { SingleShellPair, SingleShellQuartet, ShellSetPair, She
 
 // The reference's CPU/GPU routing rule (src/host/engine/dispatch_policy.cpp:12-90):
 // force hints first, CPU without a GPU, then per work type thresholds of the
-// config, moved by the Prefer hints.  In this build every integral runs on
+// config, moved by the Prefer hints.  In this library every integral runs on
 // the device whatever the decision: the rule is kept so that callers that
 // consult it see the reference's answer (HIP where the reference says CUDA).
 class DispatchPolicy {

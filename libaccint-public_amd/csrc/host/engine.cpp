@@ -46,7 +46,7 @@ Engine::~Engine() {
 
 void Engine::require_gpu(const char* what) const {
     if (!gpu_available_)
-        throw BackendError(std::string(what) + ": no HIP device is available (this build has no CPU integral path)");
+        throw BackendError(std::string(what) + ": no HIP device is available (this library has no CPU integral path)");
 }
 
 void Engine::require_two_electron(const Operator& op, const char* what) {
