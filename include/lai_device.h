@@ -291,7 +291,9 @@ int lai_dev_df_dims(const lai_df* df, int* n_orb, int* n_aux);
 #define LAI_DF_SYMMETRIC 4
 int lai_dev_df_jk(lai_df* df, const double* d_D, double* d_J, double* d_K, int flags, void* stream);
 /* as lai_dev_df_jk, synchronous, with each kernel timed by HIP events on
- * stream: ms[0] the gamma pass, ms[1] the J pass, ms[2] the X = B D GEMM,
+ * stream: ms[0] the fitted density gamma (a pass over B for J alone; with J
+ * and K together, the diagonal of X: gamma_P = sum_a (B_P D)_aa), ms[1] the
+ * J pass, ms[2] the X = B D GEMM,
  * ms[3] the K = X B^T GEMM (split over the summation index), ms[4] the sum of
  * its slices (entries of kernels not run are 0).  No reference counterpart
  * (measurement). */

@@ -278,6 +278,28 @@ __global__ __launch_bounds__(1024) void df_gamma_sum(const double* __restrict__ 
   }
 }
 
+// gamma[P] = sum_a X[a][P][a] with X[a][P][c] = sum_b B[a][P][b] D[b][c]:
+// each B_P is symmetric in (a, b), so sum_ab B_P,ab D_ab = sum_ab B_P,ab D_ba =
+// sum_a (B_P D)_aa for any D.  Reads the n n_aux diagonal entries of X
+// instead of all of B; workgroup shape and summation order as df_gamma_sum.
+__global__ __launch_bounds__(1024) void df_gamma_diag(const double* __restrict__ X, int n, int naux,
+                                                      double* __restrict__ gamma) {
+  __shared__ double red[16][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int P = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (P < naux)
+    for (int a = wave; a < n; a += 16) s += X[((long long)a * naux + P) * n + a];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && P < naux) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t += red[w][lane];
+    gamma[P] = t;
+  }
+}
+
 // J[a][b] = sum_P B[a][P][b] gamma[P]: workgroup (64 columns b, a), the four
 // wavefronts take every fourth P (lanes along b, two accumulators each) and
 // meet in LDS.
@@ -704,44 +726,28 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     if (ms) DF_HIP(hipEventRecord(d->ev[i], st));
     return LAI_OK;
   };
+  // ms[out] = time between events i0 and i1 (recorded on st, serial in the profiled runs)
+  auto take = [&](int i0, int i1, int out) -> int {
+    if (!ms) return LAI_OK;
+    float t;
+    DF_HIP(hipEventSynchronize(d->ev[i1]));
+    DF_HIP(hipEventElapsedTime(&t, d->ev[i0], d->ev[i1]));
+    ms[out] = t;
+    return LAI_OK;
+  };
   if (ms && !d->have_ev) {
     for (auto& e : d->ev) DF_HIP(hipEventCreate(&e));
     d->have_ev = true;
   }
   if (ms)
     for (int i = 0; i < 5; ++i) ms[i] = 0.0;
-  // J and K together (not profiled): the Coulomb passes (HBM-bound) on the
-  // side stream beside the exchange GEMMs (matrix cores), joined back into st
-  const bool overlap = !ms && (flags & LAI_DF_J) && (flags & LAI_DF_K) && d->side;
-  const hipStream_t sj = overlap ? d->side : st;
-  if (overlap) {
-    DF_HIP(hipEventRecord(d->fork, st));
-    DF_HIP(hipStreamWaitEvent(d->side, d->fork, 0));
-  }
-  if (flags & LAI_DF_J) {
-    int rc = mark(0);
-    if (rc) return rc;
-    hipLaunchKernelGGL(df_gamma_part, dim3(n, (naux + kGammaRows - 1) / kGammaRows), dim3(256), 0, sj, d->B, D, n,
-                       naux, d->gpart);
+  const bool both = (flags & LAI_DF_J) && (flags & LAI_DF_K);
+  auto coulomb = [&](hipStream_t s) -> int {
+    hipLaunchKernelGGL(df_coulomb, dim3((n + 63) / 64, n), dim3(256), 0, s, d->B, d->gamma, n, naux, J);
     DF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 63) / 64), dim3(1024), 0, sj, d->gpart, n, naux, d->gamma);
-    DF_HIP(hipGetLastError());
-    if ((rc = mark(1))) return rc;
-    hipLaunchKernelGGL(df_coulomb, dim3((n + 63) / 64, n), dim3(256), 0, sj, d->B, d->gamma, n, naux, J);
-    DF_HIP(hipGetLastError());
-    if ((rc = mark(2))) return rc;
-    if (ms) {
-      float t0, t1;
-      DF_HIP(hipEventSynchronize(d->ev[2]));
-      DF_HIP(hipEventElapsedTime(&t0, d->ev[0], d->ev[1]));
-      DF_HIP(hipEventElapsedTime(&t1, d->ev[1], d->ev[2]));
-      ms[0] = t0;
-      ms[1] = t1;
-    }
-  }
-  if (flags & LAI_DF_K) {
-    int rc = mark(0);
-    if (rc) return rc;
+    return LAI_OK;
+  };
+  auto x_gemm = [&]() -> int {
     GemmArgs g{};
     g.A = d->B;  // rows (a, P), n columns
     g.B = D;
@@ -753,8 +759,11 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     g.N = n;
     g.K = n;
     g.xcd = 1;
-    if ((rc = gemm(g, 1, st))) return rc;
-    if ((rc = mark(1))) return rc;
+    return gemm(g, 1, st);
+  };
+  auto k_product = [&]() -> int {  // K = X B^T over (P, b), then the slice sum
+    int rc = mark(3);
+    if (rc) return rc;
     GemmArgs h{};
     h.A = d->X;  // rows a, summation (P, b)
     h.B = d->B;  // given transposed: rows c, summation (P, b)
@@ -773,7 +782,7 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
     h.kchunk = h.skip_lower ? d->kchunk_sym : d->kchunk;
     const int nsl = h.skip_lower ? d->nsplit_sym : d->nsplit;
     if ((rc = gemm(h, nsl, st))) return rc;
-    if ((rc = mark(2))) return rc;
+    if ((rc = mark(4))) return rc;
     const long long nn = (long long)n * n;
     hipLaunchKernelGGL(sum_slices, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, d->part, nn,
                        nsl, nn, K);
@@ -782,21 +791,58 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
       hipLaunchKernelGGL(mirror_lower, dim3(unsigned(std::min(2048ll, (nn + 255) / 256))), dim3(256), 0, st, K, n);
       DF_HIP(hipGetLastError());
     }
-    if ((rc = mark(3))) return rc;
-    if (ms) {
-      float t0, t1, t2;
-      DF_HIP(hipEventSynchronize(d->ev[3]));
-      DF_HIP(hipEventElapsedTime(&t0, d->ev[0], d->ev[1]));
-      DF_HIP(hipEventElapsedTime(&t1, d->ev[1], d->ev[2]));
-      DF_HIP(hipEventElapsedTime(&t2, d->ev[2], d->ev[3]));
-      ms[2] = t0;
-      ms[3] = t1;
-      ms[4] = t2;
+    if ((rc = mark(5))) return rc;
+    if ((rc = take(3, 4, 3))) return rc;
+    return take(4, 5, 4);
+  };
+  int rc;
+  if (both) {
+    // X = B D first: J's fitted density is the diagonal of X, so B is read
+    // twice per build (X, Coulomb pass) plus once more with X by the K product.
+    // Not profiled: the Coulomb pass (HBM-bound) runs on the side stream
+    // beside the K product (matrix cores), joined back into st.
+    const bool overlap = !ms && d->side;
+    const hipStream_t sj = overlap ? d->side : st;
+    if ((rc = mark(0))) return rc;
+    if ((rc = x_gemm())) return rc;
+    if ((rc = mark(1))) return rc;
+    if ((rc = take(0, 1, 2))) return rc;
+    if (overlap) {
+      DF_HIP(hipEventRecord(d->fork, st));
+      DF_HIP(hipStreamWaitEvent(d->side, d->fork, 0));
     }
+    if ((rc = mark(0))) return rc;
+    hipLaunchKernelGGL(df_gamma_diag, dim3((naux + 63) / 64), dim3(1024), 0, sj, d->X, n, naux, d->gamma);
+    DF_HIP(hipGetLastError());
+    if ((rc = mark(1))) return rc;
+    if ((rc = coulomb(sj))) return rc;
+    if ((rc = mark(2))) return rc;
+    if ((rc = take(0, 1, 0)) || (rc = take(1, 2, 1))) return rc;
+    if ((rc = k_product())) return rc;
+    if (overlap) {
+      DF_HIP(hipEventRecord(d->join, d->side));
+      DF_HIP(hipStreamWaitEvent(st, d->join, 0));
+    }
+    return LAI_OK;
   }
-  if (overlap) {
-    DF_HIP(hipEventRecord(d->join, d->side));
-    DF_HIP(hipStreamWaitEvent(st, d->join, 0));
+  if (flags & LAI_DF_J) {  // J alone: the fitted density from one pass over B
+    if ((rc = mark(0))) return rc;
+    hipLaunchKernelGGL(df_gamma_part, dim3(n, (naux + kGammaRows - 1) / kGammaRows), dim3(256), 0, st, d->B, D, n,
+                       naux, d->gpart);
+    DF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(df_gamma_sum, dim3((naux + 63) / 64), dim3(1024), 0, st, d->gpart, n, naux, d->gamma);
+    DF_HIP(hipGetLastError());
+    if ((rc = mark(1))) return rc;
+    if ((rc = coulomb(st))) return rc;
+    if ((rc = mark(2))) return rc;
+    if ((rc = take(0, 1, 0)) || (rc = take(1, 2, 1))) return rc;
+  }
+  if (flags & LAI_DF_K) {
+    if ((rc = mark(0))) return rc;
+    if ((rc = x_gemm())) return rc;
+    if ((rc = mark(1))) return rc;
+    if ((rc = take(0, 1, 2))) return rc;
+    if ((rc = k_product())) return rc;
   }
   return LAI_OK;
 }
