@@ -91,6 +91,25 @@ def test_coulomb_fit_is_variational_and_close():
     assert np.allclose(J, J.T, atol=1e-13) and np.allclose(K, K.T, atol=1e-13)
 
 
+def test_fitted_density_is_the_diagonal_of_b_times_d():
+    """gamma_P = sum_ab B_P,ab D_ab equals sum_a (B_P D)_aa for a D that is not
+    symmetric, because each B_P is symmetric in (a, b): the device's J+K build
+    takes gamma from the diagonal of X = B D.  A B_P with an antisymmetric
+    part breaks the identity, so the check has teeth."""
+    orb = oracle.RawBasis("cc-pvdz", WATER)
+    aux = oracle.RawBasis("cc-pvdz-rifit", WATER)
+    V, T = oracle.df_tensors(orb, aux)
+    B = np.einsum("PQ,abQ->Pab", np.linalg.inv(np.linalg.cholesky(V)), T)
+    assert np.abs(B - B.transpose(0, 2, 1)).max() < 1e-12 * np.abs(B).max()
+    D = np.random.default_rng(3).standard_normal((orb.nbf, orb.nbf))
+    gamma = np.einsum("Pab,ab->P", B, D)
+    assert np.abs(np.einsum("Paa->P", B @ D) - gamma).max() < 1e-12 * np.abs(gamma).max()
+    A = np.random.default_rng(4).standard_normal(B.shape)
+    B2 = B + 1e-3 * (A - A.transpose(0, 2, 1))
+    gap = np.einsum("Paa->P", B2 @ D) - np.einsum("Pab,ab->P", B2, D)
+    assert np.abs(gap).max() > 1e-6 * np.abs(gamma).max()
+
+
 def test_metric_is_symmetric_positive_definite():
     aux = oracle.RawBasis("cc-pvdz-rifit", WATER)
     orb = oracle.RawBasis("sto-3g", WATER)
