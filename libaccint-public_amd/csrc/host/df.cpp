@@ -287,8 +287,13 @@ std::vector<Real> DFFockBuilder::compute() {
     initialize();
     std::fill(J_.begin(), J_.end(), 0.0);
     std::fill(K_.begin(), K_.end(), 0.0);
-    if (config_.compute_coulomb) compute_j_internal();
-    if (config_.compute_exchange) compute_k_internal();
+    if (config_.compute_coulomb && config_.compute_exchange && !uhf_mode_) {
+        // one device build of both: J's fitted density from the diagonal of X = B D
+        run(LAI_DF_J | LAI_DF_K | (symmetric(D_, n_orb_) ? LAI_DF_SYMMETRIC : 0), D_, &J_, &K_);
+    } else {
+        if (config_.compute_coulomb) compute_j_internal();
+        if (config_.compute_exchange) compute_k_internal();
+    }
     std::vector<Real> F(n_orb_ * n_orb_);
     for (Size i = 0; i < F.size(); ++i) F[i] = J_[i] - config_.exchange_fraction * K_[i];
     return F;

@@ -230,7 +230,10 @@ def test_symmetric_and_general_density_paths():
     for Dm in (Ds, Dg):
         J0, K0 = oracle.df_jk(orb, aux, Dm)
         f.set_density(np.ascontiguousarray(Dm))
-        assert rel(f.compute_coulomb(), J0) < TOL and rel(f.compute_exchange(), K0) < TOL
+        J1 = np.array(f.compute_coulomb())  # J alone: gamma from a pass over B
+        assert rel(J1, J0) < TOL and rel(f.compute_exchange(), K0) < TOL
+        f.compute()  # J and K together: gamma from the diagonal of X = B D
+        assert rel(f.coulomb_matrix(), J1) < 1e-13 and rel(f.exchange_matrix(), K0) < TOL
     K = f.exchange_matrix()
     assert not np.allclose(K, K.T)  # the general density's K is not symmetric
     f.set_density(Ds)
