@@ -39,6 +39,12 @@ extern "C" {
 /* J/K build flags */
 #define LAI_JK_FP64 0  /* every quartet in FP64 */
 #define LAI_JK_MIXED 1 /* quartets marked FP32 by the worklist run the FP32 recursion */
+/* flag bit, with either of the above: d_D is antisymmetric (D^T = -D); d_J is
+ * left alone and only the antisymmetric part (Kt - Kt^T)/2 of the exchange
+ * build is added to d_K.  K is linear in D, so a general D is the build of
+ * (D + D^T)/2 followed by this build of (D - D^T)/2: the exchange matrix of
+ * the reference's N^4 FockBuilder::accumulate for any D. */
+#define LAI_JK_ANTISYMMETRIC 2
 
 #define LAI_MAX_L 3 /* highest shell angular momentum of the energy kernels (f) */
 
@@ -64,6 +70,10 @@ int lai_dev_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream)
  * matrix_zero_kernel, src/device/cuda/consumers/fock_builder_gpu.cu:99-105) */
 int lai_dev_zero(void* d_ptr, size_t n_doubles, void* stream);
 int lai_dev_stream_sync(void* stream);
+/* Makes stream waiter wait for the work enqueued so far on stream producer
+ * (an event edge; no host wait).  The ERI-producer to consumer ordering of
+ * include/libaccint/engine/cuda_engine.hpp:889-894. */
+int lai_dev_stream_wait(void* waiter, void* producer);
 
 /* ---- basis ------------------------------------------------------------------
  * Uploads shells (normalised coefficients, src/shared/basis/shell.cpp:32-113)
@@ -116,6 +126,27 @@ int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* shells4, double* ou
  * src/device/cuda/consumers/fock_builder_gpu.cu:600-621). */
 int lai_dev_eri_quartets_device(lai_basis* b, int64_t n, const int* shells4, double* d_out, int64_t* offsets,
                                 void* stream);
+
+/* ---- Fock digestion of ERI blocks ---------------------------------------------
+ * One block per shell quartet (a b | c d): na*nb*nc*nd doubles at d_eri + off,
+ * row-major [a][b][c][d]; fa..fd are the shells' first basis functions. */
+typedef struct lai_eri_block {
+  int64_t off;
+  int32_t fa, fb, fc, fd;
+  int32_t na, nb, nc, nd; /* each in [1, 10] (s..f Cartesian) */
+} lai_eri_block;
+/* d_J[mu nu] += (mu nu|la si) D[la si] and d_K[mu la] += (mu nu|la si) D[nu si]
+ * over every element of n_blocks blocks (device arrays d_eri, d_blocks; each
+ * element counted once: the plain N^4 accumulate of
+ * src/shared/consumers/fock_builder.cpp:98-117).  Every index must lie in
+ * [0, nbf): a block that does not fit is skipped.  A wavefront per block sums
+ * its J and K rows in LDS and adds them with one atomic per element.
+ * Asynchronous on stream.  Replaces detail::launch_fock_accumulate_kernel and
+ * launch_fock_accumulate_batch_kernel
+ * (include/libaccint/consumers/fock_builder_gpu.hpp:178-214,
+ * src/device/cuda/consumers/fock_builder_gpu.cu:55-96,116-180). */
+int lai_dev_fock_accumulate(const double* d_eri, const lai_eri_block* d_blocks, int64_t n_blocks, const double* d_D,
+                            double* d_J, double* d_K, int nbf, void* stream);
 
 /* ---- Schwarz screening -------------------------------------------------------
  * Q_ij = sqrt(max_ab |(ab|ab)|) for every shell pair i <= j, packed at

@@ -10,7 +10,11 @@
 //    (src/shared/consumers/fock_builder.cpp:23-442).
 //  * GpuFockBuilder: owns device J, K and D; set_density copies D to the
 //    device on the builder's stream; getters copy back and synchronise
-//    (include/libaccint/consumers/fock_builder_gpu.hpp:46-159).
+//    (include/libaccint/consumers/fock_builder_gpu.hpp:46-159).  Host ERI
+//    blocks given to accumulate() are collected on the host and digested on
+//    the device in batches (one upload and one kernel per batch); device
+//    batches from Engine::compute_eri_batch_device are digested where they
+//    are (accumulate_device_eri_batch).
 #pragma once
 
 #include <libaccint/buffers.hpp>
@@ -130,30 +134,49 @@ public:
     GpuFockBuilder(const GpuFockBuilder&) = delete;
     GpuFockBuilder& operator=(const GpuFockBuilder&) = delete;
 
+    // Copies D to the device on the builder's stream (blocks accumulated so
+    // far are digested with the previous density first).
     void set_density(const Real* D, Size nbf);
-    // host ERI block -> device J/K (plain accumulate semantics)
+    // host ERI block -> device J/K (plain accumulate semantics,
+    // fock_builder_gpu.cu:475-530); the block is copied, so buf may be reused
+    // at once.  Blocks are digested in batches on the builder's stream, at the
+    // latest by the getters, synchronize(), d_J()/d_K() and set_density().
     void accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb,
                     int nc, int nd);
+    // device ERI batch -> device J/K, on the builder's stream ordered after the
+    // batch's stream (fock_builder_gpu.hpp:137, fock_builder_gpu.cu:600-621)
+    void accumulate_device_eri_batch(const DeviceEriBatch& batch, Size nbf);
     [[nodiscard]] std::vector<Real> get_coulomb_matrix() const;
     [[nodiscard]] std::vector<Real> get_exchange_matrix() const;
     [[nodiscard]] std::vector<Real> get_fock_matrix(std::span<const Real> H_core, Real exchange_fraction = 1.0) const;
+    // zeroes J and K (blocks not yet digested are dropped with them)
     void reset();
     void synchronize();
     [[nodiscard]] Size nbf() const noexcept { return nbf_; }
     [[nodiscard]] hipStream_t stream() const noexcept { return stream_; }
-    [[nodiscard]] double* d_J() noexcept { return J_.get(); }
-    [[nodiscard]] double* d_K() noexcept { return K_.get(); }
+    // device J and K with every accumulated block digested (enqueued on stream())
+    [[nodiscard]] double* d_J();
+    [[nodiscard]] double* d_K();
     [[nodiscard]] const double* d_D() const noexcept { return D_.get(); }
     [[nodiscard]] bool density_set() const noexcept { return density_set_; }
+    // host copy of the density given to set_density (nbf x nbf)
+    [[nodiscard]] const Real* h_D() const noexcept { return density_set_ ? D_host_.data() : nullptr; }
     void prepare_parallel(int = 0) {}
     void finalize_parallel() {}
 
 private:
+    // digests the staged host blocks on the device
+    void flush() const;
     Size nbf_;
     hipStream_t stream_;
     device::DeviceArray J_, K_, D_;
     std::vector<Real> D_host_;
     bool density_set_{false};
+    // host blocks waiting for the device digestion, and their device staging
+    mutable std::vector<Real> stage_eri_;
+    mutable std::vector<lai_eri_block> stage_blk_;
+    mutable device::DeviceArray d_stage_eri_;
+    mutable device::DeviceBytes d_stage_blk_;
 };
 
 }  // namespace consumers

@@ -124,6 +124,17 @@ public:
     void compute_eri_blocks(std::span<const std::array<int, 4>> quartets, std::vector<Real>& out,
                             std::vector<Index>& offsets);
     [[nodiscard]] std::vector<Real> compute_eri_tensor();
+    // The blocks of these shell quartets computed into device memory on
+    // stream, with their digestion descriptors; the reference's
+    // CudaEngine::compute_eri_batch_device_handle (cuda_engine.hpp:355-356,
+    // cuda_engine.cu:1233-1277) for an explicit quartet list.
+    using DeviceEriBatch = libaccint::DeviceEriBatch;
+    [[nodiscard]] DeviceEriBatch compute_eri_batch_device(std::span<const std::array<int, 4>> quartets,
+                                                          hipStream_t stream = nullptr);
+    // every shell quartet of a ShellSetQuartet, in its set_a x set_b x set_c x
+    // set_d loop order
+    [[nodiscard]] DeviceEriBatch compute_eri_batch_device_handle(const ShellSetQuartet& quartet,
+                                                                 hipStream_t stream = nullptr);
 
     // ---- screening ----------------------------------------------------------
     void precompute_schwarz_bounds();
@@ -202,25 +213,40 @@ public:
         compute_and_consume(op, consumer, options);
     }
 
-    // a ShellSetQuartet: every shell quartet of set_a x set_b x set_c x set_d,
-    // plain accumulate (cpu_engine.hpp:435-547)
+    // a ShellSetQuartet: every shell quartet of set_a x set_b x set_c x set_d
+    // (in that loop order), plain accumulate (cpu_engine.hpp:435-547).  A
+    // GpuFockBuilder takes the device route of cuda_engine.hpp:866-896: the
+    // ERIs stay in device memory and are digested on the builder's stream;
+    // other consumers receive host blocks.
     template <typename Consumer>
     void compute(const Operator& op, const ShellSetQuartet& quartet, Consumer& consumer,
                  BackendHint hint = BackendHint::Auto) {
         (void)hint;
         require_two_electron(op, "compute(ShellSetQuartet)");
-        std::vector<std::array<int, 4>> q;
-        const auto& A = quartet.bra_pair().shell_set_a();
-        const auto& B = quartet.bra_pair().shell_set_b();
-        const auto& C = quartet.ket_pair().shell_set_a();
-        const auto& D = quartet.ket_pair().shell_set_b();
-        for (const auto& a : A.shells())
-            for (const auto& b : B.shells())
-                for (const auto& c : C.shells())
-                    for (const auto& d : D.shells())
-                        q.push_back({int(a.shell_index()), int(b.shell_index()), int(c.shell_index()),
-                                     int(d.shell_index())});
-        consume_blocks(q, consumer, false);
+        const std::vector<std::array<int, 4>> q = shell_set_quartet_list(quartet);
+        if constexpr (std::is_same_v<Consumer, consumers::GpuFockBuilder>) {
+            if (consumer.nbf() != basis_->n_basis_functions())
+                throw InvalidArgumentException("compute(ShellSetQuartet): GpuFockBuilder nbf does not match the basis");
+            // batches of at most kDeviceBatchDoubles ERIs
+            Size s = 0;
+            while (s < q.size()) {
+                Size e = s, cnt = 0;
+                while (e < q.size()) {
+                    const Size c = block_size(q[e]);
+                    if (e > s && cnt + c > kDeviceBatchDoubles) break;
+                    cnt += c;
+                    ++e;
+                }
+                DeviceEriBatch batch =
+                    compute_eri_batch_device(std::span<const std::array<int, 4>>(q.data() + s, e - s), consumer.stream());
+                consumer.accumulate_device_eri_batch(batch, basis_->n_basis_functions());
+                // the batch's memory is released on return: its digestion has to finish first
+                consumer.synchronize();
+                s = e;
+            }
+        } else {
+            consume_blocks(q, consumer, false);
+        }
     }
 
     template <typename Consumer>
@@ -257,8 +283,16 @@ public:
     [[nodiscard]] lai_basis* device_basis();
 
 private:
+    static constexpr Size kDeviceBatchDoubles = Size(1) << 24;
     void require_gpu(const char* what) const;
     static void require_two_electron(const Operator& op, const char* what);
+    [[nodiscard]] std::vector<std::array<int, 4>> shell_set_quartet_list(const ShellSetQuartet& quartet) const;
+    [[nodiscard]] Size block_size(const std::array<int, 4>& q) const;
+    // J/K of a general density: the build of (D + D^T)/2 and, when D is not
+    // symmetric, the exchange build of (D - D^T)/2 (K is linear in D; J of an
+    // antisymmetric D is zero), i.e. the reference's N^4 accumulate for any D
+    void build_jk_general(const JKOptions& o, const Real* D_host, const Real* d_D, Real* d_J, Real* d_K,
+                          hipStream_t s);
     void jk_into(consumers::FockBuilder& fb, const JKOptions& o);
     void jk_into(consumers::GpuFockBuilder& fb, const JKOptions& o);
     void jk_into(consumers::MixedPrecisionFockBuilder& fb, const JKOptions& o);

@@ -1122,10 +1122,11 @@ static int ensure_side_streams(lai_basis* b) {
 // class adds into Jt and Kt with atomics only, so the order is free).
 static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
   const auto& Ls = launchers();
+  const bool mixed = (flags & LAI_JK_MIXED) != 0;
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
-  if (flags != LAI_JK_MIXED) a.sel.mixed = 0;
+  if (!mixed) a.sel.mixed = 0;
   const bool fan = ev == nullptr && b->fork != nullptr;
   hipStream_t streams[kSideStreams + 1] = {st};
   double load[kSideStreams + 1] = {};
@@ -1143,7 +1144,7 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
       int k = 0;
       for (int i = 1; i <= kSideStreams; ++i)
         if (load[i] < load[k]) k = i;
-      load[k] += (flags == LAI_JK_MIXED ? b->cls_f64[c] + b->cls_f32[c] : b->cls_f64[c]) + 1e6 * double(C.nchunks);
+      load[k] += (mixed ? b->cls_f64[c] + b->cls_f32[c] : b->cls_f64[c]) + 1e6 * double(C.nchunks);
       s = streams[k];
     }
     if (ev) (void)hipEventRecord(ev[2 * c], st);
@@ -1154,7 +1155,7 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
       t.alt = b->tune[c].alt;
       t.blocks_per_cu = b->tune[c].bpc;
       t.n_cu = b->ctx->n_cu;
-      (flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c])(a, s, t);
+      (mixed ? Ls.f32[c] : Ls.f64[c])(a, s, t);
     }
     if (ev) (void)hipEventRecord(ev[2 * c + 1], st);
   }
@@ -1183,11 +1184,13 @@ static EriArgs jk_args(lai_basis* b, const double* d_D) {
   return a;
 }
 
-static int symmetrize(lai_basis* b, double* d_J, double* d_K, hipStream_t st) {
+// J += (Jt + Jt^T)/2, K += (Kt + Kt^T)/2; with anti (antisymmetric D) only
+// K += (Kt - Kt^T)/2
+static int symmetrize(lai_basis* b, double* d_J, double* d_K, hipStream_t st, bool anti = false) {
   const size_t nn = size_t(b->max_nbf) * b->max_nbf;
   const unsigned blocks = unsigned(std::max<size_t>(1, std::min<size_t>((nn + 255) / 256, 2048)));
   hipLaunchKernelGGL(symmetrize_add_batch_kernel, dim3(blocks, unsigned(b->n_mol)), dim3(256), 0, st, b->Jt.p,
-                     b->Kt.p, d_J, d_K, b->dmoff.p, b->dmnbf.p);
+                     b->Kt.p, d_J, d_K, b->dmoff.p, b->dmnbf.p, anti ? 1 : 0);
   LAI_HIP(hipGetLastError());
   return LAI_OK;
 }
@@ -1195,7 +1198,7 @@ static int symmetrize(lai_basis* b, double* d_J, double* d_K, hipStream_t st) {
 extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
   if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: invalid arguments");
   if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk: worklist not built");
-  if (flags != LAI_JK_FP64 && flags != LAI_JK_MIXED)
+  if ((flags & ~(LAI_JK_MIXED | LAI_JK_ANTISYMMETRIC)) != 0)
     return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: unknown flags");
   hipStream_t st = hipStream_t(stream);
   const size_t nn = size_t(b->mat_size);
@@ -1209,7 +1212,7 @@ extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* 
   if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
   if ((rc = launch_jk_classes(b, jk_args(b, d_D), flags, st, nullptr)) != LAI_OK) return rc;
   LAI_HIP(hipGetLastError());
-  return symmetrize(b, d_J, d_K, st);
+  return symmetrize(b, d_J, d_K, st, (flags & LAI_JK_ANTISYMMETRIC) != 0);
 }
 
 extern "C" int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags,
@@ -1647,6 +1650,9 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
                                   double* ms_fp64, double* ms_fp32) {
   if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_profile: invalid arguments");
   if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_profile: worklist not built");
+  if ((flags & ~(LAI_JK_MIXED | LAI_JK_ANTISYMMETRIC)) != 0)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_profile: unknown flags");
+  const bool mixed = (flags & LAI_JK_MIXED) != 0;
   hipStream_t st = hipStream_t(stream);
   const size_t nn = size_t(b->mat_size);
   int rc;
@@ -1656,15 +1662,15 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
   for (auto& e : ev) LAI_HIP(hipEventCreate(&e));
   launch_jk_classes(b, jk_args(b, d_D), flags, st, ev.data());
   LAI_HIP(hipGetLastError());
-  if ((rc = symmetrize(b, d_J, d_K, st)) != LAI_OK) return rc;
+  if ((rc = symmetrize(b, d_J, d_K, st, (flags & LAI_JK_ANTISYMMETRIC) != 0)) != LAI_OK) return rc;
   LAI_HIP(hipStreamSynchronize(st));
   // one launch per class serves both precisions: its time is reported as
   // FP64 for LAI_JK_FP64 and as FP32 (mixed) for LAI_JK_MIXED
   for (int c = 0; c < kNumClasses; ++c) {
     float t = 0.f;
     LAI_HIP(hipEventElapsedTime(&t, ev[2 * c], ev[2 * c + 1]));
-    if (ms_fp64) ms_fp64[c] = flags == LAI_JK_MIXED ? 0.0 : t;
-    if (ms_fp32) ms_fp32[c] = flags == LAI_JK_MIXED ? t : 0.0;
+    if (ms_fp64) ms_fp64[c] = mixed ? 0.0 : t;
+    if (ms_fp32) ms_fp32[c] = mixed ? t : 0.0;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return LAI_OK;

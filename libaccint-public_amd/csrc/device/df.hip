@@ -811,19 +811,31 @@ int df_run(lai_df* d, const double* D, double* J, double* K, int flags, hipStrea
       DF_HIP(hipEventRecord(d->fork, st));
       DF_HIP(hipStreamWaitEvent(d->side, d->fork, 0));
     }
-    if ((rc = mark(0))) return rc;
-    hipLaunchKernelGGL(df_gamma_diag, dim3((naux + 63) / 64), dim3(1024), 0, sj, d->X, n, naux, d->gamma);
-    DF_HIP(hipGetLastError());
-    if ((rc = mark(1))) return rc;
-    if ((rc = coulomb(sj))) return rc;
-    if ((rc = mark(2))) return rc;
-    if ((rc = take(0, 1, 0)) || (rc = take(1, 2, 1))) return rc;
-    if ((rc = k_product())) return rc;
+    // from here on st is joined to the side stream on every return path, so
+    // that a failed launch never leaves work writing J or gamma unordered
+    // with the caller's stream
+    auto side_part = [&]() -> int {
+      int r;
+      if ((r = mark(0))) return r;
+      hipLaunchKernelGGL(df_gamma_diag, dim3((naux + 63) / 64), dim3(1024), 0, sj, d->X, n, naux, d->gamma);
+      DF_HIP(hipGetLastError());
+      if ((r = mark(1))) return r;
+      if ((r = coulomb(sj))) return r;
+      if ((r = mark(2))) return r;
+      if ((r = take(0, 1, 0)) || (r = take(1, 2, 1))) return r;
+      return k_product();
+    };
+    rc = side_part();
     if (overlap) {
-      DF_HIP(hipEventRecord(d->join, d->side));
-      DF_HIP(hipStreamWaitEvent(st, d->join, 0));
+      const hipError_t e1 = hipEventRecord(d->join, d->side);
+      const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(st, d->join, 0) : e1;
+      if (e2 != hipSuccess) {
+        (void)hipStreamSynchronize(d->side);  // no event edge: order by waiting on the host
+        if (rc == LAI_OK)
+          rc = set_error(LAI_ERR_BACKEND, std::string("lai_dev_df_jk: join: ") + hipGetErrorString(e2));
+      }
     }
-    return LAI_OK;
+    return rc;
   }
   if (flags & LAI_DF_J) {  // J alone: the fitted density from one pass over B
     if ((rc = mark(0))) return rc;

@@ -79,15 +79,20 @@ __global__ __launch_bounds__(256) void worklist_stats_kernel(WlStatsArgs a) {
 
 __global__ void symmetrize_add_batch_kernel(const double* __restrict__ Jt, const double* __restrict__ Kt,
                                             double* __restrict__ J, double* __restrict__ K,
-                                            const long long* __restrict__ moff, const int* __restrict__ mnbf) {
+                                            const long long* __restrict__ moff, const int* __restrict__ mnbf,
+                                            int anti) {
   const int m = blockIdx.y;
   const size_t n = size_t(mnbf[m]), nn = n * n;
   const size_t o = size_t(moff[m]);
   for (size_t x = size_t(blockIdx.x) * blockDim.x + threadIdx.x; x < nn; x += size_t(gridDim.x) * blockDim.x) {
     const size_t i = x / n, j = x - i * n;
     const size_t t = j * n + i;
-    J[o + x] += 0.5 * (Jt[o + x] + Jt[o + t]);
-    K[o + x] += 0.5 * (Kt[o + x] + Kt[o + t]);
+    if (anti) {
+      K[o + x] += 0.5 * (Kt[o + x] - Kt[o + t]);
+    } else {
+      J[o + x] += 0.5 * (Jt[o + x] + Jt[o + t]);
+      K[o + x] += 0.5 * (Kt[o + x] + Kt[o + t]);
+    }
   }
 }
 

@@ -28,9 +28,11 @@ struct WlStatsArgs {
 __global__ void worklist_stats_kernel(WlStatsArgs a);
 
 // J[m] += (Jt[m] + Jt[m]^T)/2, K[m] likewise, for every molecule block m
-// (blockIdx.y = molecule)
+// (blockIdx.y = molecule); with anti != 0 (an antisymmetric density) only
+// K[m] += (Kt[m] - Kt[m]^T)/2
 __global__ void symmetrize_add_batch_kernel(const double* __restrict__ Jt, const double* __restrict__ Kt,
                                             double* __restrict__ J, double* __restrict__ K,
-                                            const long long* __restrict__ moff, const int* __restrict__ mnbf);
+                                            const long long* __restrict__ moff, const int* __restrict__ mnbf,
+                                            int anti);
 
 }  // namespace lai

@@ -296,10 +296,16 @@ GpuFockBuilder::GpuFockBuilder(Size nbf, hipStream_t stream)
     device::check(lai_dev_stream_sync(stream_), "GpuFockBuilder");
 }
 
+namespace {
+// doubles of host ERI blocks collected before one device digestion
+constexpr Size kStageDoubles = Size(1) << 20;
+}  // namespace
+
 void GpuFockBuilder::set_density(const Real* D, Size nbf) {
     if (nbf != nbf_)
         throw InvalidArgumentException("GpuFockBuilder::set_density: nbf mismatch (expected " +
                                        std::to_string(nbf_) + ", got " + std::to_string(nbf) + ")");
+    flush();  // staged blocks belong to the previous density
     D_host_.assign(D, D + nbf * nbf);
     D_.upload(D, nbf * nbf, stream_);
     density_set_ = true;
@@ -308,30 +314,84 @@ void GpuFockBuilder::set_density(const Real* D, Size nbf) {
 void GpuFockBuilder::accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na,
                                 int nb, int nc, int nd) {
     if (!density_set_) throw InvalidStateException("GpuFockBuilder::accumulate called without density matrix");
-    const Size n = nbf_;
-    std::vector<Real> J(n * n), K(n * n);
-    J_.download(J.data(), n * n, stream_);
-    K_.download(K.data(), n * n, stream_);
+    for (int x : {na, nb, nc, nd})
+        if (x < 1 || x > 10) throw InvalidArgumentException("GpuFockBuilder::accumulate: block dimension out of range");
+    const Index n = static_cast<Index>(nbf_);
+    if (fa < 0 || fb < 0 || fc < 0 || fd < 0 || fa + na > n || fb + nb > n || fc + nc > n || fd + nd > n)
+        throw InvalidArgumentException("GpuFockBuilder::accumulate: function offsets outside the basis");
+    const Size cnt = static_cast<Size>(na) * nb * nc * nd;
+    if (buf.na() < na || buf.nb() < nb || buf.nc() < nc || buf.nd() < nd)
+        throw InvalidArgumentException("GpuFockBuilder::accumulate: buffer smaller than the block");
+    if (stage_eri_.size() + cnt > kStageDoubles) flush();
+    lai_eri_block blk{};
+    blk.off = static_cast<int64_t>(stage_eri_.size());
+    blk.fa = static_cast<int32_t>(fa);
+    blk.fb = static_cast<int32_t>(fb);
+    blk.fc = static_cast<int32_t>(fc);
+    blk.fd = static_cast<int32_t>(fd);
+    blk.na = na;
+    blk.nb = nb;
+    blk.nc = nc;
+    blk.nd = nd;
+    stage_blk_.push_back(blk);
     for (int a = 0; a < na; ++a)
         for (int b = 0; b < nb; ++b)
             for (int c = 0; c < nc; ++c)
-                for (int d = 0; d < nd; ++d) {
-                    const Size mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
-                    const Real g = buf(a, b, c, d);
-                    J[mu * n + nu] += g * D_host_[la * n + si];
-                    K[mu * n + la] += g * D_host_[nu * n + si];
-                }
-    J_.upload(J.data(), n * n, stream_);
-    K_.upload(K.data(), n * n, stream_);
+                for (int d = 0; d < nd; ++d) stage_eri_.push_back(buf(a, b, c, d));
+}
+
+void GpuFockBuilder::flush() const {
+    if (stage_blk_.empty()) return;
+    const Size nd = stage_eri_.size(), nbk = stage_blk_.size();
+    if (d_stage_eri_.size() < nd) d_stage_eri_ = device::DeviceArray(std::max(nd, kStageDoubles));
+    if (d_stage_blk_.bytes() < nbk * sizeof(lai_eri_block))
+        d_stage_blk_ = device::DeviceBytes(std::max(nbk, kStageDoubles / 16) * sizeof(lai_eri_block));
+    // synchronous copies on the builder's stream: the previous digestion has
+    // read the staging buffers before they are overwritten
+    d_stage_eri_.upload(stage_eri_.data(), nd, stream_);
+    d_stage_blk_.upload(stage_blk_.data(), nbk * sizeof(lai_eri_block), stream_);
+    auto* self = const_cast<GpuFockBuilder*>(this);
+    device::check(lai_dev_fock_accumulate(d_stage_eri_.get(), static_cast<const lai_eri_block*>(d_stage_blk_.get()),
+                                          static_cast<int64_t>(nbk), D_.get(), self->J_.get(), self->K_.get(),
+                                          static_cast<int>(nbf_), stream_),
+                  "GpuFockBuilder::accumulate");
+    stage_eri_.clear();
+    stage_blk_.clear();
+}
+
+void GpuFockBuilder::accumulate_device_eri_batch(const DeviceEriBatch& batch, Size nbf) {
+    if (nbf != nbf_)
+        throw InvalidArgumentException("GpuFockBuilder::accumulate_device_eri_batch: nbf mismatch (expected " +
+                                       std::to_string(nbf_) + ", got " + std::to_string(nbf) + ")");
+    if (!density_set_)
+        throw InvalidStateException("GpuFockBuilder::accumulate_device_eri_batch called without density matrix");
+    if (batch.empty()) return;
+    flush();
+    device::stream_wait(stream_, batch.stream());
+    device::check(lai_dev_fock_accumulate(batch.data(), batch.blocks(), static_cast<int64_t>(batch.n_blocks()),
+                                          D_.get(), J_.get(), K_.get(), static_cast<int>(nbf_), stream_),
+                  "GpuFockBuilder::accumulate_device_eri_batch");
+}
+
+double* GpuFockBuilder::d_J() {
+    flush();
+    return J_.get();
+}
+
+double* GpuFockBuilder::d_K() {
+    flush();
+    return K_.get();
 }
 
 std::vector<Real> GpuFockBuilder::get_coulomb_matrix() const {
+    flush();
     std::vector<Real> J(nbf_ * nbf_);
     J_.download(J.data(), J.size(), stream_);
     return J;
 }
 
 std::vector<Real> GpuFockBuilder::get_exchange_matrix() const {
+    flush();
     std::vector<Real> K(nbf_ * nbf_);
     K_.download(K.data(), K.size(), stream_);
     return K;
@@ -347,10 +407,15 @@ std::vector<Real> GpuFockBuilder::get_fock_matrix(std::span<const Real> H, Real 
 }
 
 void GpuFockBuilder::reset() {
+    stage_eri_.clear();
+    stage_blk_.clear();
     J_.zero(stream_);
     K_.zero(stream_);
 }
 
-void GpuFockBuilder::synchronize() { device::check(lai_dev_stream_sync(stream_), "GpuFockBuilder::synchronize"); }
+void GpuFockBuilder::synchronize() {
+    flush();
+    device::check(lai_dev_stream_sync(stream_), "GpuFockBuilder::synchronize");
+}
 
 }  // namespace libaccint::consumers

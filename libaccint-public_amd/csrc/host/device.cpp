@@ -71,4 +71,32 @@ void DeviceArray::download(double* host, Size n, hipStream_t s) const {
 
 void DeviceArray::zero(hipStream_t s) { check(lai_dev_zero(p_, n_, s), "DeviceArray::zero"); }
 
+DeviceBytes::DeviceBytes(Size n) : n_(n) {
+    check(lai_dev_malloc(context(), n, &p_), "DeviceBytes");
+}
+
+DeviceBytes::~DeviceBytes() {
+    if (p_) (void)lai_dev_free(nullptr, p_);
+}
+
+DeviceBytes& DeviceBytes::operator=(DeviceBytes&& o) noexcept {
+    if (this != &o) {
+        if (p_) (void)lai_dev_free(nullptr, p_);
+        p_ = o.p_;
+        n_ = o.n_;
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    return *this;
+}
+
+void DeviceBytes::upload(const void* host, Size n, hipStream_t s) {
+    if (n > n_) throw InvalidArgumentException("DeviceBytes::upload: size exceeds allocation");
+    check(lai_dev_memcpy_h2d(p_, host, n, s), "DeviceBytes::upload");
+}
+
+void stream_wait(hipStream_t waiter, hipStream_t producer) {
+    check(lai_dev_stream_wait(waiter, producer), "stream_wait");
+}
+
 }  // namespace libaccint::device

@@ -217,6 +217,77 @@ std::vector<Real> Engine::compute_eri_tensor() {
     return T;
 }
 
+std::vector<std::array<int, 4>> Engine::shell_set_quartet_list(const ShellSetQuartet& quartet) const {
+    std::vector<std::array<int, 4>> q;
+    const auto& A = quartet.bra_pair().shell_set_a();
+    const auto& B = quartet.bra_pair().shell_set_b();
+    const auto& C = quartet.ket_pair().shell_set_a();
+    const auto& D = quartet.ket_pair().shell_set_b();
+    q.reserve(A.n_shells() * B.n_shells() * C.n_shells() * D.n_shells());
+    for (const auto& a : A.shells())
+        for (const auto& b : B.shells())
+            for (const auto& c : C.shells())
+                for (const auto& d : D.shells())
+                    q.push_back({int(a.shell_index()), int(b.shell_index()), int(c.shell_index()),
+                                 int(d.shell_index())});
+    return q;
+}
+
+Size Engine::block_size(const std::array<int, 4>& q) const {
+    Size n = 1;
+    for (int k = 0; k < 4; ++k) {
+        if (q[k] < 0 || static_cast<Size>(q[k]) >= basis_->n_shells())
+            throw InvalidArgumentException("shell index out of range");
+        n *= static_cast<Size>(basis_->shell(q[k]).n_functions());
+    }
+    return n;
+}
+
+DeviceEriBatch Engine::compute_eri_batch_device(std::span<const std::array<int, 4>> q, hipStream_t stream) {
+    lai_basis* b = device_basis();
+    DeviceEriBatch batch;
+    batch.stream_ = stream;
+    if (q.empty()) return batch;
+    std::vector<lai_eri_block> blk(q.size());
+    std::vector<int64_t> offs(q.size() + 1);
+    Size total = 0;
+    for (Size t = 0; t < q.size(); ++t) {
+        const Size c = block_size(q[t]);
+        const Shell& A = basis_->shell(q[t][0]);
+        const Shell& B = basis_->shell(q[t][1]);
+        const Shell& C = basis_->shell(q[t][2]);
+        const Shell& D = basis_->shell(q[t][3]);
+        auto& e = blk[t];
+        e.off = static_cast<int64_t>(total);
+        e.fa = static_cast<int32_t>(A.function_index());
+        e.fb = static_cast<int32_t>(B.function_index());
+        e.fc = static_cast<int32_t>(C.function_index());
+        e.fd = static_cast<int32_t>(D.function_index());
+        e.na = A.n_functions();
+        e.nb = B.n_functions();
+        e.nc = C.n_functions();
+        e.nd = D.n_functions();
+        total += c;
+    }
+    batch.eri_ = device::DeviceArray(total);
+    static_assert(sizeof(std::array<int, 4>) == 4 * sizeof(int));
+    device::check(lai_dev_eri_quartets_device(b, static_cast<int64_t>(q.size()), q.data()->data(), batch.eri_.get(),
+                                              offs.data(), stream),
+                  "compute_eri_batch_device");
+    for (Size t = 0; t < q.size(); ++t)
+        if (offs[t] != blk[t].off) throw BackendError("compute_eri_batch_device: block layout mismatch");
+    batch.blk_ = device::DeviceBytes(q.size() * sizeof(lai_eri_block));
+    batch.blk_.upload(blk.data(), q.size() * sizeof(lai_eri_block), stream);
+    batch.count_ = total;
+    batch.n_blocks_ = q.size();
+    return batch;
+}
+
+DeviceEriBatch Engine::compute_eri_batch_device_handle(const ShellSetQuartet& quartet, hipStream_t stream) {
+    const auto q = shell_set_quartet_list(quartet);
+    return compute_eri_batch_device(q, stream);
+}
+
 // ---- screening ----------------------------------------------------------------------
 void Engine::precompute_schwarz_bounds() {
     if (schwarz_.is_initialized()) return;
@@ -344,13 +415,53 @@ void Engine::capture_jk_graph(const JKOptions& o, const Real* d_D, Real* d_J, Re
 
 void Engine::launch_jk_graph(hipStream_t s) { device::check(lai_dev_jk_graph_launch(device_basis(), s), "launch_jk_graph"); }
 
+namespace {
+bool is_symmetric(const Real* D, Size n) {
+    for (Size i = 0; i < n; ++i)
+        for (Size j = i + 1; j < n; ++j)
+            if (D[i * n + j] != D[j * n + i]) return false;
+    return true;
+}
+}  // namespace
+
+void Engine::build_jk_general(const JKOptions& o, const Real* D_host, const Real* d_D, Real* d_J, Real* d_K,
+                              hipStream_t s) {
+    const Size n = basis_->n_basis_functions();
+    if (is_symmetric(D_host, n)) {
+        build_jk(o, d_D, d_J, d_K, s, D_host);
+        return;
+    }
+    // the worklist of the full D: its shell-pair max|D| bounds both parts
+    std::vector<Real> Ds(n * n), Da(n * n);
+    for (Size i = 0; i < n; ++i)
+        for (Size j = 0; j < n; ++j) {
+            Ds[i * n + j] = 0.5 * (D_host[i * n + j] + D_host[j * n + i]);
+            Da[i * n + j] = 0.5 * (D_host[i * n + j] - D_host[j * n + i]);
+        }
+    device::DeviceArray dDs(n * n), dDa(n * n);
+    dDs.upload(Ds.data(), n * n, s);
+    dDa.upload(Da.data(), n * n, s);
+    build_jk(o, dDs.get(), d_J, d_K, s, D_host);
+    device::check(lai_dev_jk(device_basis(), dDa.get(), d_J, d_K,
+                             (o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64) | LAI_JK_ANTISYMMETRIC, s),
+                  "build_jk");
+    device::check(lai_dev_stream_sync(s), "build_jk");  // dDs and dDa are released on return
+}
+
 void Engine::jk_into(consumers::FockBuilder& fb, const JKOptions& o) {
     if (fb.nbf() != basis_->n_basis_functions())
         throw InvalidArgumentException("compute_and_consume: FockBuilder nbf does not match the basis");
     if (!fb.density()) throw InvalidStateException("FockBuilder::accumulate called without density matrix");
     const Size n = fb.nbf();
     std::vector<Real> J(n * n), K(n * n);
-    build_jk_host(o, fb.density(), J.data(), K.data());
+    device::DeviceArray dD(n * n), dJ(n * n), dK(n * n);
+    dD.upload(fb.density(), n * n);
+    dJ.zero();
+    dK.zero();
+    build_jk_general(o, fb.density(), dD.get(), dJ.get(), dK.get(), nullptr);
+    device::check(lai_dev_stream_sync(nullptr), "compute_and_consume");
+    dJ.download(J.data(), n * n);
+    dK.download(K.data(), n * n);
     fb.add_jk(J.data(), K.data());
 }
 
@@ -360,7 +471,14 @@ void Engine::jk_into(consumers::MixedPrecisionFockBuilder& fb, const JKOptions& 
     if (!fb.density()) throw InvalidStateException("MixedPrecisionFockBuilder: density matrix not set");
     const Size n = fb.nbf();
     std::vector<Real> J(n * n), K(n * n);
-    build_jk_host(o, fb.density(), J.data(), K.data());
+    device::DeviceArray dD(n * n), dJ(n * n), dK(n * n);
+    dD.upload(fb.density(), n * n);
+    dJ.zero();
+    dK.zero();
+    build_jk_general(o, fb.density(), dD.get(), dJ.get(), dK.get(), nullptr);
+    device::check(lai_dev_stream_sync(nullptr), "compute_and_consume");
+    dJ.download(J.data(), n * n);
+    dK.download(K.data(), n * n);
     const Size nq = static_cast<Size>(wl_info_.n_quartets), n32 = static_cast<Size>(wl_info_.n_fp32);
     fb.add_jk(J.data(), K.data(), n32, nq - n32);
 }
@@ -369,14 +487,7 @@ void Engine::jk_into(consumers::GpuFockBuilder& fb, const JKOptions& o) {
     if (fb.nbf() != basis_->n_basis_functions())
         throw InvalidArgumentException("compute_and_consume: GpuFockBuilder nbf does not match the basis");
     if (!fb.density_set()) throw InvalidStateException("GpuFockBuilder: density matrix not set");
-    std::vector<Real> Dh;
-    const Real* D_host = nullptr;
-    if (o.density_weighted || o.mixed_precision) {
-        Dh.resize(fb.nbf() * fb.nbf());
-        device::check(lai_dev_memcpy_d2h(Dh.data(), fb.d_D(), Dh.size() * sizeof(Real), fb.stream()), "jk");
-        D_host = Dh.data();
-    }
-    build_jk(o, fb.d_D(), fb.d_J(), fb.d_K(), fb.stream(), D_host);
+    build_jk_general(o, fb.h_D(), fb.d_D(), fb.d_J(), fb.d_K(), fb.stream());
 }
 
 // ---- gradient -----------------------------------------------------------------------

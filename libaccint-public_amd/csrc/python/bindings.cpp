@@ -9,6 +9,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <sstream>
 
 namespace py = pybind11;
@@ -58,6 +59,51 @@ Point3D to_point(py::handle h) {
     auto v = h.cast<std::vector<Real>>();
     if (v.size() != 3) throw std::invalid_argument("position must have 3 components");
     return {v[0], v[1], v[2]};
+}
+
+// A Python callable as a two-electron consumer: accumulate(block, fa, fb, fc,
+// fd) with block the (na, nb, nc, nd) integrals (the TwoElectronConsumer
+// concept; Engine routes it like any consumer that is not a Fock builder).
+struct PyConsumer {
+    py::function fn;
+    void accumulate(const TwoElectronBuffer<0>& buf, Index fa, Index fb, Index fc, Index fd, int na, int nb, int nc,
+                    int nd) {
+        py::array_t<Real> a({na, nb, nc, nd});
+        Real* p = a.mutable_data();
+        for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j)
+                for (int k = 0; k < nc; ++k)
+                    for (int l = 0; l < nd; ++l) *p++ = buf(i, j, k, l);
+        fn(a, fa, fb, fc, fd);
+    }
+};
+
+// The ShellSetQuartets named by `which` for f: None = the basis's canonical
+// list (BasisSet::shell_set_quartets), else a sequence of (a, b, c, d)
+// indices into BasisSet::shell_sets() (bra pair (a, b), ket pair (c, d)).
+template <typename F>
+void with_shell_set_quartets(Engine& e, py::object which, F&& f) {
+    const BasisSet& bs = e.basis();
+    if (which.is_none()) {
+        const auto& q = bs.shell_set_quartets();
+        f(std::span<const ShellSetQuartet>(q.data(), q.size()));
+        return;
+    }
+    const auto sets = bs.shell_sets();
+    auto idx = which.cast<std::vector<std::array<int, 4>>>();
+    std::vector<ShellSetPair> pairs;
+    pairs.reserve(2 * idx.size());  // stable addresses for the quartets
+    std::vector<ShellSetQuartet> quartets;
+    quartets.reserve(idx.size());
+    for (const auto& t : idx) {
+        for (int k = 0; k < 4; ++k)
+            if (t[k] < 0 || static_cast<Size>(t[k]) >= sets.size())
+                throw InvalidArgumentException("shell set index out of range");
+        pairs.emplace_back(*sets[t[0]], *sets[t[1]]);
+        pairs.emplace_back(*sets[t[2]], *sets[t[3]]);
+        quartets.emplace_back(pairs[pairs.size() - 2], pairs.back());
+    }
+    f(std::span<const ShellSetQuartet>(quartets.data(), quartets.size()));
 }
 
 py::dict info_dict(const WorklistInfo& w) {
@@ -151,6 +197,18 @@ PYBIND11_MODULE(_core, m) {
         .def("shell_sets", &BasisSet::shell_sets, py::return_value_policy::reference_internal)
         .def("n_shell_set_pairs", [](const BasisSet& b) { return b.shell_set_pairs().size(); })
         .def("n_shell_set_quartets", [](const BasisSet& b) { return b.shell_set_quartets().size(); })
+        .def("shell_set_quartet_indices",
+             [](const BasisSet& b) {  // (a, b, c, d) indices into shell_sets() of shell_set_quartets()
+                 const auto sets = b.shell_sets();
+                 auto index = [&](const ShellSet* p) {
+                     return int(std::find(sets.begin(), sets.end(), p) - sets.begin());
+                 };
+                 std::vector<std::array<int, 4>> out;
+                 for (const auto& q : b.shell_set_quartets())
+                     out.push_back({index(&q.bra_pair().shell_set_a()), index(&q.bra_pair().shell_set_b()),
+                                    index(&q.ket_pair().shell_set_a()), index(&q.ket_pair().shell_set_b())});
+                 return out;
+             })
         .def("__repr__", [](const BasisSet& b) {
             return "BasisSet(n_shells=" + std::to_string(b.n_shells()) +
                    ", n_basis_functions=" + std::to_string(b.n_basis_functions()) + ")";
@@ -351,6 +409,13 @@ PYBIND11_MODULE(_core, m) {
         .def("set_fp32_threshold", &MixedPrecisionFockBuilder::set_fp32_threshold, py::arg("tau"))
         .def("fp32_threshold", &MixedPrecisionFockBuilder::fp32_threshold);
 
+    py::class_<DeviceEriBatch>(m, "DeviceEriBatch")
+        .def("size", &DeviceEriBatch::size)
+        .def("n_blocks", &DeviceEriBatch::n_blocks)
+        .def("empty", &DeviceEriBatch::empty)
+        .def("data_ptr", [](const DeviceEriBatch& b) { return reinterpret_cast<std::uintptr_t>(b.data()); })
+        .def("blocks_ptr", [](const DeviceEriBatch& b) { return reinterpret_cast<std::uintptr_t>(b.blocks()); });
+
     py::class_<GpuFockBuilder>(m, "GpuFockBuilder")
         .def(py::init([](Size nbf, std::uintptr_t stream) {
                  return std::make_unique<GpuFockBuilder>(nbf, reinterpret_cast<hipStream_t>(stream));
@@ -369,6 +434,14 @@ PYBIND11_MODULE(_core, m) {
                  return to_matrix(f.get_fock_matrix(std::span<const Real>(H.data(), H.size()), x), f.nbf());
              },
              py::arg("H_core"), py::arg("exchange_fraction") = 1.0)
+        .def("accumulate",
+             [](GpuFockBuilder& f, const TwoElectronBuffer<0>& b, Index fa, Index fb, Index fc, Index fd) {
+                 f.accumulate(b, fa, fb, fc, fd, b.na(), b.nb(), b.nc(), b.nd());
+             },
+             py::arg("buffer"), py::arg("fa"), py::arg("fb"), py::arg("fc"), py::arg("fd"))
+        .def("accumulate_device_eri_batch",
+             [](GpuFockBuilder& f, const DeviceEriBatch& batch) { f.accumulate_device_eri_batch(batch, f.nbf()); },
+             py::arg("batch"))
         .def("reset", &GpuFockBuilder::reset)
         .def("synchronize", &GpuFockBuilder::synchronize)
         .def("nbf", &GpuFockBuilder::nbf)
@@ -533,6 +606,51 @@ PYBIND11_MODULE(_core, m) {
                  e.compute_and_consume_screened_parallel(op, fb, o, n);
              },
              py::arg("op"), py::arg("consumer"), py::arg("options"), py::arg("n_threads") = 0)
+        .def("compute_eri_batch_device",
+             [](Engine& e, py::array_t<int, py::array::c_style | py::array::forcecast> q, std::uintptr_t stream) {
+                 if (q.ndim() != 2 || q.shape(1) != 4) throw std::invalid_argument("quartets must be (n, 4)");
+                 std::vector<std::array<int, 4>> v(q.shape(0));
+                 std::memcpy(v.data(), q.data(), sizeof(int) * 4 * v.size());
+                 return e.compute_eri_batch_device(v, reinterpret_cast<hipStream_t>(stream));
+             },
+             py::arg("quartets"), py::arg("stream") = std::uintptr_t(0))
+        .def("compute_shell_set_quartets",
+             [](Engine& e, const Operator& op, FockBuilder& fb, py::object which) {
+                 with_shell_set_quartets(e, which, [&](std::span<const ShellSetQuartet> q) { e.compute(op, q, fb); });
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("quartets") = py::none())
+        .def("compute_shell_set_quartets",
+             [](Engine& e, const Operator& op, GpuFockBuilder& fb, py::object which) {
+                 with_shell_set_quartets(e, which, [&](std::span<const ShellSetQuartet> q) { e.compute(op, q, fb); });
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("quartets") = py::none())
+        .def("compute_shell_set_quartets",
+             [](Engine& e, const Operator& op, py::function fn, py::object which) {
+                 PyConsumer c{fn};
+                 with_shell_set_quartets(e, which, [&](std::span<const ShellSetQuartet> q) {
+                     e.compute_and_consume(op, q, c);
+                 });
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("quartets") = py::none())
+        .def("compute_shell_set_quartet",
+             [](Engine& e, const Operator& op, GpuFockBuilder& fb, int a, int b, int c, int d) {
+                 with_shell_set_quartets(e, py::make_tuple(py::make_tuple(a, b, c, d)),
+                                         [&](std::span<const ShellSetQuartet> q) { e.compute(op, q[0], fb); });
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("a"), py::arg("b"), py::arg("c"), py::arg("d"))
+        .def("compute_shell_set_quartet",
+             [](Engine& e, const Operator& op, FockBuilder& fb, int a, int b, int c, int d) {
+                 with_shell_set_quartets(e, py::make_tuple(py::make_tuple(a, b, c, d)),
+                                         [&](std::span<const ShellSetQuartet> q) { e.compute(op, q[0], fb); });
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("a"), py::arg("b"), py::arg("c"), py::arg("d"))
+        .def("compute_and_consume",
+             [](Engine& e, const Operator& op, py::function fn, py::object options) {
+                 PyConsumer c{fn};
+                 if (options.is_none()) e.compute_and_consume(op, c);
+                 else e.compute_and_consume(op, c, options.cast<ScreeningOptions>());
+             },
+             py::arg("op"), py::arg("consumer"), py::arg("options") = py::none())
         .def("compute_quartet",
              [](Engine& e, const Operator& op, int i, int j, int k, int l) {
                  TwoElectronBuffer<0> buf;

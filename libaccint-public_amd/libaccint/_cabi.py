@@ -13,12 +13,20 @@ _LIB_PATH = os.path.join(_HERE, "liblai_device.so")
 LAI_OK = 0
 LAI_JK_FP64 = 0
 LAI_JK_MIXED = 1
+LAI_JK_ANTISYMMETRIC = 2
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
+
+
+class EriBlock(ctypes.Structure):
+    """lai_eri_block: one ERI block of a Fock digestion."""
+    _fields_ = [("off", ctypes.c_int64), ("fa", ctypes.c_int32), ("fb", ctypes.c_int32), ("fc", ctypes.c_int32),
+                ("fd", ctypes.c_int32), ("na", ctypes.c_int32), ("nb", ctypes.c_int32), ("nc", ctypes.c_int32),
+                ("nd", ctypes.c_int32)]
 
 
 class ClassTune(ctypes.Structure):
@@ -40,6 +48,7 @@ _SIGNATURES = {
     "lai_dev_memcpy_d2h": ([_vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "lai_dev_zero": ([_vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "lai_dev_stream_sync": ([_vp], ctypes.c_int),
+    "lai_dev_stream_wait": ([_vp, _vp], ctypes.c_int),
     "lai_dev_upload_basis": ([_vp, ctypes.c_int, _ip, _dp, _ip, _dp, _dp, ctypes.POINTER(_vp)], ctypes.c_int),
     "lai_dev_basis_free": ([_vp], ctypes.c_int),
     "lai_dev_basis_nbf": ([_vp], ctypes.c_int),
@@ -47,6 +56,7 @@ _SIGNATURES = {
     "lai_dev_batch_layout": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_eri_quartets": ([_vp, _i64, _ip, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
     "lai_dev_eri_quartets_device": ([_vp, _i64, _ip, _vp, ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "lai_dev_fock_accumulate": ([_vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_set_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_build_worklist": ([_vp, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,

@@ -301,6 +301,20 @@ class Oracle:
         self._f("jk_canonical")(self.h, _d(D), qp, ctypes.c_double(thr), _d(J), _d(K), int(n_threads))
         return J, K
 
+    def jk_canonical_touching(self, D, Q, thr, shell_mask, n_threads=0):
+        """(J, K) of the canonical screened quartets with at least one shell
+        s where shell_mask[s] is true (compiled reference only)."""
+        if not self.use_ref:
+            raise RuntimeError("jk_canonical_touching needs the compiled reference")
+        n = self.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        m = np.ascontiguousarray(np.asarray(shell_mask, dtype=np.int32))
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        self._l.ref_jk_canonical_touching(self.h, _d(D), _d(np.ascontiguousarray(Q)), ctypes.c_double(thr),
+                                          _i(m), _d(J), _d(K), int(n_threads))
+        return J, K
+
     def jk_canonical_sample(self, D, Q, thr, stride, offset=0, n_threads=0):
         """(J, K, n_quartets, n_eri) over every stride-th bra pair."""
         n = self.nbf

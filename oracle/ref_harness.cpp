@@ -183,6 +183,58 @@ void ref_jk_canonical(void* h, const double* D, const double* Q_packed, double t
     }
 }
 
+// The canonical screened loop restricted to the quartets that have at least
+// one shell s with mask[s] != 0: the part of J and K (and so of the energy
+// 1/2 tr(DJ) - 1/4 tr(DK)) that depends on those shells' centres.  Finite
+// differences of the energy in one atom's coordinates need only this part.
+void ref_jk_canonical_touching(void* h, const double* D, const double* Q_packed, double thr, const int* mask,
+                               double* J, double* K, int n_threads) {
+    const auto& b = *static_cast<RefBasis*>(h)->basis;
+    const int n = static_cast<int>(b.n_shells());
+    const Size nbf = b.n_basis_functions();
+    std::memset(J, 0, sizeof(double) * nbf * nbf);
+    std::memset(K, 0, sizeof(double) * nbf * nbf);
+    auto Q = [&](int i, int j) {
+        if (i > j) std::swap(i, j);
+        return Q_packed[static_cast<Size>(j) * (j + 1) / 2 + i];
+    };
+    std::vector<std::pair<int, int>> pairs;
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j) pairs.emplace_back(i, j);
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads)
+    {
+        consumers::FockBuilder fb(nbf);
+        fb.set_density(D, nbf);
+        TwoElectronBuffer<0> buf;
+#pragma omp for schedule(dynamic, 1)
+        for (Size p = 0; p < pairs.size(); ++p) {
+            const int i = pairs[p].first, j = pairs[p].second;
+            const bool bra_in = mask[i] || mask[j];
+            for (int k = i; k < n; ++k)
+                for (int l = (k == i) ? j : k; l < n; ++l) {
+                    if (!bra_in && !mask[k] && !mask[l]) continue;
+                    if (!(Q(i, j) * Q(k, l) >= thr)) continue;
+                    const auto& A = b.shell(i);
+                    const auto& B = b.shell(j);
+                    const auto& C = b.shell(k);
+                    const auto& E = b.shell(l);
+                    kernels::compute_eri(A, B, C, E, buf);
+                    fb.accumulate_symmetric(buf, A.function_index(), B.function_index(), C.function_index(),
+                                            E.function_index(), A.n_functions(), B.n_functions(), C.n_functions(),
+                                            E.n_functions(), i == j, k == l, i == k && j == l);
+                }
+        }
+        auto Jl = fb.get_coulomb_matrix();
+        auto Kl = fb.get_exchange_matrix();
+#pragma omp critical
+        for (Size x = 0; x < nbf * nbf; ++x) {
+            J[x] += Jl[x];
+            K[x] += Kl[x];
+        }
+    }
+}
+
 // The same canonical screened loop over a sample of bra pairs (every
 // stride-th pair of the reference order, starting at offset); counts the
 // quartets and ERIs computed so a bounded sample can be timed.

@@ -23,6 +23,21 @@ Writes into tests/golden/:
   ref_16water_ccpvdz.npz   (--sixteen-waters) the compiled reference for the seeded
                            16-water cc-pVDZ cluster: Schwarz table, screened
                            list count/checksum at 1e-12, J, K
+  ref_c40h82_def2svp.npz   (--alkane) the compiled reference for C40H82 in
+                           def2-SVP (1010 Cartesian functions), seeded D with C
+                           1010 x 161: screened list count/checksum at 1e-12,
+                           tr(DJ), tr(DK), the diagonals and row sums of J and
+                           K, 2000 seeded (mu, nu) samples of J and K
+  ref_batch_ccpvtz.npz     (--batch) the same summaries for a seeded subset of
+                           the 64 H/C/N/O molecules in cc-pVTZ (one density per
+                           molecule, the batched workload's seeds)
+  ref_water32_fd.npz       (--water32) the 32-water def2-SVP cluster: tr(DJ),
+                           tr(DK) and summaries at the reference geometry, and
+                           for 10 seeded (atom, axis) coordinates the part of
+                           E = 1/2 tr(DJ) - 1/4 tr(DK) that depends on that
+                           atom (the canonical quartets touching its shells)
+                           at +-1e-4 bohr; the central difference of that part
+                           is the central difference of E
 """
 import argparse
 import json
@@ -138,13 +153,114 @@ def ref_16water_ccpvdz():
                         atoms_xyz=np.array([p for _, p in atoms]), atoms_Z=np.array([z for z, _ in atoms]))
 
 
+def _summary(D, J, K, rng, n_samples):
+    """tr(DJ), tr(DK), diagonals, row sums and seeded element samples."""
+    n = D.shape[0]
+    idx = rng.integers(0, n, size=(n_samples, 2)).astype(np.int32)
+    return dict(trDJ=float(np.sum(D * J)), trDK=float(np.sum(D * K)), J_diag=np.diag(J).copy(),
+                K_diag=np.diag(K).copy(), J_rowsum=J.sum(axis=1), K_rowsum=K.sum(axis=1), sample_idx=idx,
+                J_sample=J[idx[:, 0], idx[:, 1]], K_sample=K[idx[:, 0], idx[:, 1]],
+                J_absmax=float(np.max(np.abs(J))), K_absmax=float(np.max(np.abs(K))))
+
+
+def ref_c40h82_def2svp(n_threads=0):
+    import math
+    import oracle
+    from libaccint import systems
+    atoms = systems.alkane(40)
+    rb = oracle.RawBasis("def2-svp", atoms)
+    r = oracle.Oracle(rb, use_ref=True)
+    D = systems.density(rb.nbf, 161, 11, 1.0 / math.sqrt(rb.nbf))
+    Q = r.schwarz()
+    n, cs = oracle.Oracle(rb).screened_list(Q, 1e-12)
+    J, K = r.jk_canonical(D, Q, 1e-12, n_threads=n_threads)
+    np.savez_compressed(os.path.join(HERE, "ref_c40h82_def2svp.npz"), n_quartets=n, checksum=np.uint64(cs),
+                        **_summary(D, J, K, np.random.default_rng(31), 2000))
+
+
+# molecules of the 64-molecule batch (numpy default_rng(4), 64 draws of
+# systems.random_molecule) in the fixture: 3, 3, 7, 7, 8 and 9 atoms
+BATCH_SUBSET = (19, 49, 45, 9, 21, 40)
+
+
+def ref_batch_ccpvtz(n_threads=0):
+    import math
+    import oracle
+    from libaccint import systems
+    rng = np.random.default_rng(4)
+    mols = [systems.random_molecule(rng) for _ in range(64)]
+    out = {"molecules": np.array(BATCH_SUBSET, dtype=np.int32)}
+    for k in BATCH_SUBSET:
+        rb = oracle.RawBasis("cc-pvtz", mols[k])
+        r = oracle.Oracle(rb, use_ref=True)
+        D = systems.density(rb.nbf, max(2, rb.nbf // 5), 200 + k, 1.0 / math.sqrt(rb.nbf))
+        Q = r.schwarz()
+        n, cs = oracle.Oracle(rb).screened_list(Q, 1e-12)
+        J, K = r.jk_canonical(D, Q, 1e-12, n_threads=n_threads)
+        out["m%d_n_quartets" % k] = n
+        out["m%d_checksum" % k] = np.uint64(cs)
+        for key, v in _summary(D, J, K, np.random.default_rng(100 + k), 400).items():
+            out["m%d_%s" % (k, key)] = v
+        print("batch molecule %d: nbf %d, %d quartets" % (k, rb.nbf, n), flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_batch_ccpvtz.npz"), **out)
+
+
+FD_STEP = 1e-4
+
+
+def ref_water32_fd(n_threads=0):
+    import math
+    import oracle
+    from libaccint import systems
+    atoms = systems.water_cluster(32, 18.0, seed=4)
+    rb = oracle.RawBasis("def2-svp", atoms)
+    D = systems.density(rb.nbf, 160, 6, 1.0 / math.sqrt(768.0))
+    r = oracle.Oracle(rb, use_ref=True)
+    Q = r.schwarz()
+    n, cs = oracle.Oracle(rb).screened_list(Q, 1e-12)
+    J, K = r.jk_canonical(D, Q, 1e-12, n_threads=n_threads)
+    out = dict(n_quartets=n, checksum=np.uint64(cs), **_summary(D, J, K, np.random.default_rng(41), 1000))
+    rng = np.random.default_rng(7)
+    coord_atoms = rng.choice(len(atoms), size=10, replace=False).astype(np.int32)
+    coord_axes = rng.integers(0, 3, size=10).astype(np.int32)
+    e_plus, e_minus = [], []
+    for a, x in zip(coord_atoms, coord_axes):
+        mask = (rb.atom == a).astype(np.int32)
+        parts = []
+        for sgn in (1.0, -1.0):
+            moved = [(z, tuple(p)) for z, p in atoms]
+            p = list(moved[a][1])
+            p[x] += sgn * FD_STEP
+            moved[a] = (moved[a][0], tuple(p))
+            rm = oracle.Oracle(oracle.RawBasis("def2-svp", moved), use_ref=True)
+            Qm = rm.schwarz()
+            Jm, Km = rm.jk_canonical_touching(D, Qm, 1e-12, mask, n_threads=n_threads)
+            parts.append(0.5 * float(np.sum(D * Jm)) - 0.25 * float(np.sum(D * Km)))
+        e_plus.append(parts[0])
+        e_minus.append(parts[1])
+        print("fd atom %d axis %d: %.10e" % (a, x, (parts[0] - parts[1]) / (2 * FD_STEP)), flush=True)
+    out.update(fd_atoms=coord_atoms, fd_axes=coord_axes, fd_step=FD_STEP, e_plus=np.array(e_plus),
+               e_minus=np.array(e_minus))
+    np.savez_compressed(os.path.join(HERE, "ref_water32_fd.npz"), **out)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--sixteen-waters", action="store_true")
+    ap.add_argument("--alkane", action="store_true")
+    ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--water32", action="store_true")
+    ap.add_argument("--threads", type=int, default=0)
     a = ap.parse_args()
+    if a.alkane:
+        ref_c40h82_def2svp(a.threads)
+    if a.batch:
+        ref_batch_ccpvtz(a.threads)
+    if a.water32:
+        ref_water32_fd(a.threads)
     if a.sixteen_waters:
         ref_16water_ccpvdz()
-    else:
+    elif not (a.alkane or a.batch or a.water32):
         literals()
         data_files()
         ref_h2o_sto3g()

@@ -149,6 +149,54 @@ def test_sixteen_waters_against_four_centre_build():
     assert rel(J, J.T) < 1e-12 and rel(K, K.T) < 1e-12
     f.compute()
     assert np.array_equal(J, f.coulomb_matrix()) and np.array_equal(K, f.exchange_matrix())
+    # J of the combined build (gamma from the diagonal of X = B D) equals J of
+    # the separate Coulomb build (gamma from a pass over B)
+    assert rel(J, np.array(f.compute_coulomb())) < 1e-13
+    # the combined symmetric build (skipped lower tiles, mirrored) equals the
+    # exchange product over all tiles: lai_dev_df_jk with K alone, no symmetry
+    import ctypes
+    import torch
+    from libaccint import _cabi
+    L = _cabi.lib()
+    h = ctypes.c_void_p(f.device_handle())
+    dD = torch.from_numpy(np.ascontiguousarray(D)).cuda()
+    dJ = torch.zeros_like(dD)
+    dK1 = torch.zeros_like(dD)
+    dK2 = torch.zeros_like(dD)
+    _cabi.check(L.lai_dev_df_jk(h, ctypes.c_void_p(dD.data_ptr()), ctypes.c_void_p(dJ.data_ptr()),
+                                ctypes.c_void_p(dK1.data_ptr()), 1 | 2 | 4, None))
+    _cabi.check(L.lai_dev_df_jk(h, ctypes.c_void_p(dD.data_ptr()), None, ctypes.c_void_p(dK2.data_ptr()), 2, None))
+    torch.cuda.synchronize()
+    K1, K2 = dK1.cpu().numpy(), dK2.cpu().numpy()
+    assert rel(K1, K2) < 1e-12 and rel(K2, K) < 1e-12
+    assert rel(dJ.cpu().numpy(), J) < 1e-13
+
+
+@pytest.fixture(scope="module")
+def six_waters_tensors():
+    atoms = systems.water_cluster(6, 11.0, seed=12)
+    orb = oracle.RawBasis("cc-pvdz", atoms)
+    aux = oracle.RawBasis("cc-pvdz-rifit", atoms)
+    return orb, aux, oracle.df_tensors(orb, aux)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_six_waters_multi_tile_matches_oracle(symmetric, six_waters_tensors):
+    """150 orbital functions: several 64 x 80 tiles of the exchange product
+    (XCD tile order, skipped lower tiles with a symmetric D) and several
+    slices of its summation, against the oracle."""
+    print("df six waters, symmetric D" if symmetric else "df six waters, general D", flush=True)
+    orb, aux, (V, T) = six_waters_tensors
+    assert orb.nbf == 150
+    D = density(orb.nbf, 25, 13)
+    if not symmetric:
+        D = D + 0.05 * np.random.default_rng(14).standard_normal(D.shape) / orb.nbf
+    J0, K0 = oracle.df_jk_from_tensors(V, T, D)
+    f = builder(orb, aux)
+    f.set_density(np.ascontiguousarray(D))
+    f.compute()
+    assert rel(f.coulomb_matrix(), J0) < TOL and rel(f.exchange_matrix(), K0) < TOL
+    assert rel(np.array(f.compute_coulomb()), J0) < TOL
 
 
 def _numpy_rhf(H, S, jk, n_occ, e_nuc, iters=200):
