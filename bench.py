@@ -51,7 +51,69 @@ def traffic_of(cls):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(threads):
+    """The CPU baseline, run in a child process so that OMP_PROC_BIND and
+    OMP_NUM_THREADS reach the OpenMP runtime before it starts (the parent has
+    torch's runtime loaded).  Times, with the compiled reference CPU sources
+    (oracle/_ref) where they are built, else their C restatement:
+      * the canonical screened loop (thr 1e-12) on every 12th bra shell pair
+        of the 16-water cc-pVDZ workload;
+      * the reference's default path, the unscreened N^4 loop of
+        CpuEngine::compute_and_consume_parallel, on four first-shell slices of
+        the same workload (projected to all 192 slices), and in full on
+        H2O/STO-3G.
+    Prints one JSON object."""
+    import oracle
+    from libaccint import systems
+    use_ref = oracle.ref_lib() is not None
+    atoms = systems.water_cluster(16, 14.0, seed=2)
+    rb = oracle.RawBasis("cc-pvdz", atoms)
+    o = oracle.Oracle(rb, use_ref=use_ref)
+    D = systems.density(rb.nbf, 80, 5, 1.0 / math.sqrt(384.0))
+    Q = o.schwarz(n_threads=threads)
+    stride, offset = 12, 7
+    t = time.perf_counter()
+    _, _, nq, ne = o.jk_canonical_sample(D, Q, 1e-12, stride, offset, n_threads=threads)
+    dt = time.perf_counter() - t
+    # N^4 default path: a seeded sample of first-shell pairs (i, j), each
+    # with all n_shells^2 (k, l)
+    ns = rb.n_shells
+    ij = np.random.default_rng(17).integers(0, ns, size=(24, 2)).astype(np.int32)
+    dt4 = None
+    if use_ref:
+        t = time.perf_counter()
+        o.jk_full_pairs(D, ij, n_threads=threads)
+        dt4 = time.perf_counter() - t
+    h2o = oracle.Oracle(oracle.RawBasis("sto-3g", systems.h2o_sto3g_geometry()), use_ref=use_ref)
+    rng = np.random.default_rng(1)
+    C = rng.standard_normal((7, 5)) / np.sqrt(7)
+    reps = 200
+    t = time.perf_counter()
+    for _ in range(reps):
+        h2o.jk_full(C @ C.T, n_threads=threads)
+    dt1 = (time.perf_counter() - t) / reps
+    out = {"kind": "reference" if use_ref else "port", "threads": threads, "cpu_model": cpu_model(),
+           "canonical_sample": {"stride": stride, "offset": offset, "quartets": nq, "eris": ne, "seconds": dt},
+           "n4_sample": {"first_shell_pairs": len(ij), "shell_quartets": len(ij) * ns ** 2, "seconds": dt4,
+                         "projected_full_seconds": None if dt4 is None else dt4 * ns * ns / len(ij)},
+           "h2o_sto3g_n4_seconds": dt1}
+    print(json.dumps(out), flush=True)
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--cpu-leg":
+        cpu_leg(int(sys.argv[2]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -429,22 +491,30 @@ def main():
     # ----------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
-        import oracle
-        rb = oracle.RawBasis("cc-pvdz", atoms)
-        use_ref = oracle.ref_lib() is not None
-        o = oracle.Oracle(rb, use_ref=use_ref)
-        Q = engine.schwarz_bounds().packed()
-        cores = min(16, os.cpu_count() or 1)
-        stride, offset = 24, 7
-        t = time.perf_counter()
-        _, _, nq, ne = o.jk_canonical_sample(D, Q, 1e-12, stride, offset, n_threads=cores)
-        dt = time.perf_counter() - t
-        rate = ne / dt
-        cpu = {"value": rate, "unit": "ERIs/s", "cores": cores, "kind": "reference" if use_ref else "port",
+        import subprocess
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", str(threads)], env=env,
+                           stdout=subprocess.PIPE, text=True, check=True)
+        c = json.loads(r.stdout.strip().splitlines()[-1])
+        cs = c["canonical_sample"]
+        rate = cs["eris"] / cs["seconds"]
+        cpu = {"value": rate, "unit": "ERIs/s", "cores": threads, "kind": c["kind"], "cpu_model": c["cpu_model"],
+               "cpus_visible": os.cpu_count(), "omp": "OMP_PROC_BIND=close OMP_PLACES=cores",
                "sample": "canonical screened J/K (thr 1e-12) over every %dth bra shell pair (offset %d) of the same "
-                         "16-water cc-pVDZ workload: %d quartets, %d ERIs in %.1f s" % (stride, offset, nq, ne, dt),
-               "projected_full_jk_seconds": info["n_eri"] / rate}
-        log("CPU baseline: %.3e ERIs/s on %d cores (%.1f s sample)" % (rate, cores, dt))
+                         "16-water cc-pVDZ workload: %d quartets, %d ERIs in %.1f s" %
+                         (cs["stride"], cs["offset"], cs["quartets"], cs["eris"], cs["seconds"]),
+               "projected_full_jk_seconds": info["n_eri"] / rate,
+               "default_n4_path": {"what": "unscreened N^4 loop of CpuEngine::compute_and_consume_parallel "
+                                           "(ThreadLocal) over %d seeded first-shell pairs (i, j) of 192^2, each "
+                                           "with all (k, l): %d shell quartets" %
+                                           (c["n4_sample"]["first_shell_pairs"], c["n4_sample"]["shell_quartets"]),
+                                   "seconds_for_sample": c["n4_sample"]["seconds"],
+                                   "projected_full_jk_seconds": c["n4_sample"]["projected_full_seconds"]},
+               "h2o_sto3g_default_n4_jk_seconds": c["h2o_sto3g_n4_seconds"]}
+        log("CPU baseline (%s, %d threads): %.3e ERIs/s canonical screened; N^4 default path projected %s s; "
+            "H2O/STO-3G N^4 %.3f ms" % (c["cpu_model"], threads, rate, c["n4_sample"]["projected_full_seconds"],
+                                        c["h2o_sto3g_n4_seconds"] * 1e3))
 
     eris_per_s = info["n_eri"] / (ms64 * 1e-3)
     out = {

@@ -328,6 +328,19 @@ class Oracle:
                                        ctypes.byref(nq), ctypes.byref(ne))
         return J, K, nq.value, ne.value
 
+    def jk_full_pairs(self, D, ij, n_threads=0):
+        """(J, K) of the N^4 loop restricted to the first-shell pairs ij
+        (array (m, 2)) with every (k, l) (compiled reference only)."""
+        if not self.use_ref:
+            raise RuntimeError("jk_full_pairs needs the compiled reference")
+        n = self.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        ij = np.ascontiguousarray(np.asarray(ij, dtype=np.int32).reshape(-1, 2))
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        self._l.ref_jk_full_pairs(self.h, _d(D), _d(J), _d(K), int(n_threads), _i(ij), len(ij))
+        return J, K
+
     def jk_full(self, D, n_threads=0, i_begin=0, i_end=0):
         n = self.nbf
         D = np.ascontiguousarray(D, dtype=np.float64)

@@ -334,6 +334,45 @@ void ref_jk_full(void* h, const double* D, double* J, double* K, int n_threads, 
     std::memcpy(K, Km.data(), sizeof(double) * nbf * nbf);
 }
 
+// The same default path restricted to the given (i, j) first-shell pairs
+// (n_ij pairs, ij[2t], ij[2t+1]) with every (k, l): a bounded sample of the
+// N^4 loop for timing; J and K hold that part.
+void ref_jk_full_pairs(void* h, const double* D, double* J, double* K, int n_threads, const int* ij, int n_ij) {
+    const auto& b = *static_cast<RefBasis*>(h)->basis;
+    const long long n = static_cast<long long>(b.n_shells());
+    const Size nbf = b.n_basis_functions();
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+    consumers::FockBuilder fb(nbf);
+    fb.set_density(D, nbf);
+    fb.set_threading_strategy(consumers::FockThreadingStrategy::ThreadLocal);
+    fb.prepare_parallel(n_threads);
+    const long long total = static_cast<long long>(n_ij) * n * n;
+#pragma omp parallel num_threads(n_threads)
+    {
+        TwoElectronBuffer<0> buf;
+#pragma omp for schedule(dynamic, 8)
+        for (long long t = 0; t < total; ++t) {
+            const int l = static_cast<int>(t % n);
+            const int k = static_cast<int>((t / n) % n);
+            const long long p = t / (n * n);
+            const int i = ij[2 * p], j = ij[2 * p + 1];
+            const auto& A = b.shell(i);
+            const auto& B = b.shell(j);
+            const auto& C = b.shell(k);
+            const auto& E = b.shell(l);
+            kernels::compute_eri(A, B, C, E, buf);
+            fb.accumulate(buf, A.function_index(), B.function_index(), C.function_index(),
+                          E.function_index(), A.n_functions(), B.n_functions(), C.n_functions(),
+                          E.n_functions());
+        }
+    }
+    fb.finalize_parallel();
+    auto Jm = fb.get_coulomb_matrix();
+    auto Km = fb.get_exchange_matrix();
+    std::memcpy(J, Jm.data(), sizeof(double) * nbf * nbf);
+    std::memcpy(K, Km.data(), sizeof(double) * nbf * nbf);
+}
+
 // Density-fitted J and K of the reference DFFockBuilder
 // (src/shared/consumers/df_fock_builder.cpp) for the orbital basis orb and the
 // fitting basis aux (a basis handle of this file; its shells become the
