@@ -126,6 +126,21 @@ int lai_dev_eri_quartets(lai_basis* b, int64_t n, const int* shells4, double* ou
  * src/device/cuda/consumers/fock_builder_gpu.cu:600-621). */
 int lai_dev_eri_quartets_device(lai_basis* b, int64_t n, const int* shells4, double* d_out, int64_t* offsets,
                                 void* stream);
+/* The same blocks as lai_dev_eri_quartets_device, bit for bit, with no device
+ * allocation: the per-class quartet descriptors are staged in a static 2 MiB
+ * device array of the library (one copy per GPU, set up by lai_dev_init and
+ * held until the process ends) instead of hipMalloc'd buffers.  At most
+ * LAI_NOALLOC_MAX_QUARTETS quartets per call (LAI_ERR_INVALID_ARGUMENT
+ * beyond).  d_out must hold offsets[n] doubles (d_out NULL fills offsets
+ * only).  Every call on a GPU shares that GPU's staging array: the calls of
+ * one process are serialised by the library, and calls that overlap on one
+ * GPU from elsewhere would race.  Synchronises the stream.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+#define LAI_NOALLOC_MAX_QUARTETS 65536
+int lai_dev_eri_quartets_device_noalloc(lai_basis* b, int64_t n, const int* shells4, double* d_out,
+                                        int64_t* offsets, void* stream);
 
 /* ---- Fock digestion of ERI blocks ---------------------------------------------
  * One block per shell quartet (a b | c d): na*nb*nc*nd doubles at d_eri + off,

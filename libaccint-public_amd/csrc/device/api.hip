@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -156,6 +157,35 @@ void parallel_for(int n, F&& f, int grain = 16) {
       }
     });
   for (auto& x : th) x.join();
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// static staging array of lai_dev_eri_quartets_device_noalloc
+// ---------------------------------------------------------------------------
+// 2 MiB of device memory in the library's code object, one copy per GPU, held
+// until the process ends.  Its name is only a host-side stand-in: host code
+// uses the device address that hipGetSymbolAddress returns for the current
+// device (kept per device in g_scratch).
+constexpr size_t kScratchBytes = size_t(2) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mtx;             // guards g_scratch and serialises the staging calls
+void* g_scratch[kMaxDevices] = {};    // device address of scratch, per device
+
+// Looks up the scratch address of `device` (current) once.
+int init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(LAI_ERR_INVALID_ARGUMENT, "device index above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mtx);
+  if (g_scratch[device]) return LAI_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(LAI_ERR_BACKEND, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = p;
+  return LAI_OK;
 }
 }  // namespace
 
@@ -309,6 +339,8 @@ extern "C" int lai_dev_init(int device, lai_ctx** out) {
   if (device < 0 || device >= count)
     return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_init: device index out of range");
   LAI_HIP(hipSetDevice(device));
+  const int src = init_scratch(device);  // the device is current
+  if (src != LAI_OK) return src;
   auto* ctx = new lai_ctx;
   ctx->device = device;
   hipDeviceProp_t prop;
@@ -760,6 +792,71 @@ extern "C" int lai_dev_eri_quartets_device(lai_basis* b, int64_t n, const int* s
   rc = run_eri_batch(b, eb, d_out, hipStream_t(stream));
   if (rc != LAI_OK) return rc;
   LAI_HIP(hipStreamSynchronize(hipStream_t(stream)));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_eri_quartets_device_noalloc(lai_basis* b, int64_t n, const int* s4, double* d_out,
+                                                   int64_t* offsets, void* stream) {
+  if (!b || n < 0 || !offsets || (n > 0 && !s4))
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets_device_noalloc: invalid arguments");
+  if (n > LAI_NOALLOC_MAX_QUARTETS)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets_device_noalloc: " + std::to_string(n) +
+                                              " quartets, more than the staging limit of " +
+                                              std::to_string(LAI_NOALLOC_MAX_QUARTETS));
+  EriBatch eb;
+  int rc = classify_batch(b, n, s4, offsets, eb, "lai_dev_eri_quartets_device_noalloc");
+  if (rc != LAI_OK || !d_out || n == 0) return rc;  // no output: offsets only
+  const int dev = b->ctx->device;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev])
+    return fail(LAI_ERR_INVALID_STATE, "lai_dev_eri_quartets_device_noalloc: staging array not set up");
+  unsigned char* base = static_cast<unsigned char*>(g_scratch[dev]);
+  // layout: per class with work, its Quartet, perm and obase arrays, each
+  // 256-byte aligned; at most 20 bytes per quartet plus 3 * 256 per class
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t need = 0;
+  for (int c = 0; c < kNumClasses; ++c) {
+    const size_t nq = eb.q[c].size();
+    if (nq) need += up(nq * sizeof(Quartet)) + up(nq * sizeof(int)) + up(nq * sizeof(long long));
+  }
+  if (need > kScratchBytes)
+    return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_eri_quartets_device_noalloc: staging array too small");
+  hipStream_t st = hipStream_t(stream);
+  std::lock_guard<std::mutex> lk(g_scratch_mtx);  // one staging call at a time in this process
+  const auto& Ls = launchers();
+  size_t off = 0;
+  for (int c = 0; c < kNumClasses; ++c) {
+    const int nq = int(eb.q[c].size());
+    if (nq == 0) continue;
+    auto* dq = reinterpret_cast<Quartet*>(base + off);
+    off += up(size_t(nq) * sizeof(Quartet));
+    auto* dp = reinterpret_cast<int*>(base + off);
+    off += up(size_t(nq) * sizeof(int));
+    auto* dob = reinterpret_cast<long long*>(base + off);
+    off += up(size_t(nq) * sizeof(long long));
+    LAI_HIP(hipMemcpyAsync(dq, eb.q[c].data(), sizeof(Quartet) * nq, hipMemcpyHostToDevice, st));
+    LAI_HIP(hipMemcpyAsync(dp, eb.perm[c].data(), sizeof(int) * nq, hipMemcpyHostToDevice, st));
+    LAI_HIP(hipMemcpyAsync(dob, eb.obase[c].data(), sizeof(long long) * nq, hipMemcpyHostToDevice, st));
+    EriArgs a{};
+    a.q = dq;
+    a.nq = nq;
+    a.pairs = b->pairs.p;
+    a.pp = b->pp.p;
+    a.rys = b->ctx->rys.p;
+    a.ppb = b->ppb.p;
+    a.ppih = b->ppih.p;
+    a.pcut32 = b->pcut32;
+    a.mode = 1;
+    a.out = d_out;
+    a.perm = dp;
+    a.obase = dob;
+    a.nbf = b->max_nbf;
+    LaunchTune t;
+    t.n_cu = b->ctx->n_cu;
+    Ls.f64[c](a, st, t);
+    LAI_HIP(hipGetLastError());
+  }
+  // the host descriptor arrays are released on return: the copies must be done
+  LAI_HIP(hipStreamSynchronize(st));
   return LAI_OK;
 }
 

@@ -14,6 +14,7 @@ LAI_OK = 0
 LAI_JK_FP64 = 0
 LAI_JK_MIXED = 1
 LAI_JK_ANTISYMMETRIC = 2
+LAI_NOALLOC_MAX_QUARTETS = 65536
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -56,6 +57,7 @@ _SIGNATURES = {
     "lai_dev_batch_layout": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "lai_dev_eri_quartets": ([_vp, _i64, _ip, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
     "lai_dev_eri_quartets_device": ([_vp, _i64, _ip, _vp, ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "lai_dev_eri_quartets_device_noalloc": ([_vp, _i64, _ip, _vp, ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "lai_dev_fock_accumulate": ([_vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_schwarz": ([_vp, _dp], ctypes.c_int),
     "lai_dev_set_schwarz": ([_vp, _dp], ctypes.c_int),
