@@ -416,10 +416,25 @@ def main():
         for _ in range(reps):
             g = geng.compute_jk_gradient(gD, 1.0)
         gs = (time.perf_counter() - t) / reps
+        g0 = np.zeros(55)
+        g1 = np.zeros(55)
+        _cabi.check(L.lai_dev_worklist_grad_flops(ctypes.c_void_p(geng.device_basis_handle()),
+                                                  g0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                  g1.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        F0, F1 = float(g0.sum()), float(g1.sum())
         grad = {"gradient_seconds": gs, "gradient_workload": "32-water def2-SVP (800 Cartesian bf), Schwarz 1e-12, "
                 "%d quartets, dE/dR for E = 1/2 tr(DJ) - 1/4 tr(DK)" % ginfo["n_quartets"],
-                "gradient_translational_residual": float(np.max(np.abs(g.sum(axis=0))))}
-        log("gradient (32-water def2-SVP): %.3f s" % gs)
+                "gradient_translational_residual": float(np.max(np.abs(g.sum(axis=0)))),
+                "gradient_roofline": {
+                    "flops_derivative_classes": F0, "tflops_derivative_classes": F0 / gs / 1e12,
+                    "frac_derivative_classes": F0 / gs / 1e12 / FP64_PEAK_TFLOPS,
+                    "flops_raised_tables": F1, "tflops_raised_tables": F1 / gs / 1e12,
+                    "frac_raised_tables": F1 / gs / 1e12 / FP64_PEAK_TFLOPS, "peak": FP64_PEAK_TFLOPS,
+                    "units": "executed primitive pairs; derivative classes: per centre (3 of 4) F_prim of the "
+                             "class with that l raised and with it lowered, + 9 N_cart per root; raised tables: "
+                             "30 + n_r (50 + 3 F_dir(La+1, Lb+1, Lc+1, Ld) + 12 N_cart)"}}
+        log("gradient (32-water def2-SVP): %.3f s, %.2f TFLOP/s (derivative classes) / %.2f (raised tables)" %
+            (gs, F0 / gs / 1e12, F1 / gs / 1e12))
 
     # ------------------------------------------------------ density fitting
     df = {}

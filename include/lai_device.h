@@ -208,6 +208,16 @@ int lai_dev_worklist_stats(const lai_basis* b, int64_t* counts, double* flops_fp
  * the primitive pairs above the primitive screening bound */
 int lai_dev_worklist_class_flops(const lai_basis* b, double* flops_fp64, double* flops_fp32);
 
+/* Per-class work of the gradient over the current worklist (55 entries each;
+ * executed primitive pairs, the same selection as lai_dev_jk_grad).  model0:
+ * per primitive quartet the derivative classes of three centres (each
+ * centre's l raised and, when l > 0, lowered by one), F_prim each, plus
+ * 9 N_cart contraction flops per root; model1: one table per (root, axis)
+ * with the three centres raised together, 30 + n_r (50 + 3 F_dir(La+1, Lb+1,
+ * Lc+1, Ld) + 12 N_cart).  Measurement aid; no reference counterpart (the
+ * reference has no derivative integrals). */
+int lai_dev_worklist_grad_flops(const lai_basis* b, double* model0, double* model1);
+
 /* ---- J/K build ---------------------------------------------------------------
  * d_J += J[D], d_K += K[D] over the current worklist (D symmetric, device).
  * J_mn = sum (mn|ls) D_ls, K_ml = sum (mn|ls) D_ns, i.e. the result of

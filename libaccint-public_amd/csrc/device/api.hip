@@ -135,6 +135,29 @@ double fprim(int La, int Lb, int Lc, int Ld) {
   const double ncart4 = double(ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld));
   return 30.0 + nr * (30.0 + 20.0 + 3.0 * fdir(La, Lb, Lc, Ld) + 3.0 * ncart4);
 }
+// Gradient work per primitive quartet of a canonical quartet (La Lb|Lc Ld),
+// the derivatives on three centres (the fourth by translational invariance):
+// model 0 evaluates, per centre, the derivative classes with that centre's l
+// raised and (l > 0) lowered by one, F_prim each, plus 9 N_cart contraction
+// flops per root; model 1 builds one table per (root, axis) with the three
+// centres raised together (what the adjoint kernels do):
+// 30 + n_r (50 + 3 F_dir(La+1, Lb+1, Lc+1, Ld) + 12 N_cart), n_r of the raised
+// total angular momentum.
+double fgrad(int model, int La, int Lb, int Lc, int Ld) {
+  const double ncart4 = double(ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld));
+  const int nrg = (La + Lb + Lc + Ld + 1) / 2 + 1;
+  if (model == 1) return 30.0 + nrg * (50.0 + 3.0 * fdir(La + 1, Lb + 1, Lc + 1, Ld) + 12.0 * ncart4);
+  double f = nrg * 9.0 * ncart4;
+  const int l[4] = {La, Lb, Lc, Ld};
+  for (int x = 0; x < 3; ++x)
+    for (int d = -1; d <= 1; d += 2) {
+      int m[4] = {l[0], l[1], l[2], l[3]};
+      m[x] += d;
+      if (m[x] < 0) continue;
+      f += fprim(m[0], m[1], m[2], m[3]);
+    }
+  return f;
+}
 
 int hw_threads() {
   unsigned n = std::thread::hardware_concurrency();
@@ -272,6 +295,7 @@ struct lai_basis {
   double wl_thr = 0.0, wl_tau = 0.0;
   int64_t counts[kNumClasses] = {};
   double cls_f64[kNumClasses] = {}, cls_f32[kNumClasses] = {};
+  double cls_grad[2 * kNumClasses] = {};  // gradient work models 0 and 1 per class
   double flops64 = 0, flops32 = 0;
   int64_t n_eri = 0, n_fp32 = 0;
   uint64_t checksum = 0;
@@ -1122,12 +1146,20 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     for (int y = 0; y <= kMaxL; ++y)
       for (int z = 0; z <= kMaxL; ++z)
         for (int w = 0; w <= kMaxL; ++w) fp_tab[((x * 4 + y) * 4 + z) * 4 + w] = fprim(x, y, z, w);
-  DevBuf<double> dfp, dfl;
+  double fg_tab[2 * 256];
+  for (int m = 0; m < 2; ++m)
+    for (int x = 0; x <= kMaxL; ++x)
+      for (int y = 0; y <= kMaxL; ++y)
+        for (int z = 0; z <= kMaxL; ++z)
+          for (int w = 0; w <= kMaxL; ++w) fg_tab[256 * m + ((x * 4 + y) * 4 + z) * 4 + w] = fgrad(m, x, y, z, w);
+  DevBuf<double> dfp, dfl, dfg, dgf;
   DevBuf<unsigned long long> dcnt, dsum;
   if ((e = dfp.assign(fp_tab, 256)) != hipSuccess || (e = dfl.alloc(4 * kNumClasses)) != hipSuccess ||
+      (e = dfg.assign(fg_tab, 512)) != hipSuccess || (e = dgf.alloc(2 * kNumClasses)) != hipSuccess ||
       (e = dcnt.alloc(3 * kNumClasses)) != hipSuccess || (e = dsum.alloc(1)) != hipSuccess)
     return fail(LAI_ERR_MEMORY, std::string("lai_dev_build_worklist: ") + hipGetErrorString(e));
   LAI_HIP(hipMemset(dfl.p, 0, sizeof(double) * 4 * kNumClasses));
+  LAI_HIP(hipMemset(dgf.p, 0, sizeof(double) * 2 * kNumClasses));
   LAI_HIP(hipMemset(dcnt.p, 0, sizeof(unsigned long long) * 3 * kNumClasses));
   LAI_HIP(hipMemset(dsum.p, 0, sizeof(unsigned long long)));
   WlStatsArgs sa{};
@@ -1140,6 +1172,8 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
   sa.counts = dcnt.p;
   sa.flops = dfl.p;
   sa.checksum = dsum.p;
+  sa.fg = dfg.p;
+  sa.gflops = dgf.p;
   for (int c = 0; c < kNumClasses; ++c) {
     if (b->cls[c].nchunks == 0) continue;
     sa.cls = c;
@@ -1155,6 +1189,7 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
   LAI_HIP(hipMemcpy(cnt.data(), dcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost));
   LAI_HIP(hipMemcpy(fl.data(), dfl.p, sizeof(double) * fl.size(), hipMemcpyDeviceToHost));
   LAI_HIP(hipMemcpy(&sum, dsum.p, sizeof(sum), hipMemcpyDeviceToHost));
+  LAI_HIP(hipMemcpy(b->cls_grad, dgf.p, sizeof(double) * 2 * kNumClasses, hipMemcpyDeviceToHost));
   int64_t total = 0, total32 = 0;
   b->flops64 = b->flops32 = 0.0;
   b->n_eri = 0;
@@ -1740,6 +1775,15 @@ extern "C" int lai_dev_worklist_class_flops(const lai_basis* b, double* f64, dou
   if (!b || !b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_worklist_class_flops: no worklist");
   if (f64) std::memcpy(f64, b->cls_f64, sizeof(b->cls_f64));
   if (f32) std::memcpy(f32, b->cls_f32, sizeof(b->cls_f32));
+  return LAI_OK;
+}
+
+extern "C" int lai_dev_worklist_grad_flops(const lai_basis* b, double* model0, double* model1) {
+  if (!b || !b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_worklist_grad_flops: no worklist");
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (model0) model0[c] = b->cls_grad[2 * c];
+    if (model1) model1[c] = b->cls_grad[2 * c + 1];
+  }
   return LAI_OK;
 }
 

@@ -270,6 +270,38 @@ __device__ __forceinline__ PrimPair load_prim(const PrimPair* base, int i) {
   return r;
 }
 
+// v[ax] of (x, y, z) for a per-lane axis 0..2, by bit masks: a select the
+// compiler cannot turn into an indexed load from a private copy of the
+// record the values come from (which would put that record in scratch)
+__device__ __forceinline__ double pick3(int ax, double x, double y, double z) {
+  const long long m0 = -(long long)(ax == 0), m1 = -(long long)(ax == 1), m2 = -(long long)(ax == 2);
+  return __longlong_as_double((__double_as_longlong(x) & m0) | (__double_as_longlong(y) & m1) |
+                              (__double_as_longlong(z) & m2));
+}
+
+// A pair record whose index is uniform over the wavefront, member by member
+// through the constant address space (scalar loads into SGPRs, no private copy)
+__device__ __forceinline__ PairRec load_pair_uniform(const PairRec* base, int i) {
+  const __attribute__((address_space(4))) PairRec* p = (const __attribute__((address_space(4))) PairRec*)(base + i);
+  PairRec r;
+  r.sa = p->sa;
+  r.sb = p->sb;
+  r.fa = p->fa;
+  r.fb = p->fb;
+  r.pp_off = p->pp_off;
+  r.npp = p->npp;
+  r.same = p->same;
+  r.npp_sig = p->npp_sig;
+  r.mn = p->mn;
+  r.mol = p->mol;
+  r.moff = p->moff;
+  r.ABx = p->ABx;
+  r.ABy = p->ABy;
+  r.ABz = p->ABz;
+  r.bsum = p->bsum;
+  return r;
+}
+
 // FP64 add to LDS (ds_add_f64)
 __device__ __forceinline__ void lds_add(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

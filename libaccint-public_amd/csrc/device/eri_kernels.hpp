@@ -433,6 +433,9 @@ __global__ __launch_bounds__(128) void eri_kernel_regs(EriArgs g) {
 // --------------------------------------------------------------------------
 // Kernel B: one workgroup of NT lanes per shell quartet, 2D tables in LDS
 // --------------------------------------------------------------------------
+#ifndef LAI_LDS_TAB_KB
+#define LAI_LDS_TAB_KB 32
+#endif
 template <int La, int Lb, int Lc, int Ld, typename R>
 struct LdsShape {
   static constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
@@ -442,13 +445,16 @@ struct LdsShape {
   static constexpr int NOUT = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld);
   static constexpr int TS = NR * 3 * S;    // table entries of one primitive quartet
   static constexpr int HSS = NR * 3 * HS;  // transfer scratch of one primitive quartet
+  // the d-major tables of the workgroup kernel: one (root, axis) table every
+  // SP (odd) entries, TSP entries per primitive quartet
+  static constexpr int SP = S | 1;
+  static constexpr int TSP = NR * 3 * SP;
   // primitive quartets whose tables are built at once by NT lanes, within
-  // 16 KB of tables and scratch (more LDS costs workgroups per CU, which
-  // these one-wavefront workgroups need more)
+  // LAI_LDS_TAB_KB of tables
   template <int NT>
   static constexpr int npq() {
     constexpr int by_lanes = NT / (3 * NR);
-    constexpr int by_lds = int((16 * 1024) / ((TS + HSS) * sizeof(R)));
+    constexpr int by_lds = int((LAI_LDS_TAB_KB * 1024) / (TSP * sizeof(R)));
     constexpr int n = by_lanes < by_lds ? by_lanes : by_lds;
     return n < 1 ? 1 : n;
   }
@@ -517,6 +523,79 @@ __device__ __forceinline__ void build_2d_lds(R* __restrict__ tab, R* __restrict_
     }
 }
 
+// The same table in the d-major layout of the wavefront kernel,
+// I[a][b][c][d] at d*SABC + (a*(Lb+1)+b)*(Lc+1)+c with SABC =
+// (La+1)(Lb+1)(Lc+1): the Ld+1 entries a product reads for one (a, b, c) lie
+// SABC apart, so the rows read by the lanes of a wavefront spread over the
+// LDS banks.  No scratch: the bra transfer of column k goes straight into
+// slot k of the (a, b) row's (Lc+1)(Ld+1) slots (slot k = (d, c) with
+// k = d*(Lc+1) + c; Lc+Ld+1 <= (Lc+1)(Ld+1)), and the ket transfer of each
+// row then runs in place from registers (all of a row's slots are read
+// before any is written; the row belongs to this lane).
+template <int La, int Lb, int Lc, int Ld, typename R>
+__device__ __forceinline__ void build_2d_inplace(R* __restrict__ tab, R PA, R QC, R AB, R CD, R B10, R B01, R B00,
+                                                 R s) {
+  constexpr int LAB = La + Lb, LCD = Lc + Ld, SABC = (La + 1) * (Lb + 1) * (Lc + 1);
+  auto slot = [](int row, int k) { return (k / (Lc + 1)) * SABC + row + k % (Lc + 1); };
+  R cm1[LAB + 1], c0[LAB + 1];
+#pragma unroll
+  for (int k = 0; k <= LCD; ++k) {
+    R c1[LAB + 1];
+    if (k == 0) {
+      c1[0] = s;
+#pragma unroll
+      for (int a = 0; a < LAB; ++a) {
+        R v = PA * c1[a];
+        if (a > 0) v += R(a) * B10 * c1[a - 1];
+        c1[a + 1] = v;
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a <= LAB; ++a) {
+        R v = QC * c0[a];
+        if (k > 1) v += R(k - 1) * B01 * cm1[a];
+        if (a > 0) v += R(a) * B00 * c0[a - 1];
+        c1[a] = v;
+      }
+    }
+    R h[Lb + 1][LAB + 1];
+#pragma unroll
+    for (int a = 0; a <= LAB; ++a) h[0][a] = c1[a];
+#pragma unroll
+    for (int b = 0; b < Lb; ++b)
+#pragma unroll
+      for (int a = 0; a < LAB - b; ++a) h[b + 1][a] = h[b][a + 1] + AB * h[b][a];
+#pragma unroll
+    for (int a = 0; a <= La; ++a)
+#pragma unroll
+      for (int b = 0; b <= Lb; ++b) tab[slot((a * (Lb + 1) + b) * (Lc + 1), k)] = h[b][a];
+#pragma unroll
+    for (int a = 0; a <= LAB; ++a) {
+      cm1[a] = c0[a];
+      c0[a] = c1[a];
+    }
+  }
+  if constexpr (Ld > 0) {  // with Ld = 0, slot k = (0, k) already holds I[a][b][k][0]
+#pragma unroll
+    for (int a = 0; a <= La; ++a)
+#pragma unroll
+      for (int b = 0; b <= Lb; ++b) {
+        const int row = (a * (Lb + 1) + b) * (Lc + 1);
+        R X[Ld + 1][LCD + 1];
+#pragma unroll
+        for (int k = 0; k <= LCD; ++k) X[0][k] = tab[slot(row, k)];
+#pragma unroll
+        for (int d = 0; d < Ld; ++d)
+#pragma unroll
+          for (int c = 0; c < LCD - d; ++c) X[d + 1][c] = X[d][c + 1] + CD * X[d][c];
+#pragma unroll
+        for (int c = 0; c <= Lc; ++c)
+#pragma unroll
+          for (int d = 0; d <= Ld; ++d) tab[d * SABC + row + c] = X[d][c];
+      }
+  }
+}
+
 // One Rys root/weight pair (index r of N) from the table.
 template <int N>
 __device__ __forceinline__ void rys_root_one(double T, const double* __restrict__ tab, int r,
@@ -551,12 +630,14 @@ __device__ __forceinline__ void rys_root_one(double T, const double* __restrict_
 // Products are tiled by task = (a, b, c): a lane owns tasks t = tid + NT*j
 // and computes all nd outputs (a, b, c, d) of each from three rows of Ld+1
 // table entries held in registers.  ofs[j] packs the row start of each axis
-// (10 bits each).
+// (10 bits each): ((a*(Lb+1)+b)*(Lc+1)+c)*(Ld+1) in the d-minor layout
+// (ROW_LD), (a*(Lb+1)+b)*(Lc+1)+c in the d-major layout of the wavefront
+// kernel.
 template <int La, int Lb, int Lc, int Ld>
 struct LdsTasks {
   static constexpr int NTASK = ncart(La) * ncart(Lb) * ncart(Lc);
 };
-template <int La, int Lb, int Lc, int Ld, int NT, int NJ>
+template <int La, int Lb, int Lc, int Ld, int NT, int NJ, bool ROW_LD = true>
 __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
   constexpr int nb = ncart(Lb), nc = ncart(Lc);
   constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
@@ -588,7 +669,7 @@ __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
       int p = 0;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        const int off = ((ax[x] * (Lb + 1) + bx[x]) * (Lc + 1) + cx[x]) * (Ld + 1);
+        const int off = ((ax[x] * (Lb + 1) + bx[x]) * (Lc + 1) + cx[x]) * (ROW_LD ? Ld + 1 : 1);
         p |= off << (10 * x);
       }
       ofs[j] = p;
@@ -604,11 +685,12 @@ __device__ __forceinline__ void lds_component_offsets(int tid, int (&ofs)[NJ]) {
 // ends with a workgroup barrier.
 template <int La, int Lb, int Lc, int Ld, typename R, int NT, int NJ, int NPQ>
 __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket,
-                                                bool all_prims, R* tab, R* hs, const int (&ofs)[NJ],
+                                                bool all_prims, R* tab, const int (&ofs)[NJ],
                                                 R (&acc)[NJ][ncart(Ld)]) {
   constexpr int nd = ncart(Ld);
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
-  constexpr int NR = Sh::NR, S = Sh::S, TS = Sh::TS, HSS = Sh::HSS, TASKS = 3 * NR;
+  constexpr int NR = Sh::NR, S = Sh::S, SP = Sh::SP, TS = Sh::TSP, TASKS = 3 * NR;
+  constexpr int SABC = (La + 1) * (Lb + 1) * (Lc + 1);
   static_assert(NPQ * TASKS <= NT, "not enough lanes for the table batch");
   const int tid = threadIdx.x;
 #pragma unroll
@@ -619,13 +701,13 @@ __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec&
   const int ntot = npp_b * npp_k;
   constexpr bool fp32 = sizeof(R) == 4;
   const float qcut = fp32 && !all_prims ? float(g.pcut32 * bra.bsum * ket.bsum) : 0.0f;
-  __syncthreads();  // the previous user of tab and hs has finished
+  __syncthreads();  // the previous user of tab has finished
   for (int t0 = 0; t0 < ntot; t0 += NPQ) {
     if (tid < NPQ * TASKS) {
       const int pq = tid / TASKS, task = tid - pq * TASKS;
       const int r = task / 3, ax = task - 3 * r;
       const int t = t0 + pq;
-      R* tb = tab + pq * TS + task * S;
+      R* tb = tab + pq * TS + task * SP;
       bool live = t < ntot;
       const int ib = live ? t / npp_k : 0, ik = live ? t - ib * npp_k : 0;
       if (live && fp32 && g.ppb[bra.pp_off + ib] * g.ppb[ket.pp_off + ik] < qcut) live = false;
@@ -643,14 +725,14 @@ __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec&
         const R B00 = R(0.5 * ur * izpe);
         const R B10 = R(h_z - h_z * rz * ur);
         const R B01 = R(h_e - h_e * re * ur);
-        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
-        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
-        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
-        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
-        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+        const double PQ = pick3(ax, PQx, PQy, PQz);
+        const double PA = pick3(ax, P.PAx, P.PAy, P.PAz);
+        const double QC = pick3(ax, Q.PAx, Q.PAy, Q.PAz);
+        const double AB = pick3(ax, bra.ABx, bra.ABy, bra.ABz);
+        const double CD = pick3(ax, ket.ABx, ket.ABy, ket.ABz);
         const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
-        build_2d_lds<La, Lb, Lc, Ld, R>(tb, hs + pq * HSS + task * Sh::HS, R(PA - rz * ur * PQ),
-                                        R(QC + re * ur * PQ), R(AB), R(CD), B10, B01, B00, R(sc));
+        build_2d_inplace<La, Lb, Lc, Ld, R>(tb, R(PA - rz * ur * PQ), R(QC + re * ur * PQ), R(AB), R(CD), B10,
+                                            B01, B00, R(sc));
       } else {
         for (int e = 0; e < S; ++e) tb[e] = R(0);  // contributes nothing
       }
@@ -659,18 +741,18 @@ __device__ __forceinline__ void lds_quartet_acc(const EriArgs& g, const PairRec&
     const int npq = ntot - t0 < NPQ ? ntot - t0 : NPQ;
 #pragma unroll 1
     for (int pq = 0; pq < npq; ++pq) {
-#pragma unroll 1
+#pragma unroll
       for (int r = 0; r < NR; ++r) {
-        const R* tx = tab + pq * TS + (r * 3) * S;
+        const R* tx = tab + pq * TS + (r * 3) * SP;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
           const int p = ofs[j];
           R X[Ld + 1], Y[Ld + 1], Z[Ld + 1];
 #pragma unroll
           for (int e = 0; e <= Ld; ++e) {
-            X[e] = tx[(p & 1023) + e];
-            Y[e] = tx[S + ((p >> 10) & 1023) + e];
-            Z[e] = tx[2 * S + (p >> 20) + e];
+            X[e] = tx[(p & 1023) + e * SABC];
+            Y[e] = tx[SP + ((p >> 10) & 1023) + e * SABC];
+            Z[e] = tx[2 * SP + (p >> 20) + e * SABC];
           }
 #pragma unroll
           for (int d = 0; d < nd; ++d)
@@ -690,18 +772,17 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
   constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
   constexpr int NJ = (NTASK + NT - 1) / NT;
   constexpr int NPQ = Sh::template npq<NT>();
-  __shared__ R tab[NPQ * Sh::TS];
-  __shared__ R hs[NPQ * Sh::HSS];
+  __shared__ R tab[NPQ * Sh::TSP];
   const int tid = threadIdx.x;
   int ofs[NJ];
-  lds_component_offsets<La, Lb, Lc, Ld, NT, NJ>(tid, ofs);
+  lds_component_offsets<La, Lb, Lc, Ld, NT, NJ, false>(tid, ofs);
 
   for (int qi = blockIdx.x; qi < g.nq; qi += gridDim.x) {
     const Quartet q = g.q[qi];
     const PairRec bra = g.pairs[q.bra];
     const PairRec ket = g.pairs[q.ket];
     R acc[NJ][nd];
-    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, true, tab, hs, ofs, acc);
+    lds_quartet_acc<La, Lb, Lc, Ld, R, NT, NJ, NPQ>(g, bra, ket, true, tab, ofs, acc);
     const int perm = g.perm[qi];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
@@ -1119,22 +1200,22 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   constexpr int ROWS = na + nb;
   constexpr int NPQ = Sh::template npq<NT>();
   // FP64 tables; the FP32 recursion uses the same storage as float
-  __shared__ double tab[NPQ * Sh::TS];
-  __shared__ double hs[NPQ * Sh::HSS];
+  __shared__ double tab[NPQ * Sh::TSP];
   // without LDS rows: the quartet's six J/K blocks, summed in LDS and added to
   // HBM once per quartet (Jt[ab], Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd])
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
                 OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
   __shared__ double blk[NBLK];
+  __shared__ double dblk[NBLK];  // the quartet's density blocks D[xy], laid out like blk
   extern __shared__ double dyn[];
   const int tid = threadIdx.x;
   const bool lds = g.klds != 0;
   int ofs[NJ];
-  lds_component_offsets<La, Lb, Lc, Ld, NT, NJ>(tid, ofs);
+  lds_component_offsets<La, Lb, Lc, Ld, NT, NJ, false>(tid, ofs);
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
-    const PairRec bra = g.pairs[ch.bra];
+    const PairRec bra = load_pair_uniform(g.pairs, ch.bra);
     const int n = bra.mn;
     const double* __restrict__ D = g.D + bra.moff;
     double* Jt = g.Jt + bra.moff;
@@ -1153,21 +1234,21 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
     int seg = 0;
     for (int k = ch.k0; k < ch.k1; ++k) {
       const int kp = chunk_ket(g.kets, sg, k, seg);
-      const PairRec ket = g.pairs[kp];
+      const PairRec ket = load_pair_uniform(g.pairs, kp);
       const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
       if (mode == 0) continue;  // uniform over the workgroup
       double v[NJ][nd];
       if (MIXED && mode == 2) {
         float acc[NJ][nd];
-        lds_quartet_acc<La, Lb, Lc, Ld, float, NT, NJ, NPQ>(g, bra, ket, false, reinterpret_cast<float*>(tab),
-                                                            reinterpret_cast<float*>(hs), ofs, acc);
+        lds_quartet_acc<La, Lb, Lc, Ld, float, NT, NJ, NPQ>(g, bra, ket, false, reinterpret_cast<float*>(tab), ofs,
+                                                            acc);
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
           for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       } else {
         double acc[NJ][nd];
-        lds_quartet_acc<La, Lb, Lc, Ld, double, NT, NJ, NPQ>(g, bra, ket, false, tab, hs, ofs, acc);
+        lds_quartet_acc<La, Lb, Lc, Ld, double, NT, NJ, NPQ>(g, bra, ket, false, tab, ofs, acc);
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
@@ -1177,35 +1258,58 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
       const bool kl = ket.same, bk = (kp == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
       const double fK = bk ? 1.0 : 2.0;
+      // the quartet's six density blocks D[xy] at the offsets Oxy of the J/K
+      // blocks of the same shape (the previous quartet's readers have passed
+      // the barriers of lds_quartet_acc)
+      for (int o = tid; o < NBLK; o += NT) {
+        size_t r, q;
+        if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; }
+        else if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; }
+        else if (o < OAD) { r = fa + (o - OAC) / nc; q = fc + (o - OAC) % nc; }
+        else if (o < OBC) { r = fa + (o - OAD) / nd; q = fd + (o - OAD) % nd; }
+        else if (o < OBD) { r = fb + (o - OBC) / nc; q = fc + (o - OBC) % nc; }
+        else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
+        dblk[o] = D[r * n + q];
+      }
+      __syncthreads();
 #pragma unroll
-      for (int j = 0; j < NJ; ++j)
-#pragma unroll
-       for (int d = 0; d < nd; ++d) {
+      for (int j = 0; j < NJ; ++j) {
         const int t = tid + NT * j;
         if (t >= NTASK) continue;
         const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-        const double x = v[j][d] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
-        const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
-        const double jv = fJ * x * D[la * n + si];
-        const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
-        const double k1 = fK * x * D[nu * n + si];
-        const double k2 = kl ? 0.0 : fK * x * D[nu * n + la];
-        const double k3 = ij ? 0.0 : fK * x * D[mu * n + si];
-        const double k4 = (ij || kl) ? 0.0 : fK * x * D[mu * n + la];
+        const double sabc = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
+        double x[nd];
+#pragma unroll
+        for (int d = 0; d < nd; ++d) x[d] = v[j][d] * sabc * norm_corr(Ld, d);
+        // per-task sums over d: J[ab] (D[cd]), K[ac] (D[bd]), K[bc] (D[ad])
+        double sj = 0.0, s1 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          sj += x[d] * dblk[OCD + c * nd + d];
+          s1 += x[d] * dblk[OBD + b * nd + d];
+          s3 += x[d] * dblk[OAD + a * nd + d];
+        }
+        const double dab = dblk[OAB + a * nb + b], dbc = dblk[OBC + b * nc + c], dac = dblk[OAC + a * nc + c];
         if (lds) {
-          lds_add(&jab[a * nb + b], jv);
-          if (!bk) lds_add(&jcd[c * nd + d], jw);
-          lds_add(&krow[a * n + la], k1);
-          if (!kl) lds_add(&krow[a * n + si], k2);
-          if (!ij) lds_add(&krow[(na + b) * n + la], k3);
-          if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
+          lds_add(&jab[a * nb + b], fJ * sj);
+          lds_add(&krow[a * n + fc + c], fK * s1);
+          if (!ij) lds_add(&krow[(na + b) * n + fc + c], fK * s3);
         } else {
-          lds_add(&blk[OAB + a * nb + b], jv);
-          if (!bk) lds_add(&blk[OCD + c * nd + d], jw);
-          lds_add(&blk[OAC + a * nc + c], k1);
-          if (!kl) lds_add(&blk[OAD + a * nd + d], k2);
-          if (!ij) lds_add(&blk[OBC + b * nc + c], k3);
-          if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], k4);
+          lds_add(&blk[OAB + a * nb + b], fJ * sj);
+          lds_add(&blk[OAC + a * nc + c], fK * s1);
+          if (!ij) lds_add(&blk[OBC + b * nc + c], fK * s3);
+        }
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          if (lds) {
+            if (!bk) lds_add(&jcd[c * nd + d], fJ * x[d] * dab);
+            if (!kl) lds_add(&krow[a * n + fd + d], fK * x[d] * dbc);
+            if (!ij && !kl) lds_add(&krow[(na + b) * n + fd + d], fK * x[d] * dac);
+          } else {
+            if (!bk) lds_add(&blk[OCD + c * nd + d], fJ * x[d] * dab);
+            if (!kl) lds_add(&blk[OAD + a * nd + d], fK * x[d] * dbc);
+            if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], fK * x[d] * dac);
+          }
         }
       }
       if (!lds) {  // the quartet's blocks to HBM, one atomic per element
@@ -1264,11 +1368,13 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
 // --------------------------------------------------------------------------
 template <int La, int Lb, int Lc, int Ld, typename R, int NJ, int NPQ>
 __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket, int ntot,
-                                                 R* tab, R* hs, int lane, const int (&ofs)[NJ],
-                                                 R (&acc)[NJ][ncart(Ld)]) {
+                                                 R* tab, int lane, const int (&ofs)[NJ], R (&acc)[NJ][ncart(Ld)]) {
   using Sh = LdsShape<La, Lb, Lc, Ld, R>;
   constexpr int nd = ncart(Ld);
-  constexpr int NR = Sh::NR, S = Sh::S, TS = Sh::TS, HSS = Sh::HSS, TASKS = 3 * NR;
+  constexpr int NR = Sh::NR, S = Sh::S, TASKS = 3 * NR;
+  // tables one (root, axis) apart by an odd number of entries (bank spread of
+  // the table lanes' writes); SABC apart between the d entries of a row
+  constexpr int SP = S | 1, TS = TASKS * SP, SABC = (La + 1) * (Lb + 1) * (Lc + 1);
   static_assert(NPQ * TASKS <= 64, "not enough lanes for the table batch");
 #pragma unroll
   for (int j = 0; j < NJ; ++j)
@@ -1283,7 +1389,7 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
       const int pq = lane / TASKS, task = lane - pq * TASKS;
       const int r = task / 3, ax = task - 3 * r;
       const int t = t0 + pq;
-      R* tb = tab + pq * TS + task * S;
+      R* tb = tab + pq * TS + task * SP;
       bool live = t < ntot;
       const int ib = live ? t / npp_k : 0, ik = live ? t - ib * npp_k : 0;
       if (live && fp32 && g.ppb[bra.pp_off + ib] * g.ppb[ket.pp_off + ik] < qcut) live = false;
@@ -1301,14 +1407,14 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
         const R B00 = R(0.5 * ur * izpe);
         const R B10 = R(h_z - h_z * rz * ur);
         const R B01 = R(h_e - h_e * re * ur);
-        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
-        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
-        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
-        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
-        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+        const double PQ = pick3(ax, PQx, PQy, PQz);
+        const double PA = pick3(ax, P.PAx, P.PAy, P.PAz);
+        const double QC = pick3(ax, Q.PAx, Q.PAy, Q.PAz);
+        const double AB = pick3(ax, bra.ABx, bra.ABy, bra.ABz);
+        const double CD = pick3(ax, ket.ABx, ket.ABy, ket.ABz);
         const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
-        build_2d_lds<La, Lb, Lc, Ld, R>(tb, hs + pq * HSS + task * Sh::HS, R(PA - rz * ur * PQ),
-                                        R(QC + re * ur * PQ), R(AB), R(CD), B10, B01, B00, R(sc));
+        build_2d_inplace<La, Lb, Lc, Ld, R>(tb, R(PA - rz * ur * PQ), R(QC + re * ur * PQ), R(AB), R(CD), B10,
+                                            B01, B00, R(sc));
       } else {
         for (int e = 0; e < S; ++e) tb[e] = R(0);  // contributes nothing
       }
@@ -1318,18 +1424,18 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
     const int npq = rem < NPQ ? (rem > 0 ? rem : 0) : NPQ;
 #pragma unroll 1
     for (int pq = 0; pq < npq; ++pq) {
-#pragma unroll 1
+#pragma unroll
       for (int r = 0; r < NR; ++r) {
-        const R* tx = tab + pq * TS + (r * 3) * S;
+        const R* tx = tab + pq * TS + (r * 3) * SP;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
           const int p = ofs[j];
           R X[Ld + 1], Y[Ld + 1], Z[Ld + 1];
 #pragma unroll
           for (int e = 0; e <= Ld; ++e) {
-            X[e] = tx[(p & 1023) + e];
-            Y[e] = tx[S + ((p >> 10) & 1023) + e];
-            Z[e] = tx[2 * S + (p >> 20) + e];
+            X[e] = tx[(p & 1023) + e * SABC];
+            Y[e] = tx[SP + ((p >> 10) & 1023) + e * SABC];
+            Z[e] = tx[2 * SP + (p >> 20) + e * SABC];
           }
 #pragma unroll
           for (int d = 0; d < nd; ++d)
@@ -1341,15 +1447,20 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
   }
 }
 
+#ifndef LAI_WAVE_TAB_KB
+#define LAI_WAVE_TAB_KB 12
+#endif
 template <int La, int Lb, int Lc, int Ld>
 struct WaveShape {
   static constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
   static constexpr int NJ = (NTASK + 63) / 64;
-  // primitive quartets per wave and step, within 6 KB of tables per wave
+  // table entries of one primitive quartet (odd stride per (root, axis))
+  static constexpr int TS = 3 * LdsShape<La, Lb, Lc, Ld, double>::NR * (LdsShape<La, Lb, Lc, Ld, double>::S | 1);
+  // primitive quartets per wave and step, within LAI_WAVE_TAB_KB of tables per wave
   static constexpr int NPQ = [] {
     using Sh = LdsShape<La, Lb, Lc, Ld, double>;
     constexpr int by_lanes = 64 / (3 * Sh::NR);
-    constexpr int by_lds = int((6 * 1024) / ((Sh::TS + Sh::HSS) * sizeof(double)));
+    constexpr int by_lds = int((LAI_WAVE_TAB_KB * 1024) / (3 * Sh::NR * (Sh::S | 1) * sizeof(double)));
     constexpr int n = by_lanes < by_lds ? by_lanes : by_lds;
     return n < 1 ? 1 : n;
   }();
@@ -1365,18 +1476,18 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
   constexpr int ROWS = na + nb;
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
                 OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
-  __shared__ double tab[NW][NPQ * Sh::TS];
-  __shared__ double hs[NW][NPQ * Sh::HSS];
-  __shared__ double blk[NW][NBLK];  // per wave: the quartet's J/K blocks (jcd only, with LDS rows)
+  __shared__ double tab[NW][NPQ * Ws::TS];
+  __shared__ double blk[NW][NBLK];
+  __shared__ double dblk[NW][NBLK];  // per wave: the quartet's density blocks, laid out like blk  // per wave: the quartet's J/K blocks (jcd only, with LDS rows)
   extern __shared__ double dyn[];   // exchange rows [na + nb][mn], then the bra block [na*nb]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
   const bool lds = g.klds != 0;
   int ofs[NJ];
-  lds_component_offsets<La, Lb, Lc, Ld, 64, NJ>(lane, ofs);
+  lds_component_offsets<La, Lb, Lc, Ld, 64, NJ, false>(lane, ofs);
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
-    const PairRec bra = g.pairs[ch.bra];
+    const PairRec bra = load_pair_uniform(g.pairs, ch.bra);
     const int n = bra.mn;
     const double* __restrict__ D = g.D + bra.moff;
     double* Jt = g.Jt + bra.moff;
@@ -1394,22 +1505,22 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
     int seg = 0;
     for (int k = ch.k0 + wave; k < ch.k1; k += NW) {  // uniform over the wavefront
       const int kp = chunk_ket(g.kets, sg, k, seg);
-      const PairRec ket = g.pairs[kp];
+      const PairRec ket = load_pair_uniform(g.pairs, kp);
       const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
       if (mode == 0) continue;
       const int ntot = bra.npp_sig * ket.npp_sig;
       double v[NJ][nd];
       if (MIXED && mode == 2) {
         float acc[NJ][nd];
-        wave_quartet_acc<La, Lb, Lc, Ld, float, NJ, NPQ>(g, bra, ket, ntot, reinterpret_cast<float*>(tab[wave]),
-                                                         reinterpret_cast<float*>(hs[wave]), lane, ofs, acc);
+        wave_quartet_acc<La, Lb, Lc, Ld, float, NJ, NPQ>(g, bra, ket, ntot, reinterpret_cast<float*>(tab[wave]), lane,
+                                                         ofs, acc);
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
           for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       } else {
         double acc[NJ][nd];
-        wave_quartet_acc<La, Lb, Lc, Ld, double, NJ, NPQ>(g, bra, ket, ntot, tab[wave], hs[wave], lane, ofs, acc);
+        wave_quartet_acc<La, Lb, Lc, Ld, double, NJ, NPQ>(g, bra, ket, ntot, tab[wave], lane, ofs, acc);
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
 #pragma unroll
@@ -1419,36 +1530,59 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
       const bool kl = ket.same, bk = (kp == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
       const double fK = bk ? 1.0 : 2.0;
+      // the quartet's six density blocks D[xy], staged in this wave's LDS at
+      // the offsets of the J/K blocks of the same shape (Oxy)
+      double* dw = dblk[wave];
+      for (int o = lane; o < NBLK; o += 64) {
+        size_t r, q;
+        if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; }
+        else if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; }
+        else if (o < OAD) { r = fa + (o - OAC) / nc; q = fc + (o - OAC) % nc; }
+        else if (o < OBC) { r = fa + (o - OAD) / nd; q = fd + (o - OAD) % nd; }
+        else if (o < OBD) { r = fb + (o - OBC) / nc; q = fc + (o - OBC) % nc; }
+        else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
+        dw[o] = D[r * n + q];
+      }
+      wave_sync();
 #pragma unroll
-      for (int j = 0; j < NJ; ++j)
+      for (int j = 0; j < NJ; ++j) {
+        const int t = lane + 64 * j;
+        if (t >= NTASK) continue;
+        const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+        const double sabc = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
+        double x[nd];
+#pragma unroll
+        for (int d = 0; d < nd; ++d) x[d] = v[j][d] * sabc * norm_corr(Ld, d);
+        // per-task sums over d: J[ab] (D[cd]), K[ac] (D[bd]), K[bc] (D[ad])
+        double sj = 0.0, s1 = 0.0, s3 = 0.0;
 #pragma unroll
         for (int d = 0; d < nd; ++d) {
-          const int t = lane + 64 * j;
-          if (t >= NTASK) continue;
-          const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-          const double x = v[j][d] * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
-          const size_t mu = fa + a, nu = fb + b, la = fc + c, si = fd + d;
-          const double jv = fJ * x * D[la * n + si];
-          const double jw = bk ? 0.0 : fJ * x * D[mu * n + nu];
-          const double k1 = fK * x * D[nu * n + si];
-          const double k2 = kl ? 0.0 : fK * x * D[nu * n + la];
-          const double k3 = ij ? 0.0 : fK * x * D[mu * n + si];
-          const double k4 = (ij || kl) ? 0.0 : fK * x * D[mu * n + la];
-          if (!bk) lds_add(&bw[OCD + c * nd + d], jw);
+          sj += x[d] * dw[OCD + c * nd + d];
+          s1 += x[d] * dw[OBD + b * nd + d];
+          s3 += x[d] * dw[OAD + a * nd + d];
+        }
+        const double dab = dw[OAB + a * nb + b], dbc = dw[OBC + b * nc + c], dac = dw[OAC + a * nc + c];
+        if (lds) {
+          lds_add(&jab[a * nb + b], fJ * sj);
+          lds_add(&krow[a * n + fc + c], fK * s1);
+          if (!ij) lds_add(&krow[(na + b) * n + fc + c], fK * s3);
+        } else {
+          lds_add(&bw[OAB + a * nb + b], fJ * sj);
+          lds_add(&bw[OAC + a * nc + c], fK * s1);
+          if (!ij) lds_add(&bw[OBC + b * nc + c], fK * s3);
+        }
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {
+          if (!bk) lds_add(&bw[OCD + c * nd + d], fJ * x[d] * dab);
           if (lds) {
-            lds_add(&jab[a * nb + b], jv);
-            lds_add(&krow[a * n + la], k1);
-            if (!kl) lds_add(&krow[a * n + si], k2);
-            if (!ij) lds_add(&krow[(na + b) * n + la], k3);
-            if (!ij && !kl) lds_add(&krow[(na + b) * n + si], k4);
+            if (!kl) lds_add(&krow[a * n + fd + d], fK * x[d] * dbc);
+            if (!ij && !kl) lds_add(&krow[(na + b) * n + fd + d], fK * x[d] * dac);
           } else {
-            lds_add(&bw[OAB + a * nb + b], jv);
-            lds_add(&bw[OAC + a * nc + c], k1);
-            if (!kl) lds_add(&bw[OAD + a * nd + d], k2);
-            if (!ij) lds_add(&bw[OBC + b * nc + c], k3);
-            if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], k4);
+            if (!kl) lds_add(&bw[OAD + a * nd + d], fK * x[d] * dbc);
+            if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], fK * x[d] * dac);
           }
         }
+      }
       wave_sync();  // the wave's quartet block is complete
       // this wave's quartet block to HBM, one atomic per element
       for (int o = lane; o < NBLK; o += 64) {
@@ -1496,13 +1630,13 @@ struct ClassPolicy {
   static constexpr int NCH = (na + CHA - 1) / CHA;
   static constexpr bool regs = rest <= 27 && NCH <= 6;
   static constexpr int NOUT = na * nb * nc * nd;
-  static constexpr int NT = NOUT <= 64 * 24 ? 64 : (NOUT <= 128 * 24 ? 128 : 256);
+  static constexpr int NT = 256;  // lanes of the workgroup-per-quartet shape
   // LDS-shape classes of at most 6 (a, b, c) tasks per lane of a wavefront
   // run one quartet per wavefront, four wavefronts per workgroup
   static constexpr bool waves = !regs && LdsTasks<La, Lb, Lc, Ld>::NTASK <= 6 * 64;
-  // four wavefronts, two where one wavefront's tables exceed 12 KB
+  // four wavefronts, two where one wavefront's tables exceed the per-wave budget
   static constexpr int NW =
-      (LdsShape<La, Lb, Lc, Ld, double>::TS + LdsShape<La, Lb, Lc, Ld, double>::HSS) * 8 <= 12 * 1024 ? 4 : 2;
+      WaveShape<La, Lb, Lc, Ld>::NPQ * WaveShape<La, Lb, Lc, Ld>::TS * 8 <= LAI_WAVE_TAB_KB * 1024 ? 4 : 2;
   // shape codes: 0 lane per quartet, 1 wavefront per quartet, 2 workgroup per quartet
   static constexpr int shape = regs ? 0 : (waves ? 1 : 2);
   // the second shape compiled for the autotuner: register classes that split

@@ -141,11 +141,11 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
         rys_root_one<NR>(T, g.rys, r, ur, wr);
         const double h_z = 0.5 / zeta, h_e = 0.5 / eta;
         const double B00 = 0.5 * ur * izpe, B10 = h_z - h_z * rz * ur, B01 = h_e - h_e * re * ur;
-        const double PQ = ax == 0 ? PQx : (ax == 1 ? PQy : PQz);
-        const double PA = ax == 0 ? P.PAx : (ax == 1 ? P.PAy : P.PAz);
-        const double QC = ax == 0 ? Q.PAx : (ax == 1 ? Q.PAy : Q.PAz);
-        const double AB = ax == 0 ? bra.ABx : (ax == 1 ? bra.ABy : bra.ABz);
-        const double CD = ax == 0 ? ket.ABx : (ax == 1 ? ket.ABy : ket.ABz);
+        const double PQ = pick3(ax, PQx, PQy, PQz);
+        const double PA = pick3(ax, P.PAx, P.PAy, P.PAz);
+        const double QC = pick3(ax, Q.PAx, Q.PAy, Q.PAz);
+        const double AB = pick3(ax, bra.ABx, bra.ABy, bra.ABz);
+        const double CD = pick3(ax, ket.ABx, ket.ABy, ket.ABz);
         const double sc = ax == 2 ? P.Kc * Q.Kc * sqrt(izpe) * wr : 1.0;
         build_2d_lds<La + 1, Lb + 1, Lc + 1, Ld, double>(tab + ((q * NR + r) * 3 + ax) * S,
                                                          hs + ((q * NR + r) * 3 + ax) * HS, PA - rz * ur * PQ,

@@ -20,7 +20,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 __global__ __launch_bounds__(256) void worklist_stats_kernel(WlStatsArgs a) {
   const int tid = threadIdx.x;
   unsigned long long n = 0, n32 = 0, neri = 0, sum = 0;
-  double fm64 = 0.0, fm32 = 0.0, fx64 = 0.0, fx32 = 0.0;
+  double fm64 = 0.0, fm32 = 0.0, fx64 = 0.0, fx32 = 0.0, g0 = 0.0, g1 = 0.0;
   for (int ci = blockIdx.x; ci < a.nchunks; ci += gridDim.x) {
     const JkChunk ch = a.chunks[ci];
     const PairRec bra = a.pairs[ch.bra];
@@ -45,6 +45,9 @@ __global__ __launch_bounds__(256) void worklist_stats_kernel(WlStatsArgs a) {
       const int Li = a.L[i], Lj = a.L[j], Lk = a.L[kk], Ll = a.L[l];
       const double F = a.fp[((Li * 4 + Lj) * 4 + Lk) * 4 + Ll];
       const double fm = double(bra.npp) * ket.npp * F, fx = double(bra.npp_sig) * ket.npp_sig * F;
+      const int gi = ((Li * 4 + Lj) * 4 + Lk) * 4 + Ll;
+      g0 += double(bra.npp_sig) * ket.npp_sig * a.fg[gi];
+      g1 += double(bra.npp_sig) * ket.npp_sig * a.fg[256 + gi];
       neri += (unsigned long long)(ncart(Li) * ncart(Lj)) * (ncart(Lk) * ncart(Ll));
       ++n;
       if (mode == 2) {
@@ -65,7 +68,11 @@ __global__ __launch_bounds__(256) void worklist_stats_kernel(WlStatsArgs a) {
   fm32 = wave_sum(fm32);
   fx64 = wave_sum(fx64);
   fx32 = wave_sum(fx32);
+  g0 = wave_sum(g0);
+  g1 = wave_sum(g1);
   if ((tid & 63) == 0 && n > 0) {
+    atomic_add(&a.gflops[2 * a.cls], g0);
+    atomic_add(&a.gflops[2 * a.cls + 1], g1);
     atomicAdd(&a.counts[3 * a.cls], n);
     atomicAdd(&a.counts[3 * a.cls + 1], n32);
     atomicAdd(&a.counts[3 * a.cls + 2], neri);

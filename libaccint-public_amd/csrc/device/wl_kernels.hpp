@@ -23,6 +23,11 @@ struct WlStatsArgs {
   unsigned long long* counts;  // [kNumClasses][3]
   double* flops;               // [kNumClasses][4]
   unsigned long long* checksum;
+  // gradient work per primitive quartet, two models ([La][Lb][Lc][Ld], 4^4
+  // each), and the per-class sums over the executed primitive pairs of every
+  // selected quartet ([kNumClasses][2])
+  const double* __restrict__ fg;
+  double* gflops;
 };
 
 __global__ void worklist_stats_kernel(WlStatsArgs a);

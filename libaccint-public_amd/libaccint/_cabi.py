@@ -67,6 +67,7 @@ _SIGNATURES = {
     "lai_quartet_mix64": ([ctypes.c_int] * 4, ctypes.c_uint64),
     "lai_dev_worklist_stats": ([_vp, ctypes.POINTER(_i64), _dp, _dp, ctypes.POINTER(_i64)], ctypes.c_int),
     "lai_dev_worklist_class_flops": ([_vp, _dp, _dp], ctypes.c_int),
+    "lai_dev_worklist_grad_flops": ([_vp, _dp, _dp], ctypes.c_int),
     "lai_dev_jk": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "lai_dev_jk_profile": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _dp, _dp], ctypes.c_int),
     "lai_dev_jk_graph_capture": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
