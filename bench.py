@@ -308,11 +308,47 @@ def main():
         Jm, Km = m_once(om)
         Em = 0.5 * np.sum(mD * Jm) - 0.25 * np.sum(mD * Km)
         ms_m = m_timed(om, msteps, 1)
+        # the mixed build's FP64 and FP32 class launches timed one after
+        # another, with the executed flops of each precision
+        hm = ctypes.c_void_p(meng.device_basis_handle())
+        dp = ctypes.POINTER(ctypes.c_double)
+        fm64, fm32, pm64, pm32 = np.zeros(55), np.zeros(55), np.zeros(55), np.zeros(55)
+        _cabi.check(L.lai_dev_worklist_class_flops(hm, fm64.ctypes.data_as(dp), fm32.ctypes.data_as(dp)))
+        _cabi.check(L.lai_dev_jk_profile(hm, ctypes.c_void_p(tmD.data_ptr()), ctypes.c_void_p(tmJ.data_ptr()),
+                                         ctypes.c_void_p(tmK.data_ptr()), _cabi.LAI_JK_MIXED,
+                                         ctypes.c_void_p(stream.cuda_stream), pm64.ctypes.data_as(dp),
+                                         pm32.ctypes.data_as(dp)))
+        split = {"class_ms_fp64": float(pm64.sum()), "class_ms_fp32": float(pm32.sum()),
+                 "tflops_fp64": float(fm64.sum() / max(pm64.sum(), 1e-9) / 1e9),
+                 "tflops_fp32": float(fm32.sum() / max(pm32.sum(), 1e-9) / 1e9)}
+        log("C40H82 mixed split: FP64 launches %.3f ms at %.2f TFLOP/s, FP32 launches %.3f ms at %.2f TFLOP/s" %
+            (split["class_ms_fp64"], split["tflops_fp64"], split["class_ms_fp32"], split["tflops_fp32"]))
         # the same FP32 split without the FP32 primitive cut
         on = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7, fp32_prim_cut=0.0)
         Jn, Kn = m_once(on)
         En = 0.5 * np.sum(mD * Jn) - 0.25 * np.sum(mD * Kn)
         ms_n = m_timed(on, msteps, 1)
+        # every quartet through the float instantiation (tau above every bound,
+        # no primitive cut) against the FP64 build, class by class: what the
+        # FP32 recursion alone buys
+        def m_profile(o, flags):
+            meng.prepare_worklist(o, mD if o.mixed_precision else None)
+            _cabi.check(L.lai_dev_set_mixed_prim_cut(hm, ctypes.c_double(o.fp32_prim_cut)))
+            a64, a32 = np.zeros(55), np.zeros(55)
+            for _ in range(2):
+                _cabi.check(L.lai_dev_jk_profile(hm, ctypes.c_void_p(tmD.data_ptr()), ctypes.c_void_p(tmJ.data_ptr()),
+                                                 ctypes.c_void_p(tmK.data_ptr()), flags,
+                                                 ctypes.c_void_p(stream.cuda_stream), a64.ctypes.data_as(dp),
+                                                 a32.ctypes.data_as(dp)))
+            return a64 + a32
+        p64 = m_profile(o64, 0)
+        p32 = m_profile(lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e30,
+                                      fp32_prim_cut=0.0), _cabi.LAI_JK_MIXED)
+        split["all_fp32_vs_fp64"] = {
+            "class_ms_fp64": float(p64.sum()), "class_ms_all_fp32": float(p32.sum()),
+            "top_classes_ms_fp64_fp32": [(cls_name(c), round(float(p64[c]), 3), round(float(p32[c]), 3))
+                                         for c in np.argsort(-p64)[:8]]}
+        log("C40H82 all quartets FP32: class kernels %.3f ms against %.3f ms FP64" % (p32.sum(), p64.sum()))
         mixed = {"mixed_workload": "C40H82 def2-SVP (1010 Cartesian bf), Schwarz 1e-12, %d quartets; FP32 Rys "
                                    "recursion where Q_ij Q_kl max|D| < 1e-7" % mi64["n_quartets"],
                  "jk_seconds_fp64_c40h82": ms_d * 1e-3, "jk_seconds_mixed": ms_m * 1e-3,
@@ -321,7 +357,7 @@ def main():
                  "dE_mixed_Eh": float(Em - Ed), "dJ_mixed_max": float(np.max(np.abs(Jm - Jd))),
                  "dK_mixed_max": float(np.max(np.abs(Km - Kd))), "fp32_threshold": 1e-7,
                  "fp32_prim_cut": 1e-7, "jk_seconds_mixed_no_prim_cut": ms_n * 1e-3,
-                 "dE_mixed_no_prim_cut_Eh": float(En - Ed)}
+                 "dE_mixed_no_prim_cut_Eh": float(En - Ed), "mixed_class_split": split}
         log("C40H82: FP64 %.3f ms, mixed %.3f ms, speedup %.2fx, dE %.2e Eh, fp32 fraction %.3f" %
             (ms_d, ms_m, ms_d / ms_m, Em - Ed, mixed["mixed_fp32_quartet_fraction"]))
         del meng, tmD, tmJ, tmK
@@ -344,13 +380,16 @@ def main():
         bo = lai.JKOptions(schwarz_threshold=1e-12)
         binfo = be.prepare_worklist(bo)
         bsetup = time.perf_counter() - t0
+        log("batch (%d molecules): setup %.2f s, %d quartets, %.3e executed flops" %
+            (len(sel), bsetup, binfo["n_quartets"], binfo["flops_fp64"] + binfo["flops_fp32"]))
         tbD = torch.from_numpy(bD).to(dev)
         tbJ = torch.zeros_like(tbD)
         tbK = torch.zeros_like(tbD)
 
-        def b_timed(o, steps):
+        def b_timed(o, steps, what):
             be.capture_jk_graph(tbD.data_ptr(), tbJ.data_ptr(), tbK.data_ptr(), o, stream.cuda_stream)
-            be.launch_jk_graph(stream.cuda_stream)
+            if not args.batch_full:  # the full batch is timed on its first replay
+                be.launch_jk_graph(stream.cuda_stream)
             torch.cuda.synchronize()
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
@@ -359,7 +398,9 @@ def main():
                 be.launch_jk_graph(stream.cuda_stream)
             ev1.record(stream)
             torch.cuda.synchronize()
-            return ev0.elapsed_time(ev1) / steps
+            ms = ev0.elapsed_time(ev1) / steps
+            log("batch %s build: %.3f ms" % (what, ms))
+            return ms
 
         bsteps = 1 if args.batch_full else 3
         bt = 0.0
@@ -367,11 +408,11 @@ def main():
             t0 = time.perf_counter()
             be.autotune(tbD.data_ptr(), bo, stream.cuda_stream, 1)
             bt = time.perf_counter() - t0
-        ms_b = b_timed(bo, bsteps)
+        ms_b = b_timed(bo, bsteps, "FP64")
         bm = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
         be.prepare_worklist(bm, bD)
         bminfo = be.worklist_info()
-        ms_bm = b_timed(bm, bsteps)
+        ms_bm = b_timed(bm, bsteps, "mixed")
         # the same molecules one engine at a time (no batching), FP64
         engines = [lai.Engine(bb) for bb in bases]
         offs = be.matrix_offsets()

@@ -227,9 +227,11 @@ int lai_dev_worklist_grad_flops(const lai_basis* b, double* model0, double* mode
  * with digestion fused into the ERI kernels.  Asynchronous on stream. */
 int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream);
 
-/* The same J/K build with a hipEvent pair around every class kernel; writes
- * the per-class milliseconds of the FP64 and FP32 lists (55 entries each) and
- * synchronises the stream.  Measurement aid; no reference counterpart. */
+/* The same J/K build with a hipEvent pair around every class launch, one
+ * class after another; writes the per-class milliseconds of the FP64 launch
+ * and of the FP32 launch (55 entries each; a mixed build runs each class as
+ * one launch per precision, the FP32 ones are 0 otherwise) and synchronises
+ * the stream.  Measurement aid; no reference counterpart. */
 int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream,
                        double* ms_fp64, double* ms_fp32);
 

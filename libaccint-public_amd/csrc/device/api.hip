@@ -295,6 +295,7 @@ struct lai_basis {
   double wl_thr = 0.0, wl_tau = 0.0;
   int64_t counts[kNumClasses] = {};
   double cls_f64[kNumClasses] = {}, cls_f32[kNumClasses] = {};
+  int64_t cls_n32[kNumClasses] = {};  // quartets of each class the worklist marks for FP32
   double cls_grad[2 * kNumClasses] = {};  // gradient work models 0 and 1 per class
   double flops64 = 0, flops32 = 0;
   int64_t n_eri = 0, n_fp32 = 0;
@@ -1202,6 +1203,7 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
     b->flops32 += fl[4 * c + 1];
     b->cls_f64[c] = fl[4 * c + 2];
     b->cls_f32[c] = fl[4 * c + 3];
+    b->cls_n32[c] = int64_t(cnt[3 * c + 1]);
   }
   b->n_fp32 = total32;
   b->checksum = sum;
@@ -1245,16 +1247,21 @@ static int ensure_side_streams(lai_basis* b) {
   return LAI_OK;
 }
 
-// Class kernels over the bra-anchored chunks, heavy classes first.  With ev
-// (4 events per class) a hipEvent pair brackets each of the two parts, one
-// class after another on st.  Without: the classes are spread over st and
-// the side streams (each class to the stream with the least work so far, by
-// the classes' executed flops), forked from and joined back into st, so
-// that the last workgroups of one class overlap the next classes (every
-// class adds into Jt and Kt with atomics only, so the order is free).
+// Class kernels over the bra-anchored chunks, heavy classes first.  A mixed
+// build runs each class as two launches, the FP64 instantiation over the
+// quartets selected for FP64 (QuartetSel::keep = 1) and the float one over
+// those selected for FP32 (keep = 2), so that each has its own register
+// budget; a part with no quartets is not launched (the worklist statistics
+// count them with the same selection).  With ev (4 events per class) a
+// hipEvent pair brackets each of the two parts, one after another on st.
+// Without: the parts are spread over st and the side streams (each to the
+// stream with the least work so far, by executed flops), forked from and
+// joined back into st, so that the last workgroups of one class overlap the
+// next classes (every class adds into Jt and Kt with atomics only, so the
+// order is free).
 static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
   const auto& Ls = launchers();
-  const bool mixed = (flags & LAI_JK_MIXED) != 0;
+  const bool mixed = (flags & LAI_JK_MIXED) != 0 && b->wl_mixed;
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
@@ -1271,25 +1278,39 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
   }
   for (int c = kNumClasses - 1; c >= 0; --c) {
     const ClassList& C = b->cls[c];
-    hipStream_t s = st;
-    if (fan && C.nchunks > 0) {
-      int k = 0;
-      for (int i = 1; i <= kSideStreams; ++i)
-        if (load[i] < load[k]) k = i;
-      load[k] += (mixed ? b->cls_f64[c] + b->cls_f32[c] : b->cls_f64[c]) + 1e6 * double(C.nchunks);
-      s = streams[k];
+    for (int part = 0; part < 2; ++part) {  // 0: FP64, 1: FP32 (mixed builds only)
+      hipEvent_t* e = ev ? ev + 4 * c + 2 * part : nullptr;
+      if (e) (void)hipEventRecord(e[0], st);
+      const int64_t n32 = mixed ? b->cls_n32[c] : 0;
+      const bool run = C.nchunks > 0 && (part == 0 ? b->counts[c] - n32 > 0 : n32 > 0);
+      if (run) {
+        hipStream_t s = st;
+        if (fan) {
+          int k = 0;
+          for (int i = 1; i <= kSideStreams; ++i)
+            if (load[i] < load[k]) k = i;
+          const double fl = part == 1 ? b->cls_f32[c] : mixed ? b->cls_f64[c] : b->cls_f64[c] + b->cls_f32[c];
+          load[k] += fl + 1e6 * double(C.nchunks);
+          s = streams[k];
+        }
+        a.chunks = C.chunks.p;
+        a.nchunks = int(C.nchunks);
+        a.sel.keep = mixed ? part + 1 : 0;
+        LaunchTune t;
+        t.alt = b->tune[c].alt;
+        t.blocks_per_cu = b->tune[c].bpc;
+        t.n_cu = b->ctx->n_cu;
+        (part == 1 ? Ls.f32[c] : Ls.f64[c])(a, s, t);
+      }
+      if (e) (void)hipEventRecord(e[1], st);
+      if (!mixed) {
+        if (e) {  // the FP32 part is empty: a zero-length pair
+          (void)hipEventRecord(e[2], st);
+          (void)hipEventRecord(e[3], st);
+        }
+        break;
+      }
     }
-    if (ev) (void)hipEventRecord(ev[2 * c], st);
-    if (C.nchunks > 0) {
-      a.chunks = C.chunks.p;
-      a.nchunks = int(C.nchunks);
-      LaunchTune t;
-      t.alt = b->tune[c].alt;
-      t.blocks_per_cu = b->tune[c].bpc;
-      t.n_cu = b->ctx->n_cu;
-      (mixed ? Ls.f32[c] : Ls.f64[c])(a, s, t);
-    }
-    if (ev) (void)hipEventRecord(ev[2 * c + 1], st);
   }
   if (fan)
     for (int i = 0; i < kSideStreams; ++i) {
@@ -1649,7 +1670,8 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
-  if (flags != LAI_JK_MIXED) a.sel.mixed = 0;
+  const bool mixed = flags == LAI_JK_MIXED && b->wl_mixed;
+  if (!mixed) a.sel.mixed = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   LAI_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1659,7 +1681,8 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
   // the trial launches accumulate into the J/K scratch, which every build clears
   int rc = LAI_OK;
   int n_rep = 0;
-  // milliseconds per launch of class c under b->tune[c] (chunks uploaded)
+  // milliseconds per launch of class c under b->tune[c] (chunks uploaded);
+  // a mixed build times its FP64 and FP32 launches together, as it runs them
   auto time_class = [&](int c, double& ms) -> int {
     const ClassList& C = b->cls[c];
     a.chunks = C.chunks.p;
@@ -1668,11 +1691,22 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
     t.alt = b->tune[c].alt;
     t.blocks_per_cu = b->tune[c].bpc;
     t.n_cu = n_cu;
-    const ClassLauncher f = flags == LAI_JK_MIXED ? Ls.f32[c] : Ls.f64[c];
-    f(a, st, t);  // warm-up
+    const int64_t n32 = mixed ? b->cls_n32[c] : 0;
+    const bool p64 = b->counts[c] - n32 > 0, p32 = n32 > 0;
+    auto f = [&](hipStream_t s, const LaunchTune& tt) {
+      if (p64) {
+        a.sel.keep = mixed ? 1 : 0;
+        Ls.f64[c](a, s, tt);
+      }
+      if (p32) {
+        a.sel.keep = 2;
+        Ls.f32[c](a, s, tt);
+      }
+    };
+    f(st, t);  // warm-up
     LAI_HIP(hipGetLastError());
     LAI_HIP(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) f(a, st, t);
+    for (int r = 0; r < reps; ++r) f(st, t);
     LAI_HIP(hipEventRecord(e1, st));
     LAI_HIP(hipGetLastError());
     LAI_HIP(hipEventSynchronize(e1));
@@ -1793,25 +1827,25 @@ extern "C" int lai_dev_jk_profile(lai_basis* b, const double* d_D, double* d_J, 
   if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk_profile: worklist not built");
   if ((flags & ~(LAI_JK_MIXED | LAI_JK_ANTISYMMETRIC)) != 0)
     return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk_profile: unknown flags");
-  const bool mixed = (flags & LAI_JK_MIXED) != 0;
   hipStream_t st = hipStream_t(stream);
   const size_t nn = size_t(b->mat_size);
   int rc;
   if ((rc = lai_dev_zero(b->Jt.p, nn, stream)) != LAI_OK) return rc;
   if ((rc = lai_dev_zero(b->Kt.p, nn, stream)) != LAI_OK) return rc;
-  std::vector<hipEvent_t> ev(2 * kNumClasses, nullptr);
+  std::vector<hipEvent_t> ev(4 * kNumClasses, nullptr);
   for (auto& e : ev) LAI_HIP(hipEventCreate(&e));
   launch_jk_classes(b, jk_args(b, d_D), flags, st, ev.data());
   LAI_HIP(hipGetLastError());
   if ((rc = symmetrize(b, d_J, d_K, st, (flags & LAI_JK_ANTISYMMETRIC) != 0)) != LAI_OK) return rc;
   LAI_HIP(hipStreamSynchronize(st));
-  // one launch per class serves both precisions: its time is reported as
-  // FP64 for LAI_JK_FP64 and as FP32 (mixed) for LAI_JK_MIXED
+  // the FP64 and FP32 launches of each class (the FP32 one is empty unless
+  // the build is mixed)
   for (int c = 0; c < kNumClasses; ++c) {
-    float t = 0.f;
-    LAI_HIP(hipEventElapsedTime(&t, ev[2 * c], ev[2 * c + 1]));
-    if (ms_fp64) ms_fp64[c] = mixed ? 0.0 : t;
-    if (ms_fp32) ms_fp32[c] = mixed ? t : 0.0;
+    float t64 = 0.f, t32 = 0.f;
+    LAI_HIP(hipEventElapsedTime(&t64, ev[4 * c], ev[4 * c + 1]));
+    LAI_HIP(hipEventElapsedTime(&t32, ev[4 * c + 2], ev[4 * c + 3]));
+    if (ms_fp64) ms_fp64[c] = t64;
+    if (ms_fp32) ms_fp32[c] = t32;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return LAI_OK;

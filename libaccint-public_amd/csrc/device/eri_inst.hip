@@ -37,8 +37,9 @@ int use_rows(K kern, size_t stat, size_t dyn, size_t budget) {
   return 1;
 }
 
-// MIXED = false: every quartet in FP64 (and the raw-ERI mode); true: the
-// quartets the worklist marks for FP32 run the recursion in float.
+// MIXED = false: the recursion in FP64 (and the raw-ERI mode); true: in
+// float, launched with QuartetSel::keep == 2 so that it takes only the
+// quartets the worklist marks for FP32.
 template <int ID, bool MIXED>
 void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);

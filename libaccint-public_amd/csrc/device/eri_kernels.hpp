@@ -51,22 +51,25 @@ struct QuartetSel {
   double thr;
   int mixed;                             // FP32 recursion where Q_ij Q_kl max|D| < tau
   double tau;
+  int keep;                              // 0: every selected quartet; 1: FP64 only; 2: FP32 only
 };
 
 // 0: removed by the density screen, 1: FP64, 2: FP32 recursion.  The same
 // arithmetic as the host worklist rules (exchange-type Dk over (ik, il, jk,
-// jl); the FP32 test adds D_ij and D_kl).
+// jl); the FP32 test adds D_ij and D_kl).  With keep set, quartets of the
+// other precision return 0: a mixed build runs each class as two launches,
+// one per precision.
 __device__ __forceinline__ int quartet_select(const QuartetSel& s, int bp, const PairRec& bra, int kp,
                                               const PairRec& ket) {
-  if (!s.dscreen && !s.mixed) return 1;
+  if (!s.dscreen && !s.mixed) return s.keep == 2 ? 0 : 1;
   const double prod = s.Qpair[bp] * s.Qpair[kp];
   const double* dm = s.sdmax + s.dm_off[bra.mol];
   const int ns = s.dm_ns[bra.mol];
   const int i = s.sh_loc[bra.sa], j = s.sh_loc[bra.sb], k = s.sh_loc[ket.sa], l = s.sh_loc[ket.sb];
   const double dk = fmax(fmax(dm[i * ns + k], dm[i * ns + l]), fmax(dm[j * ns + k], dm[j * ns + l]));
   if (s.dscreen && !(dk * prod >= s.thr)) return 0;
-  if (s.mixed && prod * fmax(dk, fmax(dm[i * ns + j], dm[k * ns + l])) < s.tau) return 2;
-  return 1;
+  const int mode = (s.mixed && prod * fmax(dk, fmax(dm[i * ns + j], dm[k * ns + l])) < s.tau) ? 2 : 1;
+  return (s.keep == 0 || s.keep == mode) ? mode : 0;
 }
 
 // Ket pair of position k of a chunk's run; seg walks forward with k.
@@ -809,8 +812,9 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
 // per chunk; the bra Coulomb block Jt[ab] is summed over the chunk as well.
 // Only the ket-side Coulomb update Jt[cd] goes to HBM per quartet.  With
 // klds == 0 (rows too long for LDS) the exchange updates go straight to HBM.
-// MIXED: quartets selected for FP32 run the recursion in float.  Lanes take
-// the chunk's quartets in the order of sort_run.
+// MIXED: every quartet of the launch runs the recursion in float (the launch
+// selects only the FP32 quartets, QuartetSel::keep == 2); otherwise FP64.
+// Lanes take the chunk's quartets in the order of sort_run.
 // --------------------------------------------------------------------------
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int A0, int A1>
 __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt) {
@@ -851,10 +855,9 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
     if (i >= nsel) continue;
     const unsigned e = srt.kp[i];
     const int kp = int(e & ~kSortF32);
-    const int mode = (e & kSortF32) ? 2 : 1;
     const PairRec ket = g.pairs[kp];
     double gv[NA][nb][nc][nd];
-    if (MIXED && mode == 2)
+    if constexpr (MIXED)
       quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
     else
       quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1>(g, bra, ket, false, gv);
@@ -1038,10 +1041,9 @@ __device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk&
       if (valid) {
         const unsigned e = srt.kp[i];
         const int kp = int(e & ~kSortF32);
-        const int mode = (e & kSortF32) ? 2 : 1;
         const PairRec ket = g.pairs[kp];
         double gv[NA][nb][nc][nd];
-        if (MIXED && mode == 2)
+        if constexpr (MIXED)
           quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
         else
           quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1>(g, bra, ket, false, gv);
@@ -1238,7 +1240,7 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
       const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
       if (mode == 0) continue;  // uniform over the workgroup
       double v[NJ][nd];
-      if (MIXED && mode == 2) {
+      if constexpr (MIXED) {
         float acc[NJ][nd];
         lds_quartet_acc<La, Lb, Lc, Ld, float, NT, NJ, NPQ>(g, bra, ket, false, reinterpret_cast<float*>(tab), ofs,
                                                             acc);
@@ -1510,7 +1512,7 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
       if (mode == 0) continue;
       const int ntot = bra.npp_sig * ket.npp_sig;
       double v[NJ][nd];
-      if (MIXED && mode == 2) {
+      if constexpr (MIXED) {
         float acc[NJ][nd];
         wave_quartet_acc<La, Lb, Lc, Ld, float, NJ, NPQ>(g, bra, ket, ntot, reinterpret_cast<float*>(tab[wave]), lane,
                                                          ofs, acc);
