@@ -950,14 +950,15 @@ extern "C" uint64_t lai_quartet_mix64(int i, int j, int k, int l) {
 
 namespace {
 // Chunk cap of the default rule for a class run in shape `shape` with n
-// quartets: lane-per-quartet kernels take up to 8 quartets per lane of their
-// workgroup (nt lanes; at most 2048), wavefront-per-quartet kernels up to 64,
-// workgroup-per-quartet kernels up to 32; small classes get shorter chunks so
-// that a launch still has a few workgroups per CU.
+// quartets: lane-per-quartet kernels take up to 2048 quartets (a workgroup of
+// nt to 512 lanes; the launch replicates the nt-lane unit), wavefront-per-
+// quartet kernels up to 64, workgroup-per-quartet kernels up to 32; small
+// classes get shorter chunks so that a launch still has a few workgroups per
+// CU.
 int default_chunk(int shape, size_t n, int n_cu, int nt = 256) {
   const size_t q = n / (4 * size_t(n_cu));
   if (shape == kShapeRegs) {
-    const size_t lo = size_t(nt), hi = std::min<size_t>(8 * size_t(nt), 2048);
+    const size_t lo = size_t(nt), hi = 2048;
     return int(std::clamp<size_t>(q * size_t(nt) / 256, lo, hi));
   }
   if (shape == kShapeWaves) return int(std::clamp<size_t>(q, 4, 64));

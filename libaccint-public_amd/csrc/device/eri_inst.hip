@@ -11,30 +11,63 @@
 namespace lai {
 namespace {
 
-// LDS a workgroup may use for the anchored exchange rows: register-shape
-// kernels keep several workgroups per CU; the LDS-shape kernels may take the
-// whole 160 KB of gfx950
-constexpr size_t kLdsBudget = 64 * 1024;
-constexpr size_t kLdsBudgetB = 160 * 1024;
+// LDS a workgroup may use (gfx950: 160 KB per CU)
+constexpr size_t kLdsBudget = 160 * 1024;
 
 // static LDS of a kernel as compiled (queried once per instantiation)
 template <typename K>
 size_t static_lds(K kern) {
   hipFuncAttributes at{};
-  if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(kern)) != hipSuccess) return kLdsBudgetB;
+  if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(kern)) != hipSuccess) return kLdsBudget;
   return at.sharedSizeBytes;
 }
 
-// Exchange rows in LDS when static + dynamic LDS fit the budget; above 64 KB
-// the kernel is allowed the larger dynamic size first.
+// Dynamic LDS the kernel may be launched with: all of the budget beyond its
+// static LDS when the runtime allows more than the 64 KB default.
 template <typename K>
-int use_rows(K kern, size_t stat, size_t dyn, size_t budget) {
-  if (stat + dyn > budget) return 0;
-  if (stat + dyn > 64 * 1024 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          int(dyn)) != hipSuccess)
-    return 0;
-  return 1;
+size_t dyn_limit(K kern, size_t stat) {
+  const size_t want = stat < kLdsBudget ? kLdsBudget - stat : 0;
+  if (want > 64 * 1024 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess)
+    return stat < 64 * 1024 ? 64 * 1024 - stat : 0;
+  return want;
+}
+
+// Workgroup of a lane-per-quartet kernel: rep replicas of `base` lanes (at
+// most 512 lanes) share one set of the bra's exchange rows (`rows` bytes of
+// dynamic LDS) and carry `per_rep` bytes each.  Picks the rep that keeps the
+// most wavefronts resident per CU, LDS and registers counted by the runtime's
+// occupancy calculator; the rows stay in LDS whenever they fit at all (without
+// them every exchange update is an HBM atomic).
+struct RowsPick {
+  int rep = 1;
+  int rows = 0;    // 1: exchange rows in LDS
+  size_t dyn = 0;  // dynamic LDS bytes
+};
+template <typename K>
+RowsPick pick_rows(K kern, size_t dmax, int base, size_t per_rep, size_t rows) {
+  RowsPick p;
+  p.dyn = per_rep;
+  int best = -1;
+  for (int pass = 0; pass < 2 && best < 0; ++pass) {  // with the rows, else without
+    const size_t rb = pass == 0 ? rows : 0;
+    for (int r = 1; base * r <= 512; r *= 2) {
+      const size_t dyn = size_t(r) * per_rep + rb;
+      if (dyn > dmax) break;
+      int blocks = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern), base * r,
+                                                       dyn) != hipSuccess)
+        blocks = 0;
+      const int waves = blocks * base * r / 64;
+      if (blocks > 0 && waves > best) {
+        best = waves;
+        p.rep = r;
+        p.rows = pass == 0 ? 1 : 0;
+        p.dyn = dyn;
+      }
+    }
+  }
+  return p;
 }
 
 // MIXED = false: the recursion in FP64 (and the raw-ERI mode); true: in
@@ -71,33 +104,34 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
   };
   if (shape == kShapeRegs) {
     if constexpr (P::regs && P::NCH > 1) {
-      // bra cut into NCH slices: one wavefront per slice, all rows in LDS
+      // bra cut into NCH slices: groups of NCH wavefronts, one per slice
       auto kern = jk_kernel_merged<La, Lb, Lc, Ld, MIXED, P::CHA>;
       static const size_t stat = static_lds(kern);
-      const size_t dyn = size_t(na + nb) * nbf * sizeof(double);
-      a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);
-      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuRegs)), dim3(64 * P::NCH), a.klds ? dyn : 0, s, a);
+      static const size_t dmax = dyn_limit(kern, stat);
+      const RowsPick rp = pick_rows(kern, dmax, 64 * P::NCH, size_t(P::NCH) * nc * nd * 64 * sizeof(double),
+                                    size_t(na + nb) * nbf * sizeof(double));
+      a.klds = rp.rows;
+      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuRegs)), dim3(64 * P::NCH * rp.rep), rp.dyn, s, a);
     } else if constexpr (P::regs) {
-      const size_t dyn = size_t(P::CHA + nb) * nbf * sizeof(double);
-      a.klds = use_rows(jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>, 0, dyn, kLdsBudget);
+      auto kern = jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>;
+      static const size_t stat = static_lds(kern);
+      static const size_t dmax = dyn_limit(kern, stat);
+      const RowsPick rp = pick_rows(kern, dmax, 256, 0, size_t(P::CHA + nb) * nbf * sizeof(double));
+      a.klds = rp.rows;
       dim3 grid(grid_cap(kBlocksPerCuRegs), P::NCH);
-      hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>), grid, dim3(256), a.klds ? dyn : 0, s, a);
+      hipLaunchKernelGGL(kern, grid, dim3(256 * rp.rep), rp.dyn, s, a);
     }
   } else if (shape == kShapeWaves) {
     if constexpr (P::waves || P::alt == kShapeWaves) {
-      auto kern = jk_kernel_waves<La, Lb, Lc, Ld, MIXED, P::NW>;
-      static const size_t stat = static_lds(kern);
-      const size_t dyn = (size_t(na + nb) * nbf + na * nb) * sizeof(double);
-      a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
-      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuLds)), dim3(64 * P::NW), a.klds ? dyn : 0, s, a);
+      a.klds = 0;
+      hipLaunchKernelGGL((jk_kernel_waves<La, Lb, Lc, Ld, MIXED, P::NW>), dim3(grid_cap(kBlocksPerCuLds)),
+                         dim3(64 * P::NW), 0, s, a);
     }
   } else {
     if constexpr (!P::regs) {
-      auto kern = jk_kernel_lds<La, Lb, Lc, Ld, MIXED, P::NT>;
-      static const size_t stat = static_lds(kern);
-      const size_t dyn = (size_t(na + nb) * nbf + nc * nd + na * nb) * sizeof(double);
-      a.klds = use_rows(kern, stat, dyn, kLdsBudgetB);  // else per-quartet blocks to HBM
-      hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuLds)), dim3(P::NT), a.klds ? dyn : 0, s, a);
+      a.klds = 0;
+      hipLaunchKernelGGL((jk_kernel_lds<La, Lb, Lc, Ld, MIXED, P::NT>), dim3(grid_cap(kBlocksPerCuLds)),
+                         dim3(P::NT), 0, s, a);
     }
   }
 }

@@ -94,21 +94,21 @@ struct SortLds {
 };
 
 // Sorts positions [s0, s1) (at most kSortRun) of a chunk's run into s.kp and
-// returns how many were kept.  All NT lanes of the workgroup call it (NT a
-// multiple of 64, at least 64); it ends on a barrier.
-template <int NT = 256>
+// returns how many were kept.  All nt lanes of the workgroup call it (nt a
+// multiple of 64, at least NTMIN); it ends on a barrier.
+template <int NTMIN>
 __device__ __forceinline__ int sort_run(const int* __restrict__ kets, const JkSegs* __restrict__ sg,
                                         const PairRec* __restrict__ pairs, const QuartetSel& sel, int bp,
-                                        const PairRec& bra, int s0, int s1, SortLds& s) {
-  constexpr int kSortPer = (kSortRun + NT - 1) / NT;  // positions per lane
+                                        const PairRec& bra, int s0, int s1, SortLds& s, int nt) {
+  constexpr int kSortPer = (kSortRun + NTMIN - 1) / NTMIN;  // positions per lane, at most
   const int tid = threadIdx.x;
-  for (int i = tid; i < kSortBins; i += NT) s.cnt[i] = 0;
+  for (int i = tid; i < kSortBins; i += nt) s.cnt[i] = 0;
   __syncthreads();
   int kp[kSortPer], key[kSortPer], rk[kSortPer];
   int seg = 0;
 #pragma unroll
   for (int j = 0; j < kSortPer; ++j) {
-    const int k = s0 + tid + NT * j;
+    const int k = s0 + tid + nt * j;
     key[j] = -1;
     kp[j] = 0;
     rk[j] = 0;
@@ -152,12 +152,13 @@ __device__ __forceinline__ int sort_run(const int* __restrict__ kets, const JkSe
   return s.n;
 }
 
-// Sorted position of lane tid in round r: the four wavefronts take the four
-// 64-entry blocks of a round in an order that rotates with r, so that no
-// wavefront always gets the heaviest block.
-__device__ __forceinline__ int sorted_pos(int tid, int r) {
+// Sorted position of lane tid in round r of a workgroup of nw wavefronts (a
+// power of two): the wavefronts take the nw 64-entry blocks of a round in an
+// order that rotates with r, so that no wavefront always gets the heaviest
+// block.
+__device__ __forceinline__ int sorted_pos(int tid, int r, int nw) {
   const int w = tid >> 6;
-  return 64 * (4 * r + ((w + r) & 3)) + (tid & 63);
+  return 64 * (nw * r + ((w + r) & (nw - 1))) + (tid & 63);
 }
 
 struct EriArgs {
@@ -849,9 +850,9 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
     for (int b = 0; b < nb; ++b) jab[a][b] = 0.0;
 
   for (int s0 = ch.k0; s0 < ch.k1; s0 += kSortRun) {
-  const int nsel = sort_run(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt);
-  for (int r = 0; 256 * r < nsel; ++r) {
-    const int i = sorted_pos(tid, r);
+  const int nsel = sort_run<256>(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt, nt);
+  for (int r = 0; nt * r < nsel; ++r) {
+    const int i = sorted_pos(tid, r, nt >> 6);
     if (i >= nsel) continue;
     const unsigned e = srt.kp[i];
     const int kp = int(e & ~kSortF32);
@@ -977,9 +978,11 @@ __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChun
   }
 }
 
-// Kernel A, anchored: 256 lanes per chunk, bra index chunk C = blockIdx.y.
+// Kernel A, anchored: 256 or 512 lanes per chunk (the launch picks the
+// workgroup that keeps the most wavefronts resident with the exchange rows in
+// LDS), bra index chunk C = blockIdx.y.
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
-__global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
+__global__ __launch_bounds__(512) void jk_kernel_regs(EriArgs g) {
   extern __shared__ double krow[];
   __shared__ SortLds srt;
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
@@ -989,22 +992,26 @@ __global__ __launch_bounds__(256) void jk_kernel_regs(EriArgs g) {
 }
 
 // Kernel A with merged bra slices: for classes whose bra index is cut into
-// NCH slices (to fit the registers), one workgroup of NCH wavefronts takes a
-// chunk, wavefront C computing slice C of the same sorted quartets in step.
-// The exchange rows of A and B, the sort and the chunk's loads are shared; the
-// slices' partial ket Coulomb blocks Jt[cd] meet in LDS and go to HBM with one
-// atomic per element instead of one per slice.
+// NCH slices (to fit the registers), a group of NCH wavefronts takes 64
+// quartets of a chunk at a time, wavefront C of the group computing slice C
+// of the same sorted quartets in step.  A workgroup holds rep such groups
+// (rep * NCH wavefronts, at most 512 lanes; the launch picks rep), which take
+// consecutive 64-quartet blocks of the sorted run.  The exchange rows of A
+// and B, the sort and the chunk's loads are shared by the whole workgroup;
+// the slices' partial ket Coulomb blocks Jt[cd] meet in LDS and go to HBM
+// with one atomic per element instead of one per slice.
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
 __device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
                                                 double* jpart) {
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NCH = (na + CHA - 1) / CHA;
-  constexpr int NT = 64 * NCH;
   constexpr int A0 = C * CHA, A1 = (A0 + CHA < na) ? A0 + CHA : na;
   constexpr int NA = A1 - A0;
   constexpr int NCD = nc * nd;
   constexpr int ROWS = na + nb;
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63, NT = blockDim.x;
+  const int rep = NT / (64 * NCH), grp = (tid >> 6) / NCH;
+  jpart += grp * (NCH * NCD * 64);  // this group's slices
   const bool lds = g.klds != 0;
   const PairRec bra = g.pairs[ch.bra];
   const int n = bra.mn;
@@ -1032,9 +1039,10 @@ __device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk&
     for (int b = 0; b < nb; ++b) jab[a][b] = 0.0;
 
   for (int s0 = ch.k0; s0 < ch.k1; s0 += kSortRun) {
-    const int nsel = sort_run<NT>(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt);
-    for (int r = 0; 64 * r < nsel; ++r) {
-      const int i = 64 * r + lane;
+    const int nsel =
+        sort_run<64 * NCH>(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt, NT);
+    for (int r = 0; 64 * rep * r < nsel; ++r) {
+      const int i = 64 * (rep * r + grp) + lane;
       const bool valid = i < nsel;
       int fc = 0, fd = 0;
       bool bk = true;
@@ -1174,46 +1182,46 @@ __device__ __forceinline__ void dispatch_merged(const EriArgs& g, const JkChunk&
   }
 }
 
-// Kernel A, merged slices: 64 * NCH lanes per chunk, wavefront w = slice w.
-// Dynamic LDS: the exchange rows [na + nb][mn]; static: the sort and the
-// slices' ket Coulomb parts.
+// Kernel A, merged slices: rep groups of NCH wavefronts per chunk, wavefront
+// w = slice w mod NCH of group w / NCH.  Dynamic LDS: the groups' ket Coulomb
+// parts [rep][NCH][nc*nd][64], then the exchange rows [na + nb][mn]; static:
+// the sort.
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
-__global__ __launch_bounds__(64 * ((ncart(La) + CHA - 1) / CHA)) void jk_kernel_merged(EriArgs g) {
+__global__ __launch_bounds__(512) void jk_kernel_merged(EriArgs g) {
   constexpr int NCH = (ncart(La) + CHA - 1) / CHA;
-  extern __shared__ double krow[];
+  extern __shared__ double dyn[];
   __shared__ SortLds srt;
-  __shared__ double jpart[NCH * ncart(Lc) * ncart(Ld) * 64];
-  const int w = threadIdx.x >> 6;  // uniform over the wavefront
+  double* jpart = dyn;
+  double* krow = dyn + (blockDim.x / 64) * (ncart(Lc) * ncart(Ld) * 64);
+  const int w = (threadIdx.x >> 6) % NCH;  // uniform over the wavefront
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
     const JkChunk ch = g.chunks[c];
     dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, srt, jpart, w);
   }
 }
 
-// Kernel B, anchored: one workgroup per chunk, the chunk's quartets one after
-// another.  Dynamic LDS: the exchange rows [na + nb][mn], then the ket
-// Coulomb block [nc*nd] of the current quartet and the bra block [na*nb].
+// Kernel B: one workgroup per chunk, the chunk's quartets one after another.
+// Each quartet's ket Coulomb and exchange blocks go to HBM with one atomic per
+// element; the bra Coulomb block Jt[ab] is summed in LDS over the chunk.
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int NT>
 __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   using Sh = LdsShape<La, Lb, Lc, Ld, double>;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
   constexpr int NJ = (NTASK + NT - 1) / NT;
-  constexpr int ROWS = na + nb;
   constexpr int NPQ = Sh::template npq<NT>();
   // FP64 tables; the FP32 recursion uses the same storage as float
   __shared__ double tab[NPQ * Sh::TSP];
-  // without LDS rows: the quartet's six J/K blocks, summed in LDS and added to
-  // HBM once per quartet (Jt[ab], Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd])
+  // the quartet's six J/K blocks, summed in LDS and added to HBM once per
+  // quartet (Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd]) or once per chunk (Jt[ab])
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
                 OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
   __shared__ double blk[NBLK];
   __shared__ double dblk[NBLK];  // the quartet's density blocks D[xy], laid out like blk
-  extern __shared__ double dyn[];
   const int tid = threadIdx.x;
-  const bool lds = g.klds != 0;
   int ofs[NJ];
   lds_component_offsets<La, Lb, Lc, Ld, NT, NJ, false>(tid, ofs);
+  for (int e = tid; e < NBLK; e += NT) blk[e] = 0.0;  // ordered by the barriers of lds_quartet_acc
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
@@ -1222,17 +1230,9 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
     const double* __restrict__ D = g.D + bra.moff;
     double* Jt = g.Jt + bra.moff;
     double* Kt = g.Kt + bra.moff;
-    double* krow = dyn;
-    double* jcd = dyn + ROWS * n;
-    double* jab = jcd + nc * nd;
     const int fa = bra.fa, fb = bra.fb;
     const bool ij = bra.same;
     const JkSegs* sg = g.segs + ch.seg;
-    if (lds) {
-      for (int e = tid; e < ROWS * n + nc * nd + na * nb; e += NT) dyn[e] = 0.0;
-    } else {
-      for (int e = tid; e < NBLK; e += NT) blk[e] = 0.0;
-    }
     int seg = 0;
     for (int k = ch.k0; k < ch.k1; ++k) {
       const int kp = chunk_ket(g.kets, sg, k, seg);
@@ -1292,81 +1292,54 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
           s3 += x[d] * dblk[OAD + a * nd + d];
         }
         const double dab = dblk[OAB + a * nb + b], dbc = dblk[OBC + b * nc + c], dac = dblk[OAC + a * nc + c];
-        if (lds) {
-          lds_add(&jab[a * nb + b], fJ * sj);
-          lds_add(&krow[a * n + fc + c], fK * s1);
-          if (!ij) lds_add(&krow[(na + b) * n + fc + c], fK * s3);
-        } else {
-          lds_add(&blk[OAB + a * nb + b], fJ * sj);
-          lds_add(&blk[OAC + a * nc + c], fK * s1);
-          if (!ij) lds_add(&blk[OBC + b * nc + c], fK * s3);
-        }
+        lds_add(&blk[OAB + a * nb + b], fJ * sj);
+        lds_add(&blk[OAC + a * nc + c], fK * s1);
+        if (!ij) lds_add(&blk[OBC + b * nc + c], fK * s3);
 #pragma unroll
         for (int d = 0; d < nd; ++d) {
-          if (lds) {
-            if (!bk) lds_add(&jcd[c * nd + d], fJ * x[d] * dab);
-            if (!kl) lds_add(&krow[a * n + fd + d], fK * x[d] * dbc);
-            if (!ij && !kl) lds_add(&krow[(na + b) * n + fd + d], fK * x[d] * dac);
-          } else {
-            if (!bk) lds_add(&blk[OCD + c * nd + d], fJ * x[d] * dab);
-            if (!kl) lds_add(&blk[OAD + a * nd + d], fK * x[d] * dbc);
-            if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], fK * x[d] * dac);
-          }
+          if (!bk) lds_add(&blk[OCD + c * nd + d], fJ * x[d] * dab);
+          if (!kl) lds_add(&blk[OAD + a * nd + d], fK * x[d] * dbc);
+          if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], fK * x[d] * dac);
         }
       }
-      if (!lds) {  // the quartet's blocks to HBM, one atomic per element
-        __syncthreads();
-        for (int e = tid; e < NBLK; e += NT) {
-          const double x = blk[e];
-          blk[e] = 0.0;
-          if (x == 0.0) continue;
-          size_t r, q;
-          double* M = Kt;
-          if (e < OCD) { r = fa + (e - OAB) / nb; q = fb + (e - OAB) % nb; M = Jt; }
-          else if (e < OAC) { r = fc + (e - OCD) / nd; q = fd + (e - OCD) % nd; M = Jt; }
-          else if (e < OAD) { r = fa + (e - OAC) / nc; q = fc + (e - OAC) % nc; }
-          else if (e < OBC) { r = fa + (e - OAD) / nd; q = fd + (e - OAD) % nd; }
-          else if (e < OBD) { r = fb + (e - OBC) / nc; q = fc + (e - OBC) % nc; }
-          else { r = fb + (e - OBD) / nd; q = fd + (e - OBD) % nd; }
-          atomic_add(&M[r * n + q], x);
-        }
-        __syncthreads();
-      }
-      if (lds && !bk) {  // this quartet's ket Coulomb block to HBM
-        __syncthreads();
-        for (int e = tid; e < nc * nd; e += NT) {
-          const double x = jcd[e];
-          jcd[e] = 0.0;
-          if (x != 0.0) atomic_add(&Jt[size_t(fc + e / nd) * n + fd + e % nd], x);
-        }
-        __syncthreads();
-      }
-    }
-    if (lds) {
       __syncthreads();
-      for (int e = tid; e < ROWS * n; e += NT) {
-        const double x = krow[e];
-        if (x != 0.0) {
-          const int row = e / n, col = e - row * n;
-          atomic_add(&Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], x);
-        }
+      // the quartet's ket and exchange blocks to HBM, one atomic per element
+      for (int e = OCD + tid; e < NBLK; e += NT) {
+        const double x = blk[e];
+        if (x == 0.0) continue;
+        blk[e] = 0.0;
+        size_t r, q;
+        double* M = Kt;
+        if (e < OAC) { r = fc + (e - OCD) / nd; q = fd + (e - OCD) % nd; M = Jt; }
+        else if (e < OAD) { r = fa + (e - OAC) / nc; q = fc + (e - OAC) % nc; }
+        else if (e < OBC) { r = fa + (e - OAD) / nd; q = fd + (e - OAD) % nd; }
+        else if (e < OBD) { r = fb + (e - OBC) / nc; q = fc + (e - OBC) % nc; }
+        else { r = fb + (e - OBD) / nd; q = fd + (e - OBD) % nd; }
+        atomic_add(&M[r * n + q], x);
       }
-      for (int e = tid; e < na * nb; e += NT) {
-        const double x = jab[e];
-        if (x != 0.0) atomic_add(&Jt[size_t(fa + e / nb) * n + fb + e % nb], x);
-      }
-      __syncthreads();  // dyn is cleared for the next chunk
+      __syncthreads();
     }
+    // the chunk's bra Coulomb block
+    for (int e = tid; e < OCD; e += NT) {
+      const double x = blk[e];
+      if (x == 0.0) continue;
+      blk[e] = 0.0;
+      atomic_add(&Jt[size_t(fa + e / nb) * n + fb + e % nb], x);
+    }
+    __syncthreads();
   }
 }
 
 // --------------------------------------------------------------------------
-// Kernel W: NW wavefronts per workgroup, one quartet per wavefront.  The
-// quartets of a chunk share the bra, so the waves share its exchange rows and
-// bra Coulomb block in LDS; each wave builds its own (primitive quartet, root,
-// axis) tables and accumulates its (a, b, c) tasks.  The waves run
-// independently (each its own primitive count, wavefront-level ordering of
-// its own LDS); workgroup barriers only bracket a chunk's shared rows.
+// Kernel W: NW wavefronts per workgroup, one quartet per wavefront.  Each
+// wave takes every NW-th quartet of the chunk, builds its own (primitive
+// quartet, root, axis) tables and accumulates its (a, b, c) tasks; the
+// quartet's ket Coulomb and exchange blocks go to HBM with one atomic per
+// element, the bra Coulomb block Jt[ab] once per chunk.  The waves run
+// independently: no workgroup barrier, wavefront-level ordering of the
+// wave's own LDS.  (Keeping the bra's exchange rows in LDS, as kernel A
+// does, costs more in occupancy than it saves in atomics for these classes:
+// measured 1.1-1.7x slower on 400-1400 basis functions.)
 // --------------------------------------------------------------------------
 template <int La, int Lb, int Lc, int Ld, typename R, int NJ, int NPQ>
 __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec& bra, const PairRec& ket, int ntot,
@@ -1470,22 +1443,20 @@ struct WaveShape {
 
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int NW>
 __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
-  using Sh = LdsShape<La, Lb, Lc, Ld, double>;
   using Ws = WaveShape<La, Lb, Lc, Ld>;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NTASK = Ws::NTASK, NJ = Ws::NJ, NPQ = Ws::NPQ;
-  constexpr int NT = 64 * NW;
-  constexpr int ROWS = na + nb;
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
                 OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
   __shared__ double tab[NW][NPQ * Ws::TS];
-  __shared__ double blk[NW][NBLK];
-  __shared__ double dblk[NW][NBLK];  // per wave: the quartet's density blocks, laid out like blk  // per wave: the quartet's J/K blocks (jcd only, with LDS rows)
-  extern __shared__ double dyn[];   // exchange rows [na + nb][mn], then the bra block [na*nb]
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
-  const bool lds = g.klds != 0;
+  __shared__ double blk[NW][NBLK];   // per wave: the quartet's J/K blocks; Jt[ab] summed over the chunk
+  __shared__ double dblk[NW][NBLK];  // per wave: the quartet's density blocks, laid out like blk
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int ofs[NJ];
   lds_component_offsets<La, Lb, Lc, Ld, 64, NJ, false>(lane, ofs);
+  double* bw = blk[wave];
+  double* dw = dblk[wave];
+  for (int o = lane; o < NBLK; o += 64) bw[o] = 0.0;  // ordered by the first wave_sync below
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
@@ -1494,16 +1465,9 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
     const double* __restrict__ D = g.D + bra.moff;
     double* Jt = g.Jt + bra.moff;
     double* Kt = g.Kt + bra.moff;
-    double* krow = dyn;
-    double* jab = dyn + ROWS * n;
     const int fa = bra.fa, fb = bra.fb;
     const bool ij = bra.same;
     const JkSegs* sg = g.segs + ch.seg;
-    if (lds)
-      for (int e = tid; e < ROWS * n + na * nb; e += NT) dyn[e] = 0.0;
-    for (int e = tid; e < NW * NBLK; e += NT) (&blk[0][0])[e] = 0.0;
-    __syncthreads();  // rows and blocks are clear
-    double* bw = blk[wave];
     int seg = 0;
     for (int k = ch.k0 + wave; k < ch.k1; k += NW) {  // uniform over the wavefront
       const int kp = chunk_ket(g.kets, sg, k, seg);
@@ -1534,7 +1498,6 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
       const double fK = bk ? 1.0 : 2.0;
       // the quartet's six density blocks D[xy], staged in this wave's LDS at
       // the offsets of the J/K blocks of the same shape (Oxy)
-      double* dw = dblk[wave];
       for (int o = lane; o < NBLK; o += 64) {
         size_t r, q;
         if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; }
@@ -1564,60 +1527,42 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
           s3 += x[d] * dw[OAD + a * nd + d];
         }
         const double dab = dw[OAB + a * nb + b], dbc = dw[OBC + b * nc + c], dac = dw[OAC + a * nc + c];
-        if (lds) {
-          lds_add(&jab[a * nb + b], fJ * sj);
-          lds_add(&krow[a * n + fc + c], fK * s1);
-          if (!ij) lds_add(&krow[(na + b) * n + fc + c], fK * s3);
-        } else {
-          lds_add(&bw[OAB + a * nb + b], fJ * sj);
-          lds_add(&bw[OAC + a * nc + c], fK * s1);
-          if (!ij) lds_add(&bw[OBC + b * nc + c], fK * s3);
-        }
+        lds_add(&bw[OAB + a * nb + b], fJ * sj);
+        lds_add(&bw[OAC + a * nc + c], fK * s1);
+        if (!ij) lds_add(&bw[OBC + b * nc + c], fK * s3);
 #pragma unroll
         for (int d = 0; d < nd; ++d) {
           if (!bk) lds_add(&bw[OCD + c * nd + d], fJ * x[d] * dab);
-          if (lds) {
-            if (!kl) lds_add(&krow[a * n + fd + d], fK * x[d] * dbc);
-            if (!ij && !kl) lds_add(&krow[(na + b) * n + fd + d], fK * x[d] * dac);
-          } else {
-            if (!kl) lds_add(&bw[OAD + a * nd + d], fK * x[d] * dbc);
-            if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], fK * x[d] * dac);
-          }
+          if (!kl) lds_add(&bw[OAD + a * nd + d], fK * x[d] * dbc);
+          if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], fK * x[d] * dac);
         }
       }
       wave_sync();  // the wave's quartet block is complete
-      // this wave's quartet block to HBM, one atomic per element
-      for (int o = lane; o < NBLK; o += 64) {
+      // this quartet's ket and exchange blocks to HBM, one atomic per element;
+      // Jt[ab] stays in LDS for the rest of the chunk
+      for (int o = OCD + lane; o < NBLK; o += 64) {
         const double x = bw[o];
         if (x == 0.0) continue;
         bw[o] = 0.0;
         size_t r, q;
         double* M = Kt;
-        if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; M = Jt; }
-        else if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; M = Jt; }
+        if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; M = Jt; }
         else if (o < OAD) { r = fa + (o - OAC) / nc; q = fc + (o - OAC) % nc; }
         else if (o < OBC) { r = fa + (o - OAD) / nd; q = fd + (o - OAD) % nd; }
         else if (o < OBD) { r = fb + (o - OBC) / nc; q = fc + (o - OBC) % nc; }
         else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
         atomic_add(&M[r * n + q], x);
       }
-      wave_sync();  // the block is clear for the wave's next quartet
+      wave_sync();  // the blocks are clear for the wave's next quartet
     }
-    __syncthreads();  // every wave has added its quartets to the rows
-    if (lds) {
-      for (int e = tid; e < ROWS * n; e += NT) {
-        const double x = krow[e];
-        if (x != 0.0) {
-          const int row = e / n, col = e - row * n;
-          atomic_add(&Kt[size_t(row < na ? fa + row : fb + row - na) * n + col], x);
-        }
-      }
-      for (int e = tid; e < na * nb; e += NT) {
-        const double x = jab[e];
-        if (x != 0.0) atomic_add(&Jt[size_t(fa + e / nb) * n + fb + e % nb], x);
-      }
+    // the wave's share of the chunk's bra Coulomb block
+    for (int o = lane; o < OCD; o += 64) {
+      const double x = bw[o];
+      if (x == 0.0) continue;
+      bw[o] = 0.0;
+      atomic_add(&Jt[size_t(fa + o / nb) * n + fb + o % nb], x);
     }
-    __syncthreads();  // dyn and the blocks are cleared for the next chunk
+    wave_sync();
   }
 }
 

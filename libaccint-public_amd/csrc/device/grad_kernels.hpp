@@ -377,9 +377,9 @@ __global__ __launch_bounds__(256) void grad_kernel_regs(GradArgs g) {
     double fab[6] = {0, 0, 0, 0, 0, 0};
     const JkSegs* sg = g.segs + ch.seg;
     for (int s0 = ch.k0; s0 < ch.k1; s0 += kSortRun) {
-    const int nsel = sort_run(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt);
+    const int nsel = sort_run<256>(g.kets, sg, g.pairs, g.sel, ch.bra, bra, s0, min(ch.k1, s0 + kSortRun), srt, 256);
     for (int r = 0; 256 * r < nsel; ++r) {
-      const int i = sorted_pos(tid, r);
+      const int i = sorted_pos(tid, r, 4);
       if (i >= nsel) continue;
       const int kp = int(srt.kp[i] & ~kSortF32);
       const PairRec ket = g.pairs[kp];

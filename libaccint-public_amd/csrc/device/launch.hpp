@@ -33,7 +33,7 @@ struct ClassInfo {
   int La, Lb, Lc, Ld;
   int regs;    // 1: register shape (Kernel A), 0: LDS shape (Kernel B)
   int nchunk;  // Kernel A chunks along the bra index
-  int nt;      // lanes per workgroup
+  int nt;      // lanes per workgroup (lane-per-quartet shape: the unit a launch replicates, up to 512 lanes)
   int waves;   // LDS shape with one quartet per wavefront: wavefronts per workgroup, else 0
   int shape;   // primary shape (kShape*)
   int alt;     // alternative shape compiled for the autotuner, kShapeNone if none
