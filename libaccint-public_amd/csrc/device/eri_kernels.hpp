@@ -1210,8 +1210,12 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
   constexpr int NJ = (NTASK + NT - 1) / NT;
   constexpr int NPQ = Sh::template npq<NT>();
-  // FP64 tables; the FP32 recursion uses the same storage as float
+  // FP64 tables (the FP32 recursion uses the same storage as float); where
+  // they are large enough, then the quartet's normalised integrals, so that
+  // the digestion takes one task at a time (fewer registers)
   __shared__ double tab[NPQ * Sh::TSP];
+  constexpr bool XLDS = NJ * nd * NT <= NPQ * Sh::TSP;
+  constexpr int XUNROLL = XLDS ? 1 : NJ;
   // the quartet's six J/K blocks, summed in LDS and added to HBM once per
   // quartet (Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd]) or once per chunk (Jt[ab])
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
@@ -1221,6 +1225,14 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   const int tid = threadIdx.x;
   int ofs[NJ];
   lds_component_offsets<La, Lb, Lc, Ld, NT, NJ, false>(tid, ofs);
+  // normalisation of this lane's (a, b, c) tasks, computed once
+  double sabc[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int t = min(tid + NT * j, NTASK - 1);
+    const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+    sabc[j] = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
+  }
   for (int e = tid; e < NBLK; e += NT) blk[e] = 0.0;  // ordered by the barriers of lds_quartet_acc
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
@@ -1273,16 +1285,28 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
         else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
         dblk[o] = D[r * n + q];
       }
-      __syncthreads();
+      // the normalised integrals of this lane's tasks through the table
+      // storage, free after the products
+      if constexpr (XLDS) {
 #pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int d = 0; d < nd; ++d) tab[(j * nd + d) * NT + tid] = v[j][d] * sabc[j] * norm_corr(Ld, d);
+      }
+      __syncthreads();
+#pragma unroll XUNROLL
       for (int j = 0; j < NJ; ++j) {
         const int t = tid + NT * j;
         if (t >= NTASK) continue;
         const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-        const double sabc = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
         double x[nd];
 #pragma unroll
-        for (int d = 0; d < nd; ++d) x[d] = v[j][d] * sabc * norm_corr(Ld, d);
+        for (int d = 0; d < nd; ++d) {
+          if constexpr (XLDS)
+            x[d] = tab[(j * nd + d) * NT + tid];
+          else
+            x[d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
+        }
         // per-task sums over d: J[ab] (D[cd]), K[ac] (D[bd]), K[bc] (D[ad])
         double sj = 0.0, s1 = 0.0, s3 = 0.0;
 #pragma unroll
@@ -1448,12 +1472,23 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
   constexpr int NTASK = Ws::NTASK, NJ = Ws::NJ, NPQ = Ws::NPQ;
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
                 OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
-  __shared__ double tab[NW][NPQ * Ws::TS];
+  // per wave: the (primitive quartet, root, axis) tables, then the quartet's
+  // normalised integrals for the digestion
+  constexpr int TABN = NPQ * Ws::TS > NJ * nd * 64 ? NPQ * Ws::TS : NJ * nd * 64;
+  __shared__ double tab[NW][TABN];
   __shared__ double blk[NW][NBLK];   // per wave: the quartet's J/K blocks; Jt[ab] summed over the chunk
   __shared__ double dblk[NW][NBLK];  // per wave: the quartet's density blocks, laid out like blk
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int ofs[NJ];
   lds_component_offsets<La, Lb, Lc, Ld, 64, NJ, false>(lane, ofs);
+  // normalisation of this lane's (a, b, c) tasks, computed once
+  double sabc[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int t = min(lane + 64 * j, NTASK - 1);
+    const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+    sabc[j] = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
+  }
   double* bw = blk[wave];
   double* dw = dblk[wave];
   for (int o = lane; o < NBLK; o += 64) bw[o] = 0.0;  // ordered by the first wave_sync below
@@ -1508,16 +1543,23 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
         else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
         dw[o] = D[r * n + q];
       }
-      wave_sync();
+      // the normalised integrals of this lane's tasks, through the wave's
+      // table storage (free after the products), so that the digestion can
+      // take one task at a time
+      double* xw = tab[wave];
 #pragma unroll
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) xw[(j * nd + d) * 64 + lane] = v[j][d] * sabc[j] * norm_corr(Ld, d);
+      wave_sync();
+#pragma unroll 1
       for (int j = 0; j < NJ; ++j) {
         const int t = lane + 64 * j;
         if (t >= NTASK) continue;
         const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-        const double sabc = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
         double x[nd];
 #pragma unroll
-        for (int d = 0; d < nd; ++d) x[d] = v[j][d] * sabc * norm_corr(Ld, d);
+        for (int d = 0; d < nd; ++d) x[d] = xw[(j * nd + d) * 64 + lane];
         // per-task sums over d: J[ab] (D[cd]), K[ac] (D[bd]), K[bc] (D[ad])
         double sj = 0.0, s1 = 0.0, s3 = 0.0;
 #pragma unroll
