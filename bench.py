@@ -210,6 +210,10 @@ def main():
         engine.dispatch_table().to_json(args.save_table)
     ms64, wall64 = timed_graph(opts, args.steps, args.warmup)
     log("FP64 J+K: %.3f ms/step (events), wall %.3f s for %d steps" % (ms64, wall64, args.steps))
+    # the same build with the register-shape classes as one persistent launch
+    # over a device queue of their chunks
+    ms64p, _ = timed_graph(lai.JKOptions(schwarz_threshold=1e-12, persistent=True), args.steps, args.warmup)
+    log("FP64 J+K, persistent register-shape launch: %.3f ms/step" % ms64p)
 
     # per-class profile of the same build (HIP events around each class kernel)
     f64c = np.zeros(55)
@@ -409,6 +413,7 @@ def main():
             be.autotune(tbD.data_ptr(), bo, stream.cuda_stream, 1)
             bt = time.perf_counter() - t0
         ms_b = b_timed(bo, bsteps, "FP64")
+        ms_bp = b_timed(lai.JKOptions(schwarz_threshold=1e-12, persistent=True), bsteps, "FP64 persistent")
         bm = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
         be.prepare_worklist(bm, bD)
         bminfo = be.worklist_info()
@@ -435,6 +440,7 @@ def main():
                                     min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
                  "batch_setup_seconds": bsetup, "batch_autotune_seconds": bt,
                  "batch_jk_seconds_fp64": ms_b * 1e-3, "batch_eris_per_s_fp64": binfo["n_eri"] / (ms_b * 1e-3),
+                 "batch_jk_seconds_fp64_persistent_launch": ms_bp * 1e-3,
                  "batch_jk_seconds_mixed": ms_bm * 1e-3,
                  "batch_mixed_fp32_quartet_fraction": bminfo["n_fp32"] / max(bminfo["n_quartets"], 1),
                  "batch_one_engine_per_molecule_seconds_fp64": ms_seq * 1e-3}
@@ -589,6 +595,7 @@ def main():
         "config": {"workload": "16-water cluster cc-pVDZ (400 Cartesian bf), Schwarz 1e-12, FP64 J+K build",
                    "n_quartets": info["n_quartets"], "n_eri": info["n_eri"], "parallelism": "single GPU"},
         "jk_seconds_fp64": ms64 * 1e-3,
+        "jk_seconds_fp64_persistent_launch": ms64p * 1e-3,
         "quartets_per_s": info["n_quartets"] / (ms64 * 1e-3),
         "jk_build_tflops_executed": build_tflops,
         "jk_build_frac_fp64_peak": build_tflops / FP64_PEAK_TFLOPS,

@@ -45,6 +45,17 @@ extern "C" {
  * (D + D^T)/2 followed by this build of (D - D^T)/2: the exchange matrix of
  * the reference's N^4 FockBuilder::accumulate for any D. */
 #define LAI_JK_ANTISYMMETRIC 2
+/* flag bit, with LAI_JK_FP64: the register-shape classes whose chunk code
+ * fits a 256-lane workgroup run as one persistent launch over a device queue
+ * of their chunks, sorted by class, heaviest first (workgroups claim items
+ * with an atomic counter); the other classes keep their per-class launches,
+ * concurrently.  Same J/K to the order of the FP64 atomic sums.  Ignored by
+ * LAI_JK_MIXED builds.  The queue is allocated by the first such build after
+ * a worklist change (lai_dev_jk_graph_capture does that before it captures;
+ * inside a capture of the caller's own, a first build fails).  The
+ * one-launch-over-a-queue form of the reference's
+ * per-ShellSetQuartet launches (src/device/cuda/engine/cuda_engine.cu:562-665). */
+#define LAI_JK_PERSISTENT 4
 
 #define LAI_MAX_L 3 /* highest shell angular momentum of the energy kernels (f) */
 

@@ -43,7 +43,15 @@ struct JKOptions {
     // FP32 quartets skip primitive quartets whose bound is below this fraction
     // of the quartet's summed primitive bound (0: none)
     Real fp32_prim_cut = 1e-7;
+    // FP64 builds: the register-shape classes as one persistent launch over a
+    // device queue of their chunks instead of one launch per class
+    bool persistent = false;
 };
+
+// lai_dev_jk flags of a build with options o
+inline int jk_flags(const JKOptions& o) {
+    return (o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64) | (o.persistent ? LAI_JK_PERSISTENT : 0);
+}
 
 struct WorklistInfo {
     Index n_quartets{0};

@@ -67,7 +67,7 @@ def build_device(incremental=False, jobs=8):
         if not incremental or _stale(o, [src] + hdr):
             tasks.append([HIPCC] + HIP_FLAGS + ["-DLAI_PART=%d" % k, "-DLAI_NPARTS=%d" % NPARTS,
                                                 "-c", src, "-o", o])
-    for name in ("aux_kernels", "grad_inst", "wl_kernels", "df", "fock_kernels", "api"):
+    for name in ("aux_kernels", "grad_inst", "wl_kernels", "df", "fock_kernels", "persist", "api"):
         src = os.path.join(DEV, name + ".hip")
         o = os.path.join(BUILD, name + ".o")
         objs.append(o)

@@ -304,6 +304,12 @@ struct lai_basis {
   DevBuf<double> Jt, Kt;
   // per-class launch choice
   ClassTune tune[kNumClasses];
+  // queue of the persistent launch (LAI_JK_PERSISTENT), built from the
+  // worklist on first use, and its claim counter
+  DevBuf<PersistItem> pitems;
+  int n_pitems = 0;
+  bool pitems_valid = false;
+  DevBuf<int> pcounter;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -1209,6 +1215,7 @@ extern "C" int lai_dev_build_worklist(lai_basis* b, double thr, const double* sh
   b->n_fp32 = total32;
   b->checksum = sum;
   b->have_worklist = true;
+  b->pitems_valid = false;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
     b->graph_exec = nullptr;
@@ -1248,6 +1255,29 @@ static int ensure_side_streams(lai_basis* b) {
   return LAI_OK;
 }
 
+// The persistent launch's queue: the default chunks of every covered class
+// with work, class by class in launch order (heaviest first).
+static int build_persist_queue(lai_basis* b) {
+  if (b->pitems_valid) return LAI_OK;
+  const auto& Ls = launchers();
+  std::vector<PersistItem> items;
+  for (int c = kNumClasses - 1; c >= 0; --c) {
+    const ClassList& C = b->cls[c];
+    if (!persist_covers(c) || C.runs.empty()) continue;
+    size_t n = 0;
+    for (const auto& r : C.runs) n += size_t(r.k1);
+    for (const JkChunk& ch : cut_runs(C.runs, default_chunk(kShapeRegs, n, b->ctx->n_cu, Ls.info[c].nt)))
+      items.push_back(PersistItem{c, ch});
+  }
+  hipError_t e;
+  if ((e = b->pitems.assign(items.data(), items.size())) != hipSuccess ||
+      (!b->pcounter.p && (e = b->pcounter.alloc(1)) != hipSuccess))
+    return fail(LAI_ERR_MEMORY, std::string("persistent queue: ") + hipGetErrorString(e));
+  b->n_pitems = int(items.size());
+  b->pitems_valid = true;
+  return LAI_OK;
+}
+
 // Class kernels over the bra-anchored chunks, heavy classes first.  A mixed
 // build runs each class as two launches, the FP64 instantiation over the
 // quartets selected for FP64 (QuartetSel::keep = 1) and the float one over
@@ -1263,6 +1293,7 @@ static int ensure_side_streams(lai_basis* b) {
 static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st, hipEvent_t* ev) {
   const auto& Ls = launchers();
   const bool mixed = (flags & LAI_JK_MIXED) != 0 && b->wl_mixed;
+  const bool persist = (flags & LAI_JK_PERSISTENT) != 0 && (flags & LAI_JK_MIXED) == 0;
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
@@ -1277,8 +1308,16 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
       LAI_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
     }
   }
+  if (persist) {  // the covered classes: one launch on st, first
+    const int rc = build_persist_queue(b);
+    if (rc != LAI_OK) return rc;
+    for (int c = 0; c < kNumClasses; ++c)
+      if (persist_covers(c)) load[0] += b->cls_f64[c] + b->cls_f32[c] + 1e6 * double(b->cls[c].nchunks);
+    launch_persistent(a, b->pitems.p, b->n_pitems, b->pcounter.p, st, b->ctx->n_cu);
+  }
   for (int c = kNumClasses - 1; c >= 0; --c) {
     const ClassList& C = b->cls[c];
+    if (persist && persist_covers(c)) continue;  // in the persistent launch
     for (int part = 0; part < 2; ++part) {  // 0: FP64, 1: FP32 (mixed builds only)
       hipEvent_t* e = ev ? ev + 4 * c + 2 * part : nullptr;
       if (e) (void)hipEventRecord(e[0], st);
@@ -1352,7 +1391,7 @@ static int symmetrize(lai_basis* b, double* d_J, double* d_K, hipStream_t st, bo
 extern "C" int lai_dev_jk(lai_basis* b, const double* d_D, double* d_J, double* d_K, int flags, void* stream) {
   if (!b || !d_D || !d_J || !d_K) return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: invalid arguments");
   if (!b->have_worklist) return fail(LAI_ERR_INVALID_STATE, "lai_dev_jk: worklist not built");
-  if ((flags & ~(LAI_JK_MIXED | LAI_JK_ANTISYMMETRIC)) != 0)
+  if ((flags & ~(LAI_JK_MIXED | LAI_JK_ANTISYMMETRIC | LAI_JK_PERSISTENT)) != 0)
     return fail(LAI_ERR_INVALID_ARGUMENT, "lai_dev_jk: unknown flags");
   hipStream_t st = hipStream_t(stream);
   const size_t nn = size_t(b->mat_size);
@@ -1384,6 +1423,7 @@ extern "C" int lai_dev_jk_graph_capture(lai_basis* b, const double* d_D, double*
   }
   int rc = ensure_side_streams(b);  // not inside the capture
   if (rc != LAI_OK) return rc;
+  if ((flags & LAI_JK_PERSISTENT) && b->have_worklist && (rc = build_persist_queue(b)) != LAI_OK) return rc;
   LAI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   rc = lai_dev_jk(b, d_D, d_J, d_K, flags, stream);
   hipGraph_t g = nullptr;

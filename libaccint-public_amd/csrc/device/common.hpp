@@ -117,6 +117,12 @@ struct JkChunk {
   int bra, seg, k0, k1;
 };
 
+// one item of the persistent launch's queue: a chunk of class cls
+struct PersistItem {
+  int cls;
+  JkChunk ch;
+};
+
 // ---------------------------------------------------------------------------
 // Rys roots/weights.  The device table holds the piecewise-Chebyshev
 // interpolants (lai_rys_cheb_h), the Gauss-Hermite limits, and from

@@ -49,6 +49,14 @@ void fill_launchers_part_5(ClassLauncher*, ClassLauncher*, ClassInfo*);
 void fill_launchers_part_6(ClassLauncher*, ClassLauncher*, ClassInfo*);
 void fill_launchers_part_7(ClassLauncher*, ClassLauncher*, ClassInfo*);
 
+// the persistent launch (persist.hip): whether class cls is among the classes
+// it covers, and the launch over n_items queue items (counter: one device int,
+// cleared on the stream first); returns 0
+struct PersistItem;
+bool persist_covers(int cls);
+int launch_persistent(const EriArgs& a, const PersistItem* items, int n_items, int* counter, hipStream_t s,
+                      int n_cu);
+
 // gradient kernels exist for the classes whose shells are at most d
 inline constexpr int kNumGradClasses = 21;
 using GradLauncher = void (*)(const GradArgs&, hipStream_t, int max_blocks);

@@ -118,7 +118,7 @@ void BatchEngine::build_jk(const JKOptions& o, const Real* d_D, Real* d_J, Real*
                            const Real* D_host) {
     prepare_worklist(o, D_host);
     device::check(lai_dev_set_mixed_prim_cut(device_handle(), o.fp32_prim_cut), "BatchEngine::build_jk");
-    device::check(lai_dev_jk(device_handle(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
+    device::check(lai_dev_jk(device_handle(), d_D, d_J, d_K, jk_flags(o), s),
                   "BatchEngine::build_jk");
 }
 
@@ -138,7 +138,7 @@ void BatchEngine::capture_jk_graph(const JKOptions& o, const Real* d_D, Real* d_
     prepare_worklist(o, nullptr);
     device::check(lai_dev_set_mixed_prim_cut(device_handle(), o.fp32_prim_cut), "BatchEngine::capture_jk_graph");
     device::check(
-        lai_dev_jk_graph_capture(device_handle(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
+        lai_dev_jk_graph_capture(device_handle(), d_D, d_J, d_K, jk_flags(o), s),
         "BatchEngine::capture_jk_graph");
 }
 

@@ -379,13 +379,13 @@ WorklistInfo Engine::prepare_worklist(const JKOptions& o, const Real* D_host) {
 
 void Engine::build_jk(const JKOptions& o, const Real* d_D, Real* d_J, Real* d_K, hipStream_t s, const Real* D_host) {
     prepare_worklist(o, D_host);
-    const int flags = o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64;
+    const int flags = jk_flags(o);
     // ProfileOnce: the first build of a precision tunes the dispatch table
     if (config_.enable_auto_tuning && tuned_flags_ != flags &&
         static_cast<Size>(wl_info_.n_quartets) >= config_.auto_tune_min_batch)
         (void)autotune(o, d_D, s, 3, D_host);
     device::check(lai_dev_set_mixed_prim_cut(device_basis(), o.fp32_prim_cut), "build_jk");
-    device::check(lai_dev_jk(device_basis(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
+    device::check(lai_dev_jk(device_basis(), d_D, d_J, d_K, jk_flags(o), s),
                   "build_jk");
 }
 
@@ -403,13 +403,13 @@ void Engine::build_jk_host(const JKOptions& o, const Real* D, Real* J, Real* K) 
 
 void Engine::capture_jk_graph(const JKOptions& o, const Real* d_D, Real* d_J, Real* d_K, hipStream_t s) {
     prepare_worklist(o, nullptr);
-    const int flags = o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64;
+    const int flags = jk_flags(o);
     if (config_.enable_auto_tuning && tuned_flags_ != flags &&
         static_cast<Size>(wl_info_.n_quartets) >= config_.auto_tune_min_batch)
         (void)autotune(o, d_D, s, 3, nullptr);
     device::check(lai_dev_set_mixed_prim_cut(device_basis(), o.fp32_prim_cut), "capture_jk_graph");
     device::check(
-        lai_dev_jk_graph_capture(device_basis(), d_D, d_J, d_K, o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64, s),
+        lai_dev_jk_graph_capture(device_basis(), d_D, d_J, d_K, jk_flags(o), s),
         "capture_jk_graph");
 }
 
@@ -443,7 +443,7 @@ void Engine::build_jk_general(const JKOptions& o, const Real* D_host, const Real
     dDa.upload(Da.data(), n * n, s);
     build_jk(o, dDs.get(), d_J, d_K, s, D_host);
     device::check(lai_dev_jk(device_basis(), dDa.get(), d_J, d_K,
-                             (o.mixed_precision ? LAI_JK_MIXED : LAI_JK_FP64) | LAI_JK_ANTISYMMETRIC, s),
+                             jk_flags(o) | LAI_JK_ANTISYMMETRIC, s),
                   "build_jk");
     device::check(lai_dev_stream_sync(s), "build_jk");  // dDs and dDa are released on return
 }

@@ -519,16 +519,18 @@ PYBIND11_MODULE(_core, m) {
         .def("__len__", [](const DispatchTable& t) { return t.entries.size(); });
 
     py::class_<JKOptions>(m, "JKOptions")
-        .def(py::init([](Real thr, bool dw, bool mixed, Real tau, Real pcut) {
-                 return JKOptions{thr, dw, mixed, tau, pcut};
+        .def(py::init([](Real thr, bool dw, bool mixed, Real tau, Real pcut, bool persistent) {
+                 return JKOptions{thr, dw, mixed, tau, pcut, persistent};
              }),
              py::arg("schwarz_threshold") = 0.0, py::arg("density_weighted") = false,
-             py::arg("mixed_precision") = false, py::arg("fp32_threshold") = 1e-7, py::arg("fp32_prim_cut") = 1e-7)
+             py::arg("mixed_precision") = false, py::arg("fp32_threshold") = 1e-7, py::arg("fp32_prim_cut") = 1e-7,
+             py::arg("persistent") = false)
         .def_readwrite("schwarz_threshold", &JKOptions::schwarz_threshold)
         .def_readwrite("density_weighted", &JKOptions::density_weighted)
         .def_readwrite("mixed_precision", &JKOptions::mixed_precision)
         .def_readwrite("fp32_threshold", &JKOptions::fp32_threshold)
-        .def_readwrite("fp32_prim_cut", &JKOptions::fp32_prim_cut);
+        .def_readwrite("fp32_prim_cut", &JKOptions::fp32_prim_cut)
+        .def_readwrite("persistent", &JKOptions::persistent);
 
     py::class_<Engine>(m, "Engine")
         .def(py::init([](std::shared_ptr<BasisSet> b, DispatchConfig c) { return std::make_unique<Engine>(*b, c); }),

@@ -14,6 +14,7 @@ LAI_OK = 0
 LAI_JK_FP64 = 0
 LAI_JK_MIXED = 1
 LAI_JK_ANTISYMMETRIC = 2
+LAI_JK_PERSISTENT = 4
 LAI_NOALLOC_MAX_QUARTETS = 65536
 
 _vp = ctypes.c_void_p

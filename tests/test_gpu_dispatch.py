@@ -163,3 +163,55 @@ def test_concurrent_classes_equal_serial_build():
     (J0, K0), (J1, K1), (J2, K2) = out
     assert rel(J0, J1) < 1e-12 and rel(K0, K1) < 1e-12
     assert rel(J0, J2) < 1e-12 and rel(K0, K2) < 1e-12
+
+
+def test_persistent_launch_equals_per_class_launches():
+    """LAI_JK_PERSISTENT: the register-shape classes drain one device queue in
+    a single launch.  Same J/K as the per-class launches, for one molecule
+    (cc-pVDZ, every covered class with s-d shells) and a batch of several
+    molecules in cc-pVTZ (f shells, covered and uncovered classes side by
+    side), also through a captured graph replayed twice."""
+    waters = systems.water_cluster(4, 9.0, seed=3)
+    b = lai.basis_set("cc-pvdz", waters)
+    n = b.n_basis_functions()
+    D = systems.density(n, 20, 8, 1.0 / np.sqrt(n))
+    e = lai.Engine(b)
+    J0, K0 = e.build_jk(D, lai.JKOptions(schwarz_threshold=1e-12))
+    J1, K1 = e.build_jk(D, lai.JKOptions(schwarz_threshold=1e-12, persistent=True))
+    assert rel(J1, J0) < 1e-12 and rel(K1, K0) < 1e-12
+    torch = pytest.importorskip("torch")
+    s = torch.cuda.Stream()
+    # a graph captured before any persistent build of the engine
+    e2 = lai.Engine(b)
+    op = lai.JKOptions(schwarz_threshold=1e-12, persistent=True)
+    e2.prepare_worklist(op)
+    tD = torch.from_numpy(D).cuda()
+    tJ = torch.zeros_like(tD)
+    tK = torch.zeros_like(tD)
+    e2.capture_jk_graph(tD.data_ptr(), tJ.data_ptr(), tK.data_ptr(), op, s.cuda_stream)
+    e2.launch_jk_graph(s.cuda_stream)
+    s.synchronize()
+    assert rel(tJ.cpu().numpy(), J0) < 1e-12 and rel(tK.cpu().numpy(), K0) < 1e-12
+    rng = np.random.default_rng(4)
+    mols = [systems.random_molecule(rng) for _ in range(20)]
+    mols = [m for m in mols if len(m) <= 13][:3]  # 13, 7 and 10 atoms
+    assert len(mols) == 3
+    be = lai.BatchEngine([lai.basis_set("cc-pvtz", m) for m in mols])
+    nbf = be.n_basis_functions()
+    Db = np.concatenate([systems.density(k, max(2, k // 5), 40 + i, 1.0 / np.sqrt(k)).ravel()
+                         for i, k in enumerate(nbf)])
+    Jb0, Kb0 = be.build_jk(Db, lai.JKOptions(schwarz_threshold=1e-12))
+    op = lai.JKOptions(schwarz_threshold=1e-12, persistent=True)
+    Jb1, Kb1 = be.build_jk(Db, op)
+    assert rel(Jb1, Jb0) < 1e-12 and rel(Kb1, Kb0) < 1e-12
+    tD = torch.from_numpy(Db).cuda()
+    tJ = torch.zeros_like(tD)
+    tK = torch.zeros_like(tD)
+    be.capture_jk_graph(tD.data_ptr(), tJ.data_ptr(), tK.data_ptr(), op, s.cuda_stream)
+    for rep in range(2):
+        tJ.zero_()
+        tK.zero_()
+        torch.cuda.synchronize()
+        be.launch_jk_graph(s.cuda_stream)
+        s.synchronize()
+        assert rel(tJ.cpu().numpy(), Jb0) < 1e-12 and rel(tK.cpu().numpy(), Kb0) < 1e-12, rep
