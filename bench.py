@@ -7,12 +7,15 @@ with the density resident in HBM): the seeded 16-water cluster in cc-pVDZ
 (400 Cartesian functions; O-H 1.81 bohr, HOH 104.5 deg, random orientations,
 14 bohr cube, O-O >= 5 bohr), D = C C^T with C 400x80 ~ N(0, 1/384), Schwarz
 threshold 1e-12, FP64.  The build is captured once in a hipGraph and replayed.
-Also reported: FP64 and mixed-precision builds of the linear alkane C40H82 in
+Also reported: the same build as one persistent launch for the register-shape
+classes; FP64 and mixed-precision builds of the linear alkane C40H82 in
 def2-SVP (FP32 recursion for quartets with Q_ij Q_kl max|D| < 1e-7) with the
-errors of J, K and E against the FP64 build, the nuclear-gradient time for
-the 32-water def2-SVP cluster, the per-class roofline of the dominant kernel
-and a CPU baseline timed on a bounded sample with the reference CPU sources
-(oracle/_ref) or, where those are not built, their C restatement.
+errors of J, K and E against the FP64 build; the FP64 build of 64 seeded
+cc-pVTZ molecules in one BatchEngine (and comparisons on its 23 smallest);
+the nuclear-gradient time for the 32-water def2-SVP cluster; density-fitted
+J/K; the per-class roofline of the dominant kernel and a CPU baseline timed
+on a bounded sample with the reference CPU sources (oracle/_ref) or, where
+those are not built, their C restatement.
 
 Prints one JSON line.
 """
@@ -110,6 +113,52 @@ def cpu_leg(threads):
     print(json.dumps(out), flush=True)
 
 
+def class_name(c):
+    names = "spdf"
+    pb = int((math.isqrt(8 * c + 1) - 1) // 2)
+    pk = c - pb * (pb + 1) // 2
+
+    def pc(p):
+        la = int((math.isqrt(8 * p + 1) - 1) // 2)
+        return la, p - la * (la + 1) // 2
+    a, b = pc(pb)
+    cc, d = pc(pk)
+    return "(%s%s|%s%s)" % (names[a], names[b], names[cc], names[d])
+
+
+def mixed_probe(lai, _cabi, systems, torch, stream, L):
+    """C40H82 def2-SVP: per-class kernel times of the FP64 build and of the same
+    quartets all through the float instantiation (tau above every bound, no
+    primitive cut), one class after another (lai_dev_jk_profile)."""
+    import ctypes
+    atoms = systems.alkane(40)
+    basis = lai.basis_set("def2-svp", atoms)
+    n = basis.n_basis_functions()
+    D = systems.density(n, 161, 11, 1.0 / math.sqrt(n))
+    eng = lai.Engine(basis)
+    h = ctypes.c_void_p(eng.device_basis_handle())
+    tD = torch.from_numpy(D).cuda()
+    tJ = torch.zeros_like(tD)
+    tK = torch.zeros_like(tD)
+    dp = ctypes.POINTER(ctypes.c_double)
+    out = {}
+    for tag, o, flags in (("fp64", lai.JKOptions(schwarz_threshold=1e-12), 0),
+                          ("all_fp32", lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True,
+                                                     fp32_threshold=1e30, fp32_prim_cut=0.0), _cabi.LAI_JK_MIXED)):
+        eng.prepare_worklist(o, D if o.mixed_precision else None)
+        _cabi.check(L.lai_dev_set_mixed_prim_cut(h, ctypes.c_double(o.fp32_prim_cut)))
+        a64, a32 = np.zeros(55), np.zeros(55)
+        for _ in range(2):
+            _cabi.check(L.lai_dev_jk_profile(h, ctypes.c_void_p(tD.data_ptr()), ctypes.c_void_p(tJ.data_ptr()),
+                                             ctypes.c_void_p(tK.data_ptr()), flags, ctypes.c_void_p(stream.cuda_stream),
+                                             a64.ctypes.data_as(dp), a32.ctypes.data_as(dp)))
+        out[tag] = a64 + a32
+    print(json.dumps({"workload": "C40H82 def2-SVP (1010 Cartesian bf), Schwarz 1e-12: per-class J/K kernel ms",
+                      "class_ms_fp64": float(out["fp64"].sum()), "class_ms_all_fp32": float(out["all_fp32"].sum()),
+                      "classes": {class_name(c): [round(float(out["fp64"][c]), 3), round(float(out["all_fp32"][c]), 3)]
+                                  for c in np.argsort(-out["fp64"]) if out["fp64"][c] > 0.05}}), flush=True)
+
+
 def main():
     if len(sys.argv) > 2 and sys.argv[1] == "--cpu-leg":
         cpu_leg(int(sys.argv[2]))
@@ -130,8 +179,11 @@ def main():
                     help="write the 16-water build's dispatch table (JSON) to this path")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="end with a device fill and copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--batch-full", action="store_true",
-                    help="time all 64 molecules of the batched workload (several minutes per build)")
+    ap.add_argument("--mixed-probe", action="store_true",
+                    help="only the per-class FP64 and all-FP32 profiles of the C40H82 def2-SVP build (one JSON "
+                         "line; for counter runs)")
+    ap.add_argument("--no-batch-full", action="store_true",
+                    help="skip the FP64 build of all 64 molecules of the batched workload (about 100 s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this library has no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -146,6 +198,9 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream()
     L = _cabi.lib()
+    if args.mixed_probe:
+        mixed_probe(lai, _cabi, systems, torch, stream, L)
+        return
 
     # ---------------------------------------------------------------- setup
     atoms = systems.water_cluster(16, 14.0, seed=2)
@@ -231,18 +286,6 @@ def main():
                                          m32.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         ms_cls += (m64 + m32) / reps
     dom = int(np.argmax(ms_cls))
-    names = "spdf"
-
-    def cls_name(c):
-        pb = int((math.isqrt(8 * c + 1) - 1) // 2)
-        pk = c - pb * (pb + 1) // 2
-
-        def pc(p):
-            la = int((math.isqrt(8 * p + 1) - 1) // 2)
-            return la, p - la * (la + 1) // 2
-        a, b = pc(pb)
-        cc, d = pc(pk)
-        return "(%s%s|%s%s)" % (names[a], names[b], names[cc], names[d])
 
     # executed work (primitive pairs above the screening bound) of the
     # dominant class kernel over its event-timed duration
@@ -253,10 +296,10 @@ def main():
     # reference's algorithm)
     flops_total = info["flops_fp64"] + info["flops_fp32"]
     build_tflops_model = flops_total / (ms64 * 1e-3) / 1e12
-    per_class = sorted(((cls_name(c), round(ms_cls[c], 3), round((f64c[c] + f32c[c]) / max(ms_cls[c], 1e-9) / 1e9, 2))
+    per_class = sorted(((class_name(c), round(ms_cls[c], 3), round((f64c[c] + f32c[c]) / max(ms_cls[c], 1e-9) / 1e9, 2))
                         for c in range(55) if ms_cls[c] > 0), key=lambda x: -x[1])
     log("dominant class %s: %.3f ms, %.2f TFLOP/s; whole build %.2f TFLOP/s" %
-        (cls_name(dom), ms_cls[dom], achieved, build_tflops))
+        (class_name(dom), ms_cls[dom], achieved, build_tflops))
     log("per-class (name, ms, TFLOP/s x1e3): %s" % per_class[:12])
 
     # ---------------------------------------------------------------- mixed
@@ -322,11 +365,24 @@ def main():
                                          ctypes.c_void_p(tmK.data_ptr()), _cabi.LAI_JK_MIXED,
                                          ctypes.c_void_p(stream.cuda_stream), pm64.ctypes.data_as(dp),
                                          pm32.ctypes.data_as(dp)))
-        split = {"class_ms_fp64": float(pm64.sum()), "class_ms_fp32": float(pm32.sum()),
-                 "tflops_fp64": float(fm64.sum() / max(pm64.sum(), 1e-9) / 1e9),
-                 "tflops_fp32": float(fm32.sum() / max(pm32.sum(), 1e-9) / 1e9)}
-        log("C40H82 mixed split: FP64 launches %.3f ms at %.2f TFLOP/s, FP32 launches %.3f ms at %.2f TFLOP/s" %
-            (split["class_ms_fp64"], split["tflops_fp64"], split["class_ms_fp32"], split["tflops_fp32"]))
+        # the FP32-marked quartets run in float only in the wavefront and
+        # workgroup shapes (the lane-per-quartet classes run them in FP64)
+        tab = (_cabi.ClassTune * 55)()
+        ntab = ctypes.c_int()
+        _cabi.check(L.lai_dev_get_tuning(hm, tab, 55, ctypes.byref(ntab)))
+        in_float = np.zeros(55, dtype=bool)
+        for k in range(ntab.value):
+            in_float[tab[k].cls] = tab[k].strategy != 0
+        f_float = float(fm32[in_float].sum())
+        f_all = float((fm64 + fm32).sum())
+        split = {"class_ms_fp64_launches": float(pm64.sum()), "class_ms_fp32_launches": float(pm32.sum()),
+                 "tflops_fp64_launches": (f_all - f_float) / max(float(pm64.sum()), 1e-9) / 1e9,
+                 "tflops_fp32_launches": f_float / max(float(pm32.sum()), 1e-9) / 1e9,
+                 "flop_fraction_in_float": f_float / max(f_all, 1.0)}
+        log("C40H82 mixed: FP64 launches %.3f ms at %.2f TFLOP/s, FP32 launches %.3f ms at %.2f TFLOP/s, %.3f of "
+            "the flops in float" % (split["class_ms_fp64_launches"], split["tflops_fp64_launches"],
+                                    split["class_ms_fp32_launches"], split["tflops_fp32_launches"],
+                                    split["flop_fraction_in_float"]))
         # the same FP32 split without the FP32 primitive cut
         on = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7, fp32_prim_cut=0.0)
         Jn, Kn = m_once(on)
@@ -350,7 +406,7 @@ def main():
                                       fp32_prim_cut=0.0), _cabi.LAI_JK_MIXED)
         split["all_fp32_vs_fp64"] = {
             "class_ms_fp64": float(p64.sum()), "class_ms_all_fp32": float(p32.sum()),
-            "top_classes_ms_fp64_fp32": [(cls_name(c), round(float(p64[c]), 3), round(float(p32[c]), 3))
+            "top_classes_ms_fp64_fp32": [(class_name(c), round(float(p64[c]), 3), round(float(p32[c]), 3))
                                          for c in np.argsort(-p64)[:8]]}
         log("C40H82 all quartets FP32: class kernels %.3f ms against %.3f ms FP64" % (p32.sum(), p64.sum()))
         mixed = {"mixed_workload": "C40H82 def2-SVP (1010 Cartesian bf), Schwarz 1e-12, %d quartets; FP32 Rys "
@@ -368,31 +424,32 @@ def main():
 
     # -------------------------------------------------------------- batched
     # 64 seeded molecules (3-60 atoms of H/C/N/O, random-walk bonding) in
-    # cc-pVTZ through one BatchEngine; by default the molecules with at most
-    # 20 atoms (one full build of all 64 takes minutes on one GPU)
+    # cc-pVTZ through one BatchEngine: the FP64 build of all 64 (one replay of
+    # the captured build), and on the 23 molecules with at most 20 atoms the
+    # comparisons -- persistent launch, mixed precision, one engine per
+    # molecule.  No autotuner run: the default launch choices.
     batch = {}
     if not args.no_batch:
         rng = np.random.default_rng(4)
         mols = [systems.random_molecule(rng) for _ in range(64)]
-        sel = list(range(64)) if args.batch_full else [k for k, m in enumerate(mols) if len(m) <= 20]
-        bases = [lai.basis_set("cc-pvtz", mols[k]) for k in sel]
-        be = lai.BatchEngine(bases)
-        bnbf = be.n_basis_functions()
-        bD = np.concatenate([systems.density(nb_, max(2, nb_ // 5), 200 + k, 1.0 / math.sqrt(nb_)).ravel()
-                             for k, nb_ in zip(sel, bnbf)])
-        t0 = time.perf_counter()
-        bo = lai.JKOptions(schwarz_threshold=1e-12)
-        binfo = be.prepare_worklist(bo)
-        bsetup = time.perf_counter() - t0
-        log("batch (%d molecules): setup %.2f s, %d quartets, %.3e executed flops" %
-            (len(sel), bsetup, binfo["n_quartets"], binfo["flops_fp64"] + binfo["flops_fp32"]))
-        tbD = torch.from_numpy(bD).to(dev)
-        tbJ = torch.zeros_like(tbD)
-        tbK = torch.zeros_like(tbD)
 
-        def b_timed(o, steps, what):
+        def batch_setup(sel):
+            bases = [lai.basis_set("cc-pvtz", mols[k]) for k in sel]
+            be = lai.BatchEngine(bases)
+            bnbf = be.n_basis_functions()
+            bD = np.concatenate([systems.density(nb_, max(2, nb_ // 5), 200 + k, 1.0 / math.sqrt(nb_)).ravel()
+                                 for k, nb_ in zip(sel, bnbf)])
+            t0 = time.perf_counter()
+            binfo = be.prepare_worklist(lai.JKOptions(schwarz_threshold=1e-12))
+            bsetup = time.perf_counter() - t0
+            log("batch (%d molecules): setup %.2f s, %d quartets, %.3e executed flops" %
+                (len(sel), bsetup, binfo["n_quartets"], binfo["flops_fp64"] + binfo["flops_fp32"]))
+            tbD = torch.from_numpy(bD).to(dev)
+            return bases, be, bnbf, bD, tbD, torch.zeros_like(tbD), torch.zeros_like(tbD), binfo, bsetup
+
+        def b_timed(be, tbD, tbJ, tbK, o, steps, warm, what):
             be.capture_jk_graph(tbD.data_ptr(), tbJ.data_ptr(), tbK.data_ptr(), o, stream.cuda_stream)
-            if not args.batch_full:  # the full batch is timed on its first replay
+            for _ in range(warm):
                 be.launch_jk_graph(stream.cuda_stream)
             torch.cuda.synchronize()
             ev0 = torch.cuda.Event(enable_timing=True)
@@ -406,18 +463,16 @@ def main():
             log("batch %s build: %.3f ms" % (what, ms))
             return ms
 
-        bsteps = 1 if args.batch_full else 3
-        bt = 0.0
-        if not args.no_tune and not args.batch_full:
-            t0 = time.perf_counter()
-            be.autotune(tbD.data_ptr(), bo, stream.cuda_stream, 1)
-            bt = time.perf_counter() - t0
-        ms_b = b_timed(bo, bsteps, "FP64")
-        ms_bp = b_timed(lai.JKOptions(schwarz_threshold=1e-12, persistent=True), bsteps, "FP64 persistent")
+        bo = lai.JKOptions(schwarz_threshold=1e-12)
+        sel = [k for k, m in enumerate(mols) if len(m) <= 20]
+        bases, be, bnbf, bD, tbD, tbJ, tbK, binfo, bsetup = batch_setup(sel)
+        ms_b = b_timed(be, tbD, tbJ, tbK, bo, 3, 1, "FP64")
+        ms_bp = b_timed(be, tbD, tbJ, tbK, lai.JKOptions(schwarz_threshold=1e-12, persistent=True), 3, 1,
+                        "FP64 persistent")
         bm = lai.JKOptions(schwarz_threshold=1e-12, mixed_precision=True, fp32_threshold=1e-7)
         be.prepare_worklist(bm, bD)
         bminfo = be.worklist_info()
-        ms_bm = b_timed(bm, bsteps, "mixed")
+        ms_bm = b_timed(be, tbD, tbJ, tbK, bm, 3, 1, "mixed")
         # the same molecules one engine at a time (no batching), FP64
         engines = [lai.Engine(bb) for bb in bases]
         offs = be.matrix_offsets()
@@ -434,19 +489,32 @@ def main():
         ev1.record(stream)
         torch.cuda.synchronize()
         ms_seq = ev0.elapsed_time(ev1)
-        batch = {"batch_workload": "%d of the 64 seeded H/C/N/O molecules in cc-pVTZ (%s), nbf %d..%d, "
-                                   "Schwarz 1e-12, %d quartets, %.3e ERIs" %
-                                   (len(sel), "all" if args.batch_full else "those with <= 20 atoms",
-                                    min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
-                 "batch_setup_seconds": bsetup, "batch_autotune_seconds": bt,
-                 "batch_jk_seconds_fp64": ms_b * 1e-3, "batch_eris_per_s_fp64": binfo["n_eri"] / (ms_b * 1e-3),
-                 "batch_jk_seconds_fp64_persistent_launch": ms_bp * 1e-3,
-                 "batch_jk_seconds_mixed": ms_bm * 1e-3,
-                 "batch_mixed_fp32_quartet_fraction": bminfo["n_fp32"] / max(bminfo["n_quartets"], 1),
-                 "batch_one_engine_per_molecule_seconds_fp64": ms_seq * 1e-3}
-        log("batch (%d molecules): setup %.2f s, FP64 %.3f ms, mixed %.3f ms, one engine per molecule %.3f ms" %
-            (len(sel), bsetup, ms_b, ms_bm, ms_seq))
+        batch = {"batch_subset_workload": "23 of the 64 seeded H/C/N/O molecules in cc-pVTZ (those with <= 20 "
+                                          "atoms), nbf %d..%d, Schwarz 1e-12, %d quartets, %.3e ERIs" %
+                                          (min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
+                 "batch_subset_setup_seconds": bsetup,
+                 "batch_subset_jk_seconds_fp64": ms_b * 1e-3,
+                 "batch_subset_eris_per_s_fp64": binfo["n_eri"] / (ms_b * 1e-3),
+                 "batch_subset_jk_seconds_fp64_persistent_launch": ms_bp * 1e-3,
+                 "batch_subset_jk_seconds_mixed": ms_bm * 1e-3,
+                 "batch_subset_mixed_fp32_quartet_fraction": bminfo["n_fp32"] / max(bminfo["n_quartets"], 1),
+                 "batch_subset_one_engine_per_molecule_seconds_fp64": ms_seq * 1e-3}
+        log("batch (23 molecules): FP64 %.3f ms, persistent %.3f ms, mixed %.3f ms, one engine per molecule "
+            "%.3f ms" % (ms_b, ms_bp, ms_bm, ms_seq))
         del be, engines, tbD, tbJ, tbK
+        if not args.no_batch_full:
+            _, be, bnbf, bD, tbD, tbJ, tbK, binfo, bsetup = batch_setup(list(range(64)))
+            ms_f = b_timed(be, tbD, tbJ, tbK, bo, 1, 0, "FP64, all 64 molecules,")
+            flops_f = binfo["flops_fp64"] + binfo["flops_fp32"]
+            batch.update({
+                "batch_workload": "all 64 seeded H/C/N/O molecules (3-60 atoms) in cc-pVTZ, nbf %d..%d, Schwarz "
+                                  "1e-12, %d quartets, %.3e ERIs, FP64, one BatchEngine" %
+                                  (min(bnbf), max(bnbf), binfo["n_quartets"], binfo["n_eri"]),
+                "batch_setup_seconds": bsetup,
+                "batch_jk_seconds_fp64": ms_f * 1e-3,
+                "batch_eris_per_s_fp64": binfo["n_eri"] / (ms_f * 1e-3),
+                "batch_tflops_executed": flops_f / (ms_f * 1e-3) / 1e12})
+            del be, tbD, tbJ, tbK
 
     # ------------------------------------------------------------- gradient
     grad = {}
@@ -603,9 +671,11 @@ def main():
         "flops_executed": exec_total,
         "flops_all_nonzero_primitives": flops_total,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_of(cls_name(dom)),
-                     "kernel": "class %s ERI/J-K kernel" % cls_name(dom),
-                     "kernel_ms": float(ms_cls[dom])},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_of(class_name(dom)),
+                     "kernel": "class %s ERI/J-K kernel" % class_name(dom),
+                     "kernel_ms": float(ms_cls[dom]),
+                     "pipe": "FP64 vector ALU (the Rys recursion and digestion are FMA chains, not matrix "
+                             "tiles; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s)"},
         "cpu_baseline": cpu,
     }
     out.update(tune)
