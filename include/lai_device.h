@@ -38,7 +38,11 @@ extern "C" {
 
 /* J/K build flags */
 #define LAI_JK_FP64 0  /* every quartet in FP64 */
-#define LAI_JK_MIXED 1 /* quartets marked FP32 by the worklist run the FP32 recursion */
+/* quartets marked FP32 by the worklist run the FP32 recursion, in the classes
+ * of wavefront and workgroup shape (a launch per precision); the
+ * lane-per-quartet classes, bound by memory latency rather than by their
+ * arithmetic, run them in FP64 */
+#define LAI_JK_MIXED 1
 /* flag bit, with either of the above: d_D is antisymmetric (D^T = -D); d_J is
  * left alone and only the antisymmetric part (Kt - Kt^T)/2 of the exchange
  * build is added to d_K.  K is linear in D, so a general D is the build of

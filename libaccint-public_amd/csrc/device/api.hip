@@ -987,6 +987,16 @@ int class_shape(const ClassInfo& info, const ClassTune& t) {
   return (t.alt && info.alt != kShapeNone) ? info.alt : info.shape;
 }
 
+// Whether a mixed build runs class c's FP32 quartets through the float
+// instantiation (a launch of its own).  Only the wavefront and workgroup
+// shapes: there the float tables halve the LDS traffic of the products and the
+// recursion is measured faster.  The lane-per-quartet classes are bound by
+// memory latency, their float recursion is measured slower, so a mixed build
+// runs their FP32-marked quartets in FP64, in the class's one launch.
+bool class_split(const lai_basis* b, int c, bool mixed) {
+  return mixed && class_shape(launchers().info[c], b->tune[c]) != kShapeRegs;
+}
+
 // Chunks of class c under its launch choice (and, when that is not the
 // default, the default chunks for the gradient kernels) on the device.
 int upload_chunks(lai_basis* b, int c) {
@@ -1297,7 +1307,7 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
   a.segs = b->segs.p;
   a.kets = b->kets.p;
   a.sel = make_sel(b);
-  if (!mixed) a.sel.mixed = 0;
+  const int sel_mixed = a.sel.mixed;
   const bool fan = ev == nullptr && b->fork != nullptr;
   hipStream_t streams[kSideStreams + 1] = {st};
   double load[kSideStreams + 1] = {};
@@ -1311,6 +1321,7 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
   if (persist) {  // the covered classes: one launch on st, first
     const int rc = build_persist_queue(b);
     if (rc != LAI_OK) return rc;
+    a.sel.mixed = 0;
     for (int c = 0; c < kNumClasses; ++c)
       if (persist_covers(c)) load[0] += b->cls_f64[c] + b->cls_f32[c] + 1e6 * double(b->cls[c].nchunks);
     launch_persistent(a, b->pitems.p, b->n_pitems, b->pcounter.p, st, b->ctx->n_cu);
@@ -1318,10 +1329,12 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
   for (int c = kNumClasses - 1; c >= 0; --c) {
     const ClassList& C = b->cls[c];
     if (persist && persist_covers(c)) continue;  // in the persistent launch
-    for (int part = 0; part < 2; ++part) {  // 0: FP64, 1: FP32 (mixed builds only)
+    const bool split = class_split(b, c, mixed);
+    a.sel.mixed = split ? sel_mixed : 0;
+    for (int part = 0; part < 2; ++part) {  // 0: FP64, 1: FP32 (split classes only)
       hipEvent_t* e = ev ? ev + 4 * c + 2 * part : nullptr;
       if (e) (void)hipEventRecord(e[0], st);
-      const int64_t n32 = mixed ? b->cls_n32[c] : 0;
+      const int64_t n32 = split ? b->cls_n32[c] : 0;
       const bool run = C.nchunks > 0 && (part == 0 ? b->counts[c] - n32 > 0 : n32 > 0);
       if (run) {
         hipStream_t s = st;
@@ -1329,13 +1342,13 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
           int k = 0;
           for (int i = 1; i <= kSideStreams; ++i)
             if (load[i] < load[k]) k = i;
-          const double fl = part == 1 ? b->cls_f32[c] : mixed ? b->cls_f64[c] : b->cls_f64[c] + b->cls_f32[c];
+          const double fl = part == 1 ? b->cls_f32[c] : split ? b->cls_f64[c] : b->cls_f64[c] + b->cls_f32[c];
           load[k] += fl + 1e6 * double(C.nchunks);
           s = streams[k];
         }
         a.chunks = C.chunks.p;
         a.nchunks = int(C.nchunks);
-        a.sel.keep = mixed ? part + 1 : 0;
+        a.sel.keep = split ? part + 1 : 0;
         LaunchTune t;
         t.alt = b->tune[c].alt;
         t.blocks_per_cu = b->tune[c].bpc;
@@ -1343,7 +1356,7 @@ static int launch_jk_classes(lai_basis* b, EriArgs a, int flags, hipStream_t st,
         (part == 1 ? Ls.f32[c] : Ls.f64[c])(a, s, t);
       }
       if (e) (void)hipEventRecord(e[1], st);
-      if (!mixed) {
+      if (!split) {
         if (e) {  // the FP32 part is empty: a zero-length pair
           (void)hipEventRecord(e[2], st);
           (void)hipEventRecord(e[3], st);
@@ -1712,7 +1725,7 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
   a.kets = b->kets.p;
   a.sel = make_sel(b);
   const bool mixed = flags == LAI_JK_MIXED && b->wl_mixed;
-  if (!mixed) a.sel.mixed = 0;
+  const int sel_mixed = a.sel.mixed;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   LAI_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1732,11 +1745,13 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
     t.alt = b->tune[c].alt;
     t.blocks_per_cu = b->tune[c].bpc;
     t.n_cu = n_cu;
-    const int64_t n32 = mixed ? b->cls_n32[c] : 0;
+    const bool split = class_split(b, c, mixed);
+    const int64_t n32 = split ? b->cls_n32[c] : 0;
     const bool p64 = b->counts[c] - n32 > 0, p32 = n32 > 0;
+    a.sel.mixed = split ? sel_mixed : 0;
     auto f = [&](hipStream_t s, const LaunchTune& tt) {
       if (p64) {
-        a.sel.keep = mixed ? 1 : 0;
+        a.sel.keep = split ? 1 : 0;
         Ls.f64[c](a, s, tt);
       }
       if (p32) {

@@ -72,7 +72,8 @@ RowsPick pick_rows(K kern, size_t dmax, int base, size_t per_rep, size_t rows) {
 
 // MIXED = false: the recursion in FP64 (and the raw-ERI mode); true: in
 // float, launched with QuartetSel::keep == 2 so that it takes only the
-// quartets the worklist marks for FP32.
+// quartets the worklist marks for FP32 (wavefront and workgroup shapes only;
+// the host never sends a lane-per-quartet shape here).
 template <int ID, bool MIXED>
 void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
   constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
@@ -103,7 +104,10 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
     return int(a.nchunks < cap ? a.nchunks : cap);
   };
   if (shape == kShapeRegs) {
-    if constexpr (P::regs && P::NCH > 1) {
+    // (mixed builds run the lane-per-quartet classes in FP64: no float
+    // instantiation of these kernels)
+    if constexpr (MIXED) {
+    } else if constexpr (P::regs && P::NCH > 1) {
       // bra cut into NCH slices: groups of NCH wavefronts, one per slice
       auto kern = jk_kernel_merged<La, Lb, Lc, Ld, MIXED, P::CHA>;
       static const size_t stat = static_lds(kern);
