@@ -13,6 +13,14 @@ density_fitting set to a fitting-basis name, from the device DFFockBuilder
 (lai_dev_df_jk on the same stream), the eigenproblems from torch.linalg.eigh
 on the device.  Only the DIIS coefficients (a system of at most 7 equations)
 are solved on the host.
+
+With graph=True every iteration after the first is one replay of a captured
+hipGraph: the J/K build, the energy, DIIS and the new density, with no host
+synchronisation inside.  torch.linalg.eigh checks its status on the host, so
+inside the graph the density comes from the occupied-space projector by
+second-order spectral projection (SP2) and DIIS solves its bordered system on
+the device; the host reads the energy and density changes after each replay.
+The final orbitals come from one eigh outside the graph.
 """
 import math
 from dataclasses import dataclass
@@ -75,11 +83,85 @@ class _DIIS:
         return out
 
 
+def _solve_small(A, b):
+    """x with A x = b for a small dense A (k x k) on the device: Gauss-Jordan
+    elimination with partial pivoting, in tensor operations only (no host
+    synchronisation)."""
+    k = A.shape[0]
+    M = torch.cat([A, b.view(k, 1)], dim=1)
+    rows = torch.arange(k, device=A.device)
+    for j in range(k):
+        col = torch.abs(M[:, j]).masked_fill(rows < j, -1.0)
+        p = torch.argmax(col).view(1)
+        rj = M[j:j + 1].clone()
+        rp = torch.index_select(M, 0, p)
+        M = M.index_copy(0, p, rj)
+        M = M.index_copy(0, rows[j:j + 1], rp)
+        f = M[:, j] / M[j, j]
+        f = f.masked_fill(rows == j, 0.0)
+        M = M - f.view(k, 1) * M[j].view(1, -1)
+    return M[:, k] / torch.diagonal(M[:, :k])
+
+
+class _DeviceDIIS:
+    """Pulay DIIS with a fixed-size history on the device (a ring of `size`
+    Fock and error matrices): the same extrapolation as _DIIS, capturable."""
+
+    def __init__(self, n, size, dev):
+        self.size = size
+        self.F = torch.zeros((size, n, n), dtype=torch.float64, device=dev)
+        self.E = torch.zeros_like(self.F)
+        self.B = torch.zeros((size, size), dtype=torch.float64, device=dev)
+        self.count = torch.zeros((), dtype=torch.int64, device=dev)
+        self.ar = torch.arange(size, device=dev)
+        self.rhs = torch.zeros(size + 1, dtype=torch.float64, device=dev)
+        self.rhs[size] = -1.0
+
+    def step(self, F, err):
+        m = self.size
+        slot = torch.remainder(self.count, m).view(1)
+        self.F.index_copy_(0, slot, F.unsqueeze(0))
+        self.E.index_copy_(0, slot, err.unsqueeze(0))
+        dots = self.E.reshape(m, -1) @ err.reshape(-1)
+        self.B.index_copy_(0, slot, dots.view(1, m))
+        self.B.index_copy_(1, slot, dots.view(m, 1))
+        self.count += 1
+        v = (self.ar < self.count).to(F.dtype)
+        # the coefficients do not change when B is scaled: keep its entries O(1)
+        scale = torch.clamp_min(torch.max(torch.diagonal(self.B) * v), 1e-300)
+        A = torch.zeros((m + 1, m + 1), dtype=F.dtype, device=F.device)
+        A[:m, :m] = self.B / scale * (v.view(m, 1) * v.view(1, m)) + torch.diag(1.0 - v)
+        A[:m, m] = -v
+        A[m, :m] = -v
+        c = _solve_small(A, self.rhs)[:m]
+        return (c @ self.F.reshape(m, -1)).reshape(F.shape)
+
+
+def sp2_projector(Fp, n_occ, n_iter=80):
+    """Projector onto the n_occ lowest eigenvectors of the symmetric matrix Fp
+    by second-order spectral projection (A. M. N. Niklasson, Phys. Rev. B 66,
+    155115 (2002)): X = (hi I - Fp) / (hi - lo) with Gershgorin bounds lo, hi
+    of the spectrum, then n_iter steps X <- X^2 or 2X - X^2, whichever trace
+    lies nearer n_occ (a projector is a fixed point of both).  Needs a gap
+    between the n_occ-th and the next eigenvalue; device operations only."""
+    d = torch.diagonal(Fp)
+    r = torch.sum(torch.abs(Fp), dim=1) - torch.abs(d)
+    lo, hi = torch.min(d - r), torch.max(d + r)
+    X = (hi * torch.eye(Fp.shape[0], dtype=Fp.dtype, device=Fp.device) - Fp) / (hi - lo)
+    for _ in range(n_iter):
+        X2 = X @ X
+        t1, t2 = torch.trace(X), torch.trace(X2)
+        c = (torch.abs(t2 - n_occ) < torch.abs(2.0 * t1 - t2 - n_occ)).to(X.dtype)
+        X = c * X2 + (1.0 - c) * (2.0 * X - X2)
+    return X
+
+
 def rhf(atoms, basis_name, n_electrons, max_iter=100, e_tol=1e-12, d_tol=1e-10, options=None,
-        device="cuda", density_fitting=None):
+        device="cuda", density_fitting=None, graph=False):
     """RHF energy of [(Z, (x, y, z)), ...] (bohr) in basis_name with n_electrons (even).
     density_fitting: a fitting-basis name ("cc-pVDZ-RI", "def2-SVP-RI") for
-    density-fitted J and K, else the exact four-centre build."""
+    density-fitted J and K, else the exact four-centre build.  graph: run the
+    iterations as replays of one captured hipGraph (see the module text)."""
     if n_electrons % 2:
         raise ValueError("rhf: closed shells only (even electron count)")
     n_occ = n_electrons // 2
@@ -119,15 +201,18 @@ def rhf(atoms, basis_name, n_electrons, max_iter=100, e_tol=1e-12, d_tol=1e-10, 
 
     def fock(D):
         Dc = D.contiguous()
+        st = torch.cuda.current_stream(dev).cuda_stream  # the capture stream while capturing
         if dfb is not None:  # J and K overwritten
             # D = 2 C_occ C_occ^T is symmetric: LAI_DF_SYMMETRIC (4) computes K's upper part
-            _cabi.check(_cabi.lib().lai_dev_df_jk(dfh, Dc.data_ptr(), J.data_ptr(), K.data_ptr(), 3 | 4,
-                                                  stream.cuda_stream))
+            _cabi.check(_cabi.lib().lai_dev_df_jk(dfh, Dc.data_ptr(), J.data_ptr(), K.data_ptr(), 3 | 4, st))
         else:
             J.zero_()
             K.zero_()
-            engine.build_jk_device(Dc.data_ptr(), J.data_ptr(), K.data_ptr(), opts, stream.cuda_stream, None)
+            engine.build_jk_device(Dc.data_ptr(), J.data_ptr(), K.data_ptr(), opts, st, None)
         return H + J - 0.5 * K
+
+    if graph:
+        return _rhf_graph(fock, H, S, X, n_occ, e_nuc, max_iter, e_tol, d_tol, dev, density, solve)
 
     eps, C = solve(H)
     D = density(C)
@@ -151,4 +236,52 @@ def rhf(atoms, basis_name, n_electrons, max_iter=100, e_tol=1e-12, d_tol=1e-10, 
     # energy of the final density
     F = fock(D)
     E = float(0.5 * torch.sum((H + F) * D)) + e_nuc
+    return RHFResult(E, it, converged, eps.cpu().numpy(), C.cpu().numpy(), D.cpu().numpy())
+
+
+def _rhf_graph(fock, H, S, X, n_occ, e_nuc, max_iter, e_tol, d_tol, dev, density, solve):
+    """The SCF loop of rhf() with every iteration after the first one replay of
+    a captured graph."""
+    eps, C = solve(H)
+    D = density(C).contiguous()
+    n = D.shape[0]
+    diis = _DeviceDIIS(n, 6, dev)
+    E_prev = torch.full((), math.inf, dtype=torch.float64, device=dev)
+    out = torch.zeros(3, dtype=torch.float64, device=dev)  # E, |dE|, max |dD|
+
+    def iteration():
+        F = fock(D)
+        E = 0.5 * torch.sum((H + F) * D) + e_nuc
+        P = sp2_projector(X.T @ diis.step(F, F @ D @ S - S @ D @ F) @ X, n_occ)
+        D_new = 2.0 * X @ P @ X.T
+        out[0] = E
+        out[1] = torch.abs(E - E_prev)
+        out[2] = torch.max(torch.abs(D_new - D))
+        E_prev.copy_(E)
+        D.copy_(D_new)
+
+    # the first iteration runs eagerly on a side stream (it also creates the
+    # J/K build's worklist and streams), then one iteration is captured
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        iteration()
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize(dev)
+    it = 1
+    converged = False
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        iteration()
+    # capturing does not run the work: replays continue from the eager iteration
+    while it < max_iter:
+        g.replay()
+        it += 1
+        E, dE, dD = out.tolist()
+        if dE < e_tol and dD < d_tol:
+            converged = True
+            break
+    F = fock(D)
+    E = float(0.5 * torch.sum((H + F) * D)) + e_nuc
+    eps, C = solve(F)
     return RHFResult(E, it, converged, eps.cpu().numpy(), C.cpu().numpy(), D.cpu().numpy())
