@@ -1796,7 +1796,9 @@ extern "C" int lai_dev_autotune(lai_basis* b, const double* d_D, int flags, int 
     for (int alt = 0; alt <= (info.alt != kShapeNone ? 1 : 0) && rc == LAI_OK; ++alt) {
       const int shape = alt ? info.alt : info.shape;
       std::vector<int> caps;
-      if (shape == kShapeRegs) caps = {std::min(2 * info.nt, 2048), std::min(4 * info.nt, 2048), 2048};
+      // lane-per-quartet classes: up to whole ket runs per chunk (fewer
+      // exchange-row flushes, fewer workgroups)
+      if (shape == kShapeRegs) caps = {std::min(2 * info.nt, 2048), std::min(4 * info.nt, 2048), 2048, 4096, 8192};
       else if (shape == kShapeWaves) caps = {8, 16, 32, 64};
       else caps = {4, 8, 16, 32};
       if (alt) caps.insert(caps.begin(), 0);

@@ -72,6 +72,7 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
   __shared__ double W[NOUT];
   __shared__ double ex2[NPQ][3];  // 2 alpha_a, 2 alpha_b, 2 alpha_c of each primitive quartet
   __shared__ int cbase[NOUT], cexpo[NOUT];
+  __shared__ double nrm[NOUT];  // normalisation of each component, computed once
   __shared__ double red[NT / 64][9];
   extern __shared__ double gacc[];
   const int tid = threadIdx.x;
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
     }
     cbase[o] = bs;
     cexpo[o] = e;
+    nrm[o] = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
   }
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(NT) void grad_kernel(GradArgs g) {
       const double* D = g.D + bra.moff;
       const double w = 0.5 * D[mu * n + nu] * D[la * n + si] -
                        0.125 * g.xscale * (D[mu * n + la] * D[nu * n + si] + D[mu * n + si] * D[nu * n + la]);
-      W[o] = deg * w * norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c) * norm_corr(Ld, d);
+      W[o] = deg * w * nrm[o];
     }
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
     const int npp_b = bra.npp_sig, npp_k = ket.npp_sig;
