@@ -156,97 +156,121 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return r;
 }
 
-// F_0(T) and F_1(T): the Boys Taylor table below the one-root cut, the
+// Boys Taylor rows of the one- and two-root evaluations as the workgroup may
+// hold them: in the device table, or copied to LDS (row k at rows + k W, W =
+// 8 doubles for one root, 10 for two; rows 0 .. 16 T_cut(N)).
+using lds_cdptr = const __attribute__((address_space(3))) double*;
+template <int N>
+inline constexpr int kBoysRowW = N == 1 ? 8 : 10;
+template <int N>
+inline constexpr int kBoysRowsN = lai_rys_tcut[N] * LAI_BT_PER_UNIT + 1;
+
+// Gauss-Hermite limit of the N-root rule (T at or above the cut)
+template <int N>
+__device__ __forceinline__ void rys_hermite(double T, double* u, double* w) {
+  // 1/sqrt(T) by v_rsq_f64 and two Newton steps; 1/T is its square
+  double y = __builtin_amdgcn_rsq(T);
+  y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
+  y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
+  const double it = y * y;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    u[i] = lai_rys_hx2[lai_rys_hoff[N] + i] * it;
+    w[i] = lai_rys_hw[lai_rys_hoff[N] + i] * y;
+  }
+}
+
+// F_0(T) and F_1(T): the Boys Taylor rows below the one-root cut, the
 // Gauss-Hermite limit of the one-root rule above it (F_0 = w, F_1 = u w)
-__device__ __forceinline__ void boys01(double T, const double* __restrict__ tab, double& F0, double& F1) {
+template <typename P>
+__device__ __forceinline__ void boys01_at(double T, P rows, double& F0, double& F1) {
   if (T < double(lai_rys_tcut[1])) {
     const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
     const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
     double e[7];
 #pragma unroll
     for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
-    const double* r = tab + LAI_B8_OFF + k * 8;
     double rr[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) rr[m] = r[m];
+    for (int m = 0; m < 8; ++m) rr[m] = rows[k * 8 + m];
     F0 = boys_step(rr, 0, e);
     F1 = boys_step(rr, 1, e);
   } else {
-    double y = __builtin_amdgcn_rsq(T);
-    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
-    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
-    F0 = lai_rys_hw[lai_rys_hoff[1]] * y;
-    F1 = F0 * (lai_rys_hx2[lai_rys_hoff[1]] * (y * y));
+    double u, w;
+    rys_hermite<1>(T, &u, &w);
+    F0 = w;
+    F1 = w * u;
+  }
+}
+__device__ __forceinline__ void boys01(double T, const double* __restrict__ tab, double& F0, double& F1) {
+  boys01_at(T, tab + LAI_B8_OFF, F0, F1);
+}
+
+// One- and two-root rules: moments F_0..F_{2N-1} from the Boys Taylor rows
+// (T below the cut), then the Gauss rule of the moments in closed form
+template <int N, typename P>
+__device__ __forceinline__ void rys_roots12(double T, P rows, double* u, double* w) {
+  static_assert(N == 1 || N == 2, "one or two roots");
+  if (T < double(lai_rys_tcut[N])) {
+    const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
+    const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
+    double e[7];
+#pragma unroll
+    for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
+    double rr[2 * N + 6];  // F_0 .. F_{2N+5}: orders up to 2N-1 plus six Taylor terms
+#pragma unroll
+    for (int m = 0; m < 2 * N + 6; ++m) rr[m] = rows[k * kBoysRowW<N> + m];
+    const double F0 = boys_step(rr, 0, e), F1 = boys_step(rr, 1, e);
+    if constexpr (N == 1) {
+      u[0] = F1 * rcp_nr(F0);
+      w[0] = F0;
+    } else {
+      const double F2 = boys_step(rr, 2, e), F3 = boys_step(rr, 3, e);
+      // x^2 + p x + q orthogonal to 1 and x under the moments
+      const double idet = rcp_nr(fma(F0, F2, -F1 * F1));
+      const double p = fma(F1, F2, -F0 * F3) * idet;
+      const double q = fma(F1, F3, -F2 * F2) * idet;
+      const double x2 = 0.5 * (sqrt(fma(p, p, -4.0 * q)) - p);
+      const double x1 = q * rcp_nr(x2);
+      const double w1 = fma(-x2, F0, F1) * rcp_nr(x1 - x2);
+      u[0] = x1;
+      u[1] = x2;
+      w[0] = w1;
+      w[1] = F0 - w1;
+    }
+  } else {
+    rys_hermite<N>(T, u, w);
   }
 }
 
 template <int N>
 __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tab, double* u, double* w) {
   static_assert(N >= 1 && N <= LAI_RYS_NMAX, "Rys order out of table range");
-  if (T < double(lai_rys_tcut[N])) {
-    if constexpr (N <= 2) {
-      // moments F_0..F_{2N-1} from the Boys Taylor table (T < 45), then the
-      // Gauss rule of the moments in closed form
-      const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
-      const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
-      double e[7];
+  if constexpr (N <= 2) {
+    rys_roots12<N>(T, tab + (N == 1 ? LAI_B8_OFF : LAI_B10_OFF), u, w);
+  } else if (T < double(lai_rys_tcut[N])) {
+    const int k = int(T);
+    const double x = 2.0 * (T - double(k)) - 1.0;
+    const double x2 = 2.0 * x;
+    const double* c = tab + lai_rys_off[N] + k * (2 * N * LAI_RYS_NTERMS);
 #pragma unroll
-      for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
-      const double* r = N == 1 ? tab + LAI_B8_OFF + k * 8 : tab + LAI_B10_OFF + k * 10;
-      double rr[2 * N + 6];  // F_0 .. F_{2N+5}: orders up to 2N-1 plus six Taylor terms
+    for (int f = 0; f < 2 * N; ++f) {
+      const double* cf = c + f * LAI_RYS_NTERMS;
+      double d1 = 0.0, d2 = 0.0;
 #pragma unroll
-      for (int m = 0; m < 2 * N + 6; ++m) rr[m] = r[m];
-      const double F0 = boys_step(rr, 0, e), F1 = boys_step(rr, 1, e);
-      if constexpr (N == 1) {
-        u[0] = F1 * rcp_nr(F0);
-        w[0] = F0;
-      } else {
-        const double F2 = boys_step(rr, 2, e), F3 = boys_step(rr, 3, e);
-        // x^2 + p x + q orthogonal to 1 and x under the moments
-        const double idet = rcp_nr(fma(F0, F2, -F1 * F1));
-        const double p = fma(F1, F2, -F0 * F3) * idet;
-        const double q = fma(F1, F3, -F2 * F2) * idet;
-        const double x2 = 0.5 * (sqrt(fma(p, p, -4.0 * q)) - p);
-        const double x1 = q * rcp_nr(x2);
-        const double w1 = fma(-x2, F0, F1) * rcp_nr(x1 - x2);
-        u[0] = x1;
-        u[1] = x2;
-        w[0] = w1;
-        w[1] = F0 - w1;
+      for (int j = LAI_RYS_NTERMS - 1; j >= 1; --j) {
+        const double dk = fma(x2, d1, cf[j] - d2);
+        d2 = d1;
+        d1 = dk;
       }
-    } else {
-      const int k = int(T);
-      const double x = 2.0 * (T - double(k)) - 1.0;
-      const double x2 = 2.0 * x;
-      const double* c = tab + lai_rys_off[N] + k * (2 * N * LAI_RYS_NTERMS);
-#pragma unroll
-      for (int f = 0; f < 2 * N; ++f) {
-        const double* cf = c + f * LAI_RYS_NTERMS;
-        double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-        for (int j = LAI_RYS_NTERMS - 1; j >= 1; --j) {
-          const double dk = fma(x2, d1, cf[j] - d2);
-          d2 = d1;
-          d1 = dk;
-        }
-        const double v = fma(x, d1, cf[0] - d2);
-        if (f < N)
-          u[f] = v;
-        else
-          w[f - N] = v;
-      }
+      const double v = fma(x, d1, cf[0] - d2);
+      if (f < N)
+        u[f] = v;
+      else
+        w[f - N] = v;
     }
   } else {
-    // 1/sqrt(T) by v_rsq_f64 and two Newton steps; 1/T is its square
-    double y = __builtin_amdgcn_rsq(T);
-    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
-    y = fma(0.5 * y, fma(-T * y, y, 1.0), y);
-    const double it = y * y;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      u[i] = lai_rys_hx2[lai_rys_hoff[N] + i] * it;
-      w[i] = lai_rys_hw[lai_rys_hoff[N] + i] * y;
-    }
+    rys_hermite<N>(T, u, w);
   }
 }
 

@@ -35,24 +35,25 @@ size_t dyn_limit(K kern, size_t stat) {
 
 // Workgroup of a lane-per-quartet kernel: rep replicas of `base` lanes (at
 // most 512 lanes) share one set of the bra's exchange rows (`rows` bytes of
-// dynamic LDS) and carry `per_rep` bytes each.  Picks the rep that keeps the
-// most wavefronts resident per CU, LDS and registers counted by the runtime's
-// occupancy calculator; the rows stay in LDS whenever they fit at all (without
-// them every exchange update is an HBM atomic).
+// dynamic LDS) and `fixed` bytes, and carry `per_rep` bytes each.  Picks the
+// rep that keeps the most wavefronts resident per CU, LDS and registers
+// counted by the runtime's occupancy calculator; the rows stay in LDS whenever
+// they fit at all (without them every exchange update is an HBM atomic).
 struct RowsPick {
   int rep = 1;
   int rows = 0;    // 1: exchange rows in LDS
   size_t dyn = 0;  // dynamic LDS bytes
+  int waves = -1;  // resident wavefronts per CU (-1: nothing fits)
 };
 template <typename K>
-RowsPick pick_rows(K kern, size_t dmax, int base, size_t per_rep, size_t rows) {
+RowsPick pick_rows(K kern, size_t dmax, int base, size_t per_rep, size_t rows, size_t fixed = 0) {
   RowsPick p;
-  p.dyn = per_rep;
+  p.dyn = fixed + per_rep;
   int best = -1;
   for (int pass = 0; pass < 2 && best < 0; ++pass) {  // with the rows, else without
     const size_t rb = pass == 0 ? rows : 0;
     for (int r = 1; base * r <= 512; r *= 2) {
-      const size_t dyn = size_t(r) * per_rep + rb;
+      const size_t dyn = fixed + size_t(r) * per_rep + rb;
       if (dyn > dmax) break;
       int blocks = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern), base * r,
@@ -64,6 +65,7 @@ RowsPick pick_rows(K kern, size_t dmax, int base, size_t per_rep, size_t rows) {
         p.rep = r;
         p.rows = pass == 0 ? 1 : 0;
         p.dyn = dyn;
+        p.waves = waves;
       }
     }
   }
@@ -117,12 +119,36 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
       a.klds = rp.rows;
       hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuRegs)), dim3(64 * P::NCH * rp.rep), rp.dyn, s, a);
     } else if constexpr (P::regs) {
-      auto kern = jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA>;
+      const size_t rows = size_t(P::CHA + nb) * nbf * sizeof(double);
+      auto kern = jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA, false>;
       static const size_t stat = static_lds(kern);
       static const size_t dmax = dyn_limit(kern, stat);
-      const RowsPick rp = pick_rows(kern, dmax, 256, 0, size_t(P::CHA + nb) * nbf * sizeof(double));
+      RowsPick rp = pick_rows(kern, dmax, 256, 0, rows);
+      bool lb = false;
+      // one- and two-root classes: the Boys Taylor rows in LDS as well, when
+      // that keeps as many wavefronts resident with the exchange rows in LDS
+      // and at least two workgroups per CU (LDS reads instead of L2 gathers in
+      // the primitive loop; one workgroup per CU was measured slower)
+      if constexpr (kLdsBoysDoubles<La, Lb, Lc, Ld> > 0) {
+        auto kl = jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA, true>;
+        static const size_t stat_l = static_lds(kl);
+        static const size_t dmax_l = dyn_limit(kl, stat_l);
+        const RowsPick rl =
+            pick_rows(kl, dmax_l, 256, 0, rows, size_t(kLdsBoysDoubles<La, Lb, Lc, Ld>) * sizeof(double));
+        if (rl.rows >= rp.rows && rl.waves >= rp.waves && rl.waves >= 2 * 4 * rl.rep) {
+          rp = rl;
+          lb = true;
+        }
+      }
       a.klds = rp.rows;
       dim3 grid(grid_cap(kBlocksPerCuRegs), P::NCH);
+      if constexpr (kLdsBoysDoubles<La, Lb, Lc, Ld> > 0) {
+        if (lb) {
+          hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA, true>), grid, dim3(256 * rp.rep), rp.dyn,
+                             s, a);
+          return;
+        }
+      }
       hipLaunchKernelGGL(kern, grid, dim3(256 * rp.rep), rp.dyn, s, a);
     }
   } else if (shape == kShapeWaves) {

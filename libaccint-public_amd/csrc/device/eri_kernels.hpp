@@ -268,13 +268,16 @@ __device__ __forceinline__ int orig_offset(int perm, const int (&idx)[4], const 
 // Contracted, normalised block gv[a - A0][b][c][d] of the quartet (bra|ket);
 // all_prims selects every primitive pair (raw ERIs) or only those above the
 // primitive screening bound (J/K).
-template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1>
+// LB: one- and two-root classes read the Boys Taylor rows from the
+// workgroup's copy in LDS (lrows), else from the device table
+template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1, bool LB = false>
 __device__ __forceinline__ void quartet_block_regs(
     const EriArgs& g, const PairRec& bra, const PairRec& ket, bool all_prims,
-    double (&gv)[A1 - A0][ncart(Lb)][ncart(Lc)][ncart(Ld)]) {
+    double (&gv)[A1 - A0][ncart(Lb)][ncart(Lc)][ncart(Ld)], lds_cdptr lrows = nullptr) {
   constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NA = A1 - A0;
   constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
+  const double* grows = g.rys + (NR == 1 ? LAI_B8_OFF : LAI_B10_OFF);
   R acc[NA][nb][nc][nd];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
@@ -325,7 +328,10 @@ __device__ __forceinline__ void quartet_block_regs(
         // pref w (PA - (rho/zeta) u PQ) = pref (F0 PA - (rho/zeta) F1 PQ), so
         // F0 and F1 replace the root and weight
         double F0, F1;
-        boys01(T, g.rys, F0, F1);
+        if constexpr (LB)
+          boys01_at(T, lrows, F0, F1);
+        else
+          boys01_at(T, grows, F0, F1);
         const double s0 = pref * F0, s1 = -pref * F1 * rz;
         const double pa[3] = {P.PAx, P.PAy, P.PAz}, pq[3] = {PQx, PQy, PQz};
 #pragma unroll
@@ -338,7 +344,14 @@ __device__ __forceinline__ void quartet_block_regs(
         }
       } else {
       double u[NR], w[NR];
-      rys_roots<NR>(T, g.rys, u, w);
+      if constexpr (NR <= 2) {
+        if constexpr (LB)
+          rys_roots12<NR>(T, lrows, u, w);
+        else
+          rys_roots12<NR>(T, grows, u, w);
+      } else {
+        rys_roots<NR>(T, g.rys, u, w);
+      }
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         const double ur = u[r];
@@ -817,8 +830,9 @@ __global__ __launch_bounds__(NT) void eri_kernel_lds(EriArgs g) {
 // selects only the FP32 quartets, QuartetSel::keep == 2); otherwise FP64.
 // Lanes take the chunk's quartets in the order of sort_run.
 // --------------------------------------------------------------------------
-template <int La, int Lb, int Lc, int Ld, bool MIXED, int A0, int A1>
-__device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt) {
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int A0, int A1, bool LB = false>
+__device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
+                                              lds_cdptr lrows = nullptr) {
   constexpr int nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NA = A1 - A0;
   constexpr int ROWS = NA + nb;
@@ -861,7 +875,7 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
     if constexpr (MIXED)
       quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
     else
-      quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1>(g, bra, ket, false, gv);
+      quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1, LB>(g, bra, ket, false, gv, lrows);
     const int fc = ket.fa, fd = ket.fb;
     const bool kl = ket.same, bk = (kp == ch.bra);
     const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
@@ -965,29 +979,49 @@ __device__ __forceinline__ void jk_chunk_regs(const EriArgs& g, const JkChunk& c
   }
 }
 
-template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C, bool LB>
 __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
-                                                  int chunk) {
+                                                  int chunk, lds_cdptr lrows) {
   constexpr int na = ncart(La);
   constexpr int A0 = C * CHA;
   constexpr int A1 = (A0 + CHA < na) ? A0 + CHA : na;
   if (chunk == C) {
-    jk_chunk_regs<La, Lb, Lc, Ld, MIXED, A0, A1>(g, ch, krow, srt);
+    jk_chunk_regs<La, Lb, Lc, Ld, MIXED, A0, A1, LB>(g, ch, krow, srt, lrows);
   } else if constexpr (A1 < na) {
-    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, C + 1>(g, ch, krow, srt, chunk);
+    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, C + 1, LB>(g, ch, krow, srt, chunk, lrows);
   }
 }
 
+// Boys Taylor rows a one- or two-root class copies to LDS (doubles): rows
+// 0 .. 16 T_cut of F_0..F_7 (one root) or F_0..F_9 (two roots)
+template <int La, int Lb, int Lc, int Ld>
+inline constexpr int kLdsBoysDoubles = (La + Lb + Lc + Ld) / 2 + 1 <= 2
+    ? kBoysRowsN<(La + Lb + Lc + Ld) / 2 + 1> * kBoysRowW<(La + Lb + Lc + Ld) / 2 + 1>
+    : 0;
+
 // Kernel A, anchored: 256 or 512 lanes per chunk (the launch picks the
 // workgroup that keeps the most wavefronts resident with the exchange rows in
-// LDS), bra index chunk C = blockIdx.y.
-template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
+// LDS), bra index chunk C = blockIdx.y.  Dynamic LDS: with LB (one- and
+// two-root classes), the Boys Taylor rows, copied once per workgroup, then
+// the exchange rows [CHA + nb][mn].
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, bool LB>
 __global__ __launch_bounds__(512) void jk_kernel_regs(EriArgs g) {
-  extern __shared__ double krow[];
+  extern __shared__ double dyn[];
   __shared__ SortLds srt;
+  constexpr int NB = LB ? kLdsBoysDoubles<La, Lb, Lc, Ld> : 0;
+  static_assert(!LB || NB > 0, "LDS Boys rows are for one- and two-root classes");
+  lds_cdptr lrows = nullptr;
+  if constexpr (LB) {
+    constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
+    const double* src = g.rys + (NR == 1 ? LAI_B8_OFF : LAI_B10_OFF);
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) dyn[i] = src[i];
+    __syncthreads();
+    lrows = (lds_cdptr)(dyn);
+  }
+  double* krow = dyn + NB;
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
     const JkChunk ch = g.chunks[c];
-    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, srt, blockIdx.y);
+    dispatch_jk_chunk<La, Lb, Lc, Ld, MIXED, CHA, 0, LB>(g, ch, krow, srt, blockIdx.y, lrows);
   }
 }
 
