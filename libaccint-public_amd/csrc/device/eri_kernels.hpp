@@ -1499,8 +1499,86 @@ struct WaveShape {
   }();
 };
 
+// Split of a gathered contraction with O outputs and NU x NV terms each: S
+// lanes per output take the u rows u = part, part + S, ...; the S*O output
+// slots fill R rounds of 64 lanes.  Chosen for the fewest serial terms per
+// lane, a round counting as two terms (its index set-up and LDS add).
+struct GatherSplit {
+  int s, r;
+};
+constexpr GatherSplit gather_split(int O, int NU, int NV) {
+  GatherSplit best{1, (O + 63) / 64};
+  int best_cost = 1 << 30;
+  for (int s = 1; s <= NU; ++s) {
+    const int r = (O * s + 63) / 64;
+    const int cost = r * (((NU + s - 1) / s) * NV + 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = GatherSplit{s, r};
+    }
+  }
+  return best;
+}
+
+// One contraction of the wavefront's quartet, gathered by its outputs:
+//   out[OO + p*ND + d] += f * sum_{u<NU, v<NV} X[t*ND + d] * Dk[OD + u*DU + v*DV],
+//   t = p*SP + u*SU + v*SV  (the (a, b, c) task of the term),
+// for the NP*ND outputs (p, d).  X holds the quartet's normalised integrals
+// by task and ket function d; Dk the staged density blocks.  An output's u
+// range is split over S lanes (S-way LDS adds); S = 1 writes are exclusive.
+template <int NP, int ND, int NU, int NV, int SP, int SU, int SV, int OD, int DU, int DV, int OO>
+__device__ __forceinline__ void wave_gather(const double* X, const double* Dk, double* out, double f, int lane) {
+  constexpr int O = NP * ND;
+  constexpr GatherSplit sp = gather_split(O, NU, NV);
+  constexpr int S = sp.s, R = sp.r, F = NU / S, REM = NU % S;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int slot = lane + 64 * r;
+    const int o = slot % O, part = slot / O;
+    if (part < S) {
+      const int p = o / ND, d = o - p * ND;
+      const double* x = X + (p * SP + part * SU) * ND + d;
+      const double* dk = Dk + OD + part * DU;
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc += x[(S * i * SU + v * SV) * ND] * dk[S * i * DU + v * DV];
+      if constexpr (REM > 0) {
+        if (part < REM) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) acc += x[(S * F * SU + v * SV) * ND] * dk[S * F * DU + v * DV];
+        }
+      }
+      if constexpr (S == 1)
+        out[OO + o] += f * acc;
+      else
+        lds_add(&out[OO + o], f * acc);
+    }
+  }
+}
+
+// LDS of the wavefront kernel: per wave the tables (reused for the quartet's
+// integrals) and the J/K and density blocks; the wavefronts per SIMD that it
+// leaves room for bound the registers the compiler may take.
+template <int La, int Lb, int Lc, int Ld, int NW>
+struct WaveLds {
+  using Ws = WaveShape<La, Lb, Lc, Ld>;
+  static constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  static constexpr int NBLK = na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
+  static constexpr int TABN = Ws::NPQ * Ws::TS > Ws::NJ * nd * 64 ? Ws::NPQ * Ws::TS : Ws::NJ * nd * 64;
+  static constexpr int bytes = NW * (TABN + 2 * NBLK) * 8;
+  static constexpr int wpe_lds = (160 * 1024 / bytes) * NW / 4;
+  // three wavefronts per SIMD (168 registers) for at most 9 accumulators per
+  // lane in at most 3 task slots, two (256) otherwise: the products' register
+  // demand grows with the accumulators
+  static constexpr int wpe_regs = (Ws::NJ * nd <= 9 && Ws::NJ <= 3) ? 3 : 2;
+  static constexpr int wpe = wpe_lds < wpe_regs ? wpe_lds : wpe_regs;
+  static constexpr int WPE = wpe < 1 ? 1 : wpe;
+};
+
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int NW>
-__global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
+__global__ __launch_bounds__(64 * NW, (WaveLds<La, Lb, Lc, Ld, NW>::WPE)) void jk_kernel_waves(EriArgs g) {
   using Ws = WaveShape<La, Lb, Lc, Ld>;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NTASK = Ws::NTASK, NJ = Ws::NJ, NPQ = Ws::NPQ;
@@ -1508,7 +1586,7 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
                 OBD = OBC + nb * nc, NBLK = OBD + nb * nd;
   // per wave: the (primitive quartet, root, axis) tables, then the quartet's
   // normalised integrals for the digestion
-  constexpr int TABN = NPQ * Ws::TS > NJ * nd * 64 ? NPQ * Ws::TS : NJ * nd * 64;
+  constexpr int TABN = WaveLds<La, Lb, Lc, Ld, NW>::TABN;
   __shared__ double tab[NW][TABN];
   __shared__ double blk[NW][NBLK];   // per wave: the quartet's J/K blocks; Jt[ab] summed over the chunk
   __shared__ double dblk[NW][NBLK];  // per wave: the quartet's density blocks, laid out like blk
@@ -1526,6 +1604,23 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
   double* bw = blk[wave];
   double* dw = dblk[wave];
   for (int o = lane; o < NBLK; o += 64) bw[o] = 0.0;  // ordered by the first wave_sync below
+  // the density element this lane stages in each of its NDS slots o =
+  // lane + 64*i: row fr[rs] + ro, column fr[cs] + co of the shells' first
+  // functions fr = (fa, fb, fc, fd), packed rs | cs << 2 | ro << 4 | co << 8
+  constexpr int NDS = (NBLK + 63) / 64;
+  int dsl[NDS];
+#pragma unroll
+  for (int i = 0; i < NDS; ++i) {
+    const int o = lane + 64 * i;
+    int rs, cs, ro, co;
+    if (o < OCD) { rs = 0; cs = 1; ro = (o - OAB) / nb; co = (o - OAB) % nb; }
+    else if (o < OAC) { rs = 2; cs = 3; ro = (o - OCD) / nd; co = (o - OCD) % nd; }
+    else if (o < OAD) { rs = 0; cs = 2; ro = (o - OAC) / nc; co = (o - OAC) % nc; }
+    else if (o < OBC) { rs = 0; cs = 3; ro = (o - OAD) / nd; co = (o - OAD) % nd; }
+    else if (o < OBD) { rs = 1; cs = 2; ro = (o - OBC) / nc; co = (o - OBC) % nc; }
+    else { rs = 1; cs = 3; ro = (o - OBD) / nd; co = (o - OBD) % nd; }
+    dsl[i] = o < NBLK ? (rs | cs << 2 | ro << 4 | co << 8) : -1;
+  }
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
@@ -1544,6 +1639,20 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
       const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
       if (mode == 0) continue;
       const int ntot = bra.npp_sig * ket.npp_sig;
+      const int fc = ket.fa, fd = ket.fb;
+      // the quartet's six density blocks, loaded now and staged after the
+      // products so that the loads' latency is hidden behind them
+      double dpre[NDS];
+#pragma unroll
+      for (int i = 0; i < NDS; ++i) {
+        const int pk = dsl[i];
+        if (pk >= 0) {
+          const int rs = pk & 3, cs = (pk >> 2) & 3;
+          const int r = (rs == 0 ? fa : rs == 1 ? fb : fc) + ((pk >> 4) & 15);
+          const int q = (cs == 1 ? fb : cs == 2 ? fc : fd) + (pk >> 8);
+          dpre[i] = D[size_t(r) * n + q];
+        }
+      }
       double v[NJ][nd];
       if constexpr (MIXED) {
         float acc[NJ][nd];
@@ -1561,58 +1670,48 @@ __global__ __launch_bounds__(64 * NW) void jk_kernel_waves(EriArgs g) {
 #pragma unroll
           for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       }
-      const int fc = ket.fa, fd = ket.fb;
       const bool kl = ket.same, bk = (kp == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
       const double fK = bk ? 1.0 : 2.0;
-      // the quartet's six density blocks D[xy], staged in this wave's LDS at
-      // the offsets of the J/K blocks of the same shape (Oxy)
-      for (int o = lane; o < NBLK; o += 64) {
-        size_t r, q;
-        if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; }
-        else if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; }
-        else if (o < OAD) { r = fa + (o - OAC) / nc; q = fc + (o - OAC) % nc; }
-        else if (o < OBC) { r = fa + (o - OAD) / nd; q = fd + (o - OAD) % nd; }
-        else if (o < OBD) { r = fb + (o - OBC) / nc; q = fc + (o - OBC) % nc; }
-        else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
-        dw[o] = D[r * n + q];
-      }
-      // the normalised integrals of this lane's tasks, through the wave's
-      // table storage (free after the products), so that the digestion can
-      // take one task at a time
+      // the six density blocks D[xy] at the offsets of the J/K blocks of the
+      // same shape (Oxy)
+#pragma unroll
+      for (int i = 0; i < NDS; ++i)
+        if (dsl[i] >= 0) dw[lane + 64 * i] = dpre[i];
+      // the normalised integrals X[t*nd + d] of task t = (a*nb + b)*nc + c,
+      // through the wave's table storage (free after the products)
       double* xw = tab[wave];
 #pragma unroll
       for (int j = 0; j < NJ; ++j)
+        if (lane + 64 * j < NTASK)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) xw[(j * nd + d) * 64 + lane] = v[j][d] * sabc[j] * norm_corr(Ld, d);
+          for (int d = 0; d < nd; ++d) xw[(lane + 64 * j) * nd + d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
       wave_sync();
+      // contractions summed over d within a task: J[ab] (D[cd]), K[ac]
+      // (D[bd]), K[bc] (D[ad]); an output is shared by at most nc, nb or na
+      // lanes of an LDS add
 #pragma unroll 1
       for (int j = 0; j < NJ; ++j) {
         const int t = lane + 64 * j;
         if (t >= NTASK) continue;
         const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-        double x[nd];
-#pragma unroll
-        for (int d = 0; d < nd; ++d) x[d] = xw[(j * nd + d) * 64 + lane];
-        // per-task sums over d: J[ab] (D[cd]), K[ac] (D[bd]), K[bc] (D[ad])
         double sj = 0.0, s1 = 0.0, s3 = 0.0;
 #pragma unroll
         for (int d = 0; d < nd; ++d) {
-          sj += x[d] * dw[OCD + c * nd + d];
-          s1 += x[d] * dw[OBD + b * nd + d];
-          s3 += x[d] * dw[OAD + a * nd + d];
+          const double x = xw[t * nd + d];
+          sj += x * dw[OCD + c * nd + d];
+          s1 += x * dw[OBD + b * nd + d];
+          s3 += x * dw[OAD + a * nd + d];
         }
-        const double dab = dw[OAB + a * nb + b], dbc = dw[OBC + b * nc + c], dac = dw[OAC + a * nc + c];
         lds_add(&bw[OAB + a * nb + b], fJ * sj);
         lds_add(&bw[OAC + a * nc + c], fK * s1);
         if (!ij) lds_add(&bw[OBC + b * nc + c], fK * s3);
-#pragma unroll
-        for (int d = 0; d < nd; ++d) {
-          if (!bk) lds_add(&bw[OCD + c * nd + d], fJ * x[d] * dab);
-          if (!kl) lds_add(&bw[OAD + a * nd + d], fK * x[d] * dbc);
-          if (!ij && !kl) lds_add(&bw[OBD + b * nd + d], fK * x[d] * dac);
-        }
       }
+      // contractions with an output per ket function d, gathered by output:
+      // J[cd] = sum_ab X D[ab], K[ad] = sum_bc X D[bc], K[bd] = sum_ac X D[ac]
+      if (!bk) wave_gather<nc, nd, na, nb, 1, nb * nc, nc, OAB, nb, 1, OCD>(xw, dw, bw, fJ, lane);
+      if (!kl) wave_gather<na, nd, nb, nc, nb * nc, nc, 1, OBC, nc, 1, OAD>(xw, dw, bw, fK, lane);
+      if (!ij && !kl) wave_gather<nb, nd, na, nc, nc, nb * nc, 1, OAC, nc, 1, OBD>(xw, dw, bw, fK, lane);
       wave_sync();  // the wave's quartet block is complete
       // this quartet's ket and exchange blocks to HBM, one atomic per element;
       // Jt[ab] stays in LDS for the rest of the chunk
