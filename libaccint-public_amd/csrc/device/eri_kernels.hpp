@@ -1234,9 +1234,71 @@ __global__ __launch_bounds__(512) void jk_kernel_merged(EriArgs g) {
   }
 }
 
+// Split of a gathered contraction with O outputs and NU x NV terms each: S
+// lanes per output take the u rows u = part, part + S, ...; the S*O output
+// slots fill R rounds of NL lanes.  Chosen for the fewest serial terms per
+// lane, a round counting as two terms (its index set-up and LDS add).
+struct GatherSplit {
+  int s, r;
+};
+constexpr GatherSplit gather_split(int O, int NU, int NV, int NL) {
+  GatherSplit best{1, (O + NL - 1) / NL};
+  int best_cost = 1 << 30;
+  for (int s = 1; s <= NU; ++s) {
+    const int r = (O * s + NL - 1) / NL;
+    const int cost = r * (((NU + s - 1) / s) * NV + 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = GatherSplit{s, r};
+    }
+  }
+  return best;
+}
+
+// One contraction of a quartet by NL lanes (a wavefront or a workgroup),
+// gathered by its outputs:
+//   out[OO + p*ND + d] += f * sum_{u<NU, v<NV} X[t*ND + d] * Dk[OD + u*DU + v*DV],
+//   t = p*SP + u*SU + v*SV  (the (a, b, c) task of the term),
+// for the NP*ND outputs (p, d).  X holds the quartet's normalised integrals
+// by task and ket function d; Dk the staged density blocks.  An output's u
+// range is split over S lanes (S-way LDS adds); S = 1 writes are exclusive.
+template <int NL, int NP, int ND, int NU, int NV, int SP, int SU, int SV, int OD, int DU, int DV, int OO>
+__device__ __forceinline__ void gather_contract(const double* X, const double* Dk, double* out, double f, int lane) {
+  constexpr int O = NP * ND;
+  constexpr GatherSplit sp = gather_split(O, NU, NV, NL);
+  constexpr int S = sp.s, R = sp.r, F = NU / S, REM = NU % S;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int slot = lane + NL * r;
+    const int o = slot % O, part = slot / O;
+    if (part < S) {
+      const int p = o / ND, d = o - p * ND;
+      const double* x = X + (p * SP + part * SU) * ND + d;
+      const double* dk = Dk + OD + part * DU;
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc += x[(S * i * SU + v * SV) * ND] * dk[S * i * DU + v * DV];
+      if constexpr (REM > 0) {
+        if (part < REM) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) acc += x[(S * F * SU + v * SV) * ND] * dk[S * F * DU + v * DV];
+        }
+      }
+      if constexpr (S == 1)
+        out[OO + o] += f * acc;
+      else
+        lds_add(&out[OO + o], f * acc);
+    }
+  }
+}
+
 // Kernel B: one workgroup per chunk, the chunk's quartets one after another.
 // Each quartet's ket Coulomb and exchange blocks go to HBM with one atomic per
 // element; the bra Coulomb block Jt[ab] is summed in LDS over the chunk.
+// Where the quartet's integrals fit in the table storage, the contractions
+// with an output per ket function d are gathered by output (gather_contract).
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int NT>
 __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   using Sh = LdsShape<La, Lb, Lc, Ld, double>;
@@ -1248,7 +1310,8 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   // they are large enough, then the quartet's normalised integrals, so that
   // the digestion takes one task at a time (fewer registers)
   __shared__ double tab[NPQ * Sh::TSP];
-  constexpr bool XLDS = NJ * nd * NT <= NPQ * Sh::TSP;
+  // X[t*nd + d] by task t = (a*nb + b)*nc + c, for the gathered contractions
+  constexpr bool XLDS = NTASK * nd <= NPQ * Sh::TSP;
   constexpr int XUNROLL = XLDS ? 1 : NJ;
   // the quartet's six J/K blocks, summed in LDS and added to HBM once per
   // quartet (Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd]) or once per chunk (Jt[ab])
@@ -1324,8 +1387,9 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
       if constexpr (XLDS) {
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
+          if (tid + NT * j < NTASK)
 #pragma unroll
-          for (int d = 0; d < nd; ++d) tab[(j * nd + d) * NT + tid] = v[j][d] * sabc[j] * norm_corr(Ld, d);
+            for (int d = 0; d < nd; ++d) tab[(tid + NT * j) * nd + d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
       }
       __syncthreads();
 #pragma unroll XUNROLL
@@ -1337,7 +1401,7 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
 #pragma unroll
         for (int d = 0; d < nd; ++d) {
           if constexpr (XLDS)
-            x[d] = tab[(j * nd + d) * NT + tid];
+            x[d] = tab[t * nd + d];
           else
             x[d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
         }
@@ -1349,16 +1413,28 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
           s1 += x[d] * dblk[OBD + b * nd + d];
           s3 += x[d] * dblk[OAD + a * nd + d];
         }
-        const double dab = dblk[OAB + a * nb + b], dbc = dblk[OBC + b * nc + c], dac = dblk[OAC + a * nc + c];
         lds_add(&blk[OAB + a * nb + b], fJ * sj);
         lds_add(&blk[OAC + a * nc + c], fK * s1);
         if (!ij) lds_add(&blk[OBC + b * nc + c], fK * s3);
+        if constexpr (!XLDS) {
+          // the per-d contractions scattered: an output shared by up to
+          // nb*nc lanes of an LDS add
+          const double dab = dblk[OAB + a * nb + b], dbc = dblk[OBC + b * nc + c], dac = dblk[OAC + a * nc + c];
 #pragma unroll
-        for (int d = 0; d < nd; ++d) {
-          if (!bk) lds_add(&blk[OCD + c * nd + d], fJ * x[d] * dab);
-          if (!kl) lds_add(&blk[OAD + a * nd + d], fK * x[d] * dbc);
-          if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], fK * x[d] * dac);
+          for (int d = 0; d < nd; ++d) {
+            if (!bk) lds_add(&blk[OCD + c * nd + d], fJ * x[d] * dab);
+            if (!kl) lds_add(&blk[OAD + a * nd + d], fK * x[d] * dbc);
+            if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], fK * x[d] * dac);
+          }
         }
+      }
+      if constexpr (XLDS) {
+        // the per-d contractions gathered by output: J[cd] = sum_ab X D[ab],
+        // K[ad] = sum_bc X D[bc], K[bd] = sum_ac X D[ac]
+        if (!bk) gather_contract<NT, nc, nd, na, nb, 1, nb * nc, nc, OAB, nb, 1, OCD>(tab, dblk, blk, fJ, tid);
+        if (!kl) gather_contract<NT, na, nd, nb, nc, nb * nc, nc, 1, OBC, nc, 1, OAD>(tab, dblk, blk, fK, tid);
+        if (!ij && !kl)
+          gather_contract<NT, nb, nd, na, nc, nc, nb * nc, 1, OAC, nc, 1, OBD>(tab, dblk, blk, fK, tid);
       }
       __syncthreads();
       // the quartet's ket and exchange blocks to HBM, one atomic per element
@@ -1498,65 +1574,6 @@ struct WaveShape {
     return n < 1 ? 1 : n;
   }();
 };
-
-// Split of a gathered contraction with O outputs and NU x NV terms each: S
-// lanes per output take the u rows u = part, part + S, ...; the S*O output
-// slots fill R rounds of 64 lanes.  Chosen for the fewest serial terms per
-// lane, a round counting as two terms (its index set-up and LDS add).
-struct GatherSplit {
-  int s, r;
-};
-constexpr GatherSplit gather_split(int O, int NU, int NV) {
-  GatherSplit best{1, (O + 63) / 64};
-  int best_cost = 1 << 30;
-  for (int s = 1; s <= NU; ++s) {
-    const int r = (O * s + 63) / 64;
-    const int cost = r * (((NU + s - 1) / s) * NV + 2);
-    if (cost < best_cost) {
-      best_cost = cost;
-      best = GatherSplit{s, r};
-    }
-  }
-  return best;
-}
-
-// One contraction of the wavefront's quartet, gathered by its outputs:
-//   out[OO + p*ND + d] += f * sum_{u<NU, v<NV} X[t*ND + d] * Dk[OD + u*DU + v*DV],
-//   t = p*SP + u*SU + v*SV  (the (a, b, c) task of the term),
-// for the NP*ND outputs (p, d).  X holds the quartet's normalised integrals
-// by task and ket function d; Dk the staged density blocks.  An output's u
-// range is split over S lanes (S-way LDS adds); S = 1 writes are exclusive.
-template <int NP, int ND, int NU, int NV, int SP, int SU, int SV, int OD, int DU, int DV, int OO>
-__device__ __forceinline__ void wave_gather(const double* X, const double* Dk, double* out, double f, int lane) {
-  constexpr int O = NP * ND;
-  constexpr GatherSplit sp = gather_split(O, NU, NV);
-  constexpr int S = sp.s, R = sp.r, F = NU / S, REM = NU % S;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int slot = lane + 64 * r;
-    const int o = slot % O, part = slot / O;
-    if (part < S) {
-      const int p = o / ND, d = o - p * ND;
-      const double* x = X + (p * SP + part * SU) * ND + d;
-      const double* dk = Dk + OD + part * DU;
-      double acc = 0.0;
-#pragma unroll
-      for (int i = 0; i < F; ++i)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc += x[(S * i * SU + v * SV) * ND] * dk[S * i * DU + v * DV];
-      if constexpr (REM > 0) {
-        if (part < REM) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) acc += x[(S * F * SU + v * SV) * ND] * dk[S * F * DU + v * DV];
-        }
-      }
-      if constexpr (S == 1)
-        out[OO + o] += f * acc;
-      else
-        lds_add(&out[OO + o], f * acc);
-    }
-  }
-}
 
 // LDS of the wavefront kernel: per wave the tables (reused for the quartet's
 // integrals) and the J/K and density blocks; the wavefronts per SIMD that it
@@ -1709,9 +1726,9 @@ __global__ __launch_bounds__(64 * NW, (WaveLds<La, Lb, Lc, Ld, NW>::WPE)) void j
       }
       // contractions with an output per ket function d, gathered by output:
       // J[cd] = sum_ab X D[ab], K[ad] = sum_bc X D[bc], K[bd] = sum_ac X D[ac]
-      if (!bk) wave_gather<nc, nd, na, nb, 1, nb * nc, nc, OAB, nb, 1, OCD>(xw, dw, bw, fJ, lane);
-      if (!kl) wave_gather<na, nd, nb, nc, nb * nc, nc, 1, OBC, nc, 1, OAD>(xw, dw, bw, fK, lane);
-      if (!ij && !kl) wave_gather<nb, nd, na, nc, nc, nb * nc, 1, OAC, nc, 1, OBD>(xw, dw, bw, fK, lane);
+      if (!bk) gather_contract<64, nc, nd, na, nb, 1, nb * nc, nc, OAB, nb, 1, OCD>(xw, dw, bw, fJ, lane);
+      if (!kl) gather_contract<64, na, nd, nb, nc, nb * nc, nc, 1, OBC, nc, 1, OAD>(xw, dw, bw, fK, lane);
+      if (!ij && !kl) gather_contract<64, nb, nd, na, nc, nc, nb * nc, 1, OAC, nc, 1, OBD>(xw, dw, bw, fK, lane);
       wave_sync();  // the wave's quartet block is complete
       // this quartet's ket and exchange blocks to HBM, one atomic per element;
       // Jt[ab] stays in LDS for the rest of the chunk
