@@ -1299,8 +1299,18 @@ __device__ __forceinline__ void gather_contract(const double* X, const double* D
 // element; the bra Coulomb block Jt[ab] is summed in LDS over the chunk.
 // Where the quartet's integrals fit in the table storage, the contractions
 // with an output per ket function d are gathered by output (gather_contract).
+// Wavefronts per SIMD that bound the registers of the workgroup kernel: three
+// for at most 4 accumulators per lane, two up to the largest classes whose
+// integrals fit in LDS, one for the rest.
+template <int La, int Lb, int Lc, int Ld, int NT>
+constexpr int lds_kernel_wpe() {
+  constexpr int NJ = (LdsTasks<La, Lb, Lc, Ld>::NTASK + NT - 1) / NT;
+  constexpr int nd = ncart(Ld);
+  return NJ * nd <= 4 ? 3 : (LdsTasks<La, Lb, Lc, Ld>::NTASK * nd <= 6144 ? 2 : 1);
+}
+
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int NT>
-__global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
+__global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk_kernel_lds(EriArgs g) {
   using Sh = LdsShape<La, Lb, Lc, Ld, double>;
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
@@ -1309,9 +1319,13 @@ __global__ __launch_bounds__(NT) void jk_kernel_lds(EriArgs g) {
   // FP64 tables (the FP32 recursion uses the same storage as float); where
   // they are large enough, then the quartet's normalised integrals, so that
   // the digestion takes one task at a time (fewer registers)
-  __shared__ double tab[NPQ * Sh::TSP];
-  // X[t*nd + d] by task t = (a*nb + b)*nc + c, for the gathered contractions
-  constexpr bool XLDS = NTASK * nd <= NPQ * Sh::TSP;
+  // the tables, then X[t*nd + d] by task t = (a*nb + b)*nc + c for the
+  // gathered contractions: the storage grows to hold X up to 48 KiB (these
+  // classes' register demand leaves at most two workgroups per CU)
+  constexpr int XN = NTASK * nd;
+  constexpr int TABN = NPQ * Sh::TSP > XN || XN > 6144 ? NPQ * Sh::TSP : XN;
+  __shared__ double tab[TABN];
+  constexpr bool XLDS = XN <= TABN;
   constexpr int XUNROLL = XLDS ? 1 : NJ;
   // the quartet's six J/K blocks, summed in LDS and added to HBM once per
   // quartet (Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd]) or once per chunk (Jt[ab])
