@@ -243,19 +243,20 @@ __device__ __forceinline__ void rys_roots12(double T, P rows, double* u, double*
   }
 }
 
-template <int N>
-__device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tab, double* u, double* w) {
-  static_assert(N >= 1 && N <= LAI_RYS_NMAX, "Rys order out of table range");
-  if constexpr (N <= 2) {
-    rys_roots12<N>(T, tab + (N == 1 ? LAI_B8_OFF : LAI_B10_OFF), u, w);
-  } else if (T < double(lai_rys_tcut[N])) {
+// Chebyshev evaluation of the N-root rule (N >= 3) below its cut: interval
+// k = floor(T) holds 2N series of LAI_RYS_NTERMS coefficients (the N roots,
+// then the N weights) at c0 + k*stride, in the device table (stride 2N terms)
+// or in a workgroup's LDS copy (padded stride); the Gauss-Hermite limit above.
+template <int N, typename P>
+__device__ __forceinline__ void rys_cheb(double T, P c0, int stride, double* u, double* w) {
+  if (T < double(lai_rys_tcut[N])) {
     const int k = int(T);
     const double x = 2.0 * (T - double(k)) - 1.0;
     const double x2 = 2.0 * x;
-    const double* c = tab + lai_rys_off[N] + k * (2 * N * LAI_RYS_NTERMS);
+    const P c = c0 + k * stride;
 #pragma unroll
     for (int f = 0; f < 2 * N; ++f) {
-      const double* cf = c + f * LAI_RYS_NTERMS;
+      const P cf = c + f * LAI_RYS_NTERMS;
       double d1 = 0.0, d2 = 0.0;
 #pragma unroll
       for (int j = LAI_RYS_NTERMS - 1; j >= 1; --j) {
@@ -272,6 +273,20 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
   } else {
     rys_hermite<N>(T, u, w);
   }
+}
+
+// Stride of an interval's coefficients in the LDS copy: odd, so that lanes
+// of neighbouring intervals start in different banks
+template <int N>
+inline constexpr int kChebStrideLds = 2 * N * LAI_RYS_NTERMS + 1;
+
+template <int N>
+__device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tab, double* u, double* w) {
+  static_assert(N >= 1 && N <= LAI_RYS_NMAX, "Rys order out of table range");
+  if constexpr (N <= 2)
+    rys_roots12<N>(T, tab + (N == 1 ? LAI_B8_OFF : LAI_B10_OFF), u, w);
+  else
+    rys_cheb<N>(T, tab + lai_rys_off[N], 2 * N * LAI_RYS_NTERMS, u, w);
 }
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }

@@ -111,11 +111,26 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
     if constexpr (MIXED) {
     } else if constexpr (P::regs && P::NCH > 1) {
       // bra cut into NCH slices: groups of NCH wavefronts, one per slice
-      auto kern = jk_kernel_merged<La, Lb, Lc, Ld, MIXED, P::CHA>;
+      auto kern = jk_kernel_merged<La, Lb, Lc, Ld, MIXED, P::CHA, false>;
       static const size_t stat = static_lds(kern);
       static const size_t dmax = dyn_limit(kern, stat);
-      const RowsPick rp = pick_rows(kern, dmax, 64 * P::NCH, size_t(P::NCH) * nc * nd * 64 * sizeof(double),
-                                    size_t(na + nb) * nbf * sizeof(double));
+      const size_t per_rep = size_t(P::NCH) * nc * nd * 64 * sizeof(double);
+      const size_t rows = size_t(na + nb) * nbf * sizeof(double);
+      RowsPick rp = pick_rows(kern, dmax, 64 * P::NCH, per_rep, rows);
+      // the Rys tables in LDS as well (one to four roots) when that keeps as
+      // many wavefronts resident and the exchange rows still in LDS
+      if constexpr (kLdsRysDoubles<La, Lb, Lc, Ld> > 0) {
+        auto kl = jk_kernel_merged<La, Lb, Lc, Ld, MIXED, P::CHA, true>;
+        static const size_t stat_l = static_lds(kl);
+        static const size_t dmax_l = dyn_limit(kl, stat_l);
+        const RowsPick rl =
+            pick_rows(kl, dmax_l, 64 * P::NCH, per_rep, rows, size_t(kLdsRysDoubles<La, Lb, Lc, Ld>) * sizeof(double));
+        if (rl.rows >= rp.rows && rl.waves >= rp.waves) {
+          a.klds = rl.rows;
+          hipLaunchKernelGGL(kl, dim3(grid_cap(kBlocksPerCuRegs)), dim3(64 * P::NCH * rl.rep), rl.dyn, s, a);
+          return;
+        }
+      }
       a.klds = rp.rows;
       hipLaunchKernelGGL(kern, dim3(grid_cap(kBlocksPerCuRegs)), dim3(64 * P::NCH * rp.rep), rp.dyn, s, a);
     } else if constexpr (P::regs) {
@@ -125,16 +140,17 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
       static const size_t dmax = dyn_limit(kern, stat);
       RowsPick rp = pick_rows(kern, dmax, 256, 0, rows);
       bool lb = false;
-      // one- and two-root classes: the Boys Taylor rows in LDS as well, when
-      // that keeps as many wavefronts resident with the exchange rows in LDS
-      // and at least two workgroups per CU (LDS reads instead of L2 gathers in
-      // the primitive loop; one workgroup per CU was measured slower)
-      if constexpr (kLdsBoysDoubles<La, Lb, Lc, Ld> > 0) {
+      // one- to four-root classes: the Rys tables (Boys Taylor rows or
+      // Chebyshev coefficients) in LDS as well, when that keeps as many
+      // wavefronts resident with the exchange rows in LDS and at least two
+      // workgroups per CU (LDS reads instead of L2 gathers in the primitive
+      // loop; one workgroup per CU was measured slower)
+      if constexpr (kLdsRysDoubles<La, Lb, Lc, Ld> > 0) {
         auto kl = jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA, true>;
         static const size_t stat_l = static_lds(kl);
         static const size_t dmax_l = dyn_limit(kl, stat_l);
         const RowsPick rl =
-            pick_rows(kl, dmax_l, 256, 0, rows, size_t(kLdsBoysDoubles<La, Lb, Lc, Ld>) * sizeof(double));
+            pick_rows(kl, dmax_l, 256, 0, rows, size_t(kLdsRysDoubles<La, Lb, Lc, Ld>) * sizeof(double));
         if (rl.rows >= rp.rows && rl.waves >= rp.waves && rl.waves >= 2 * 4 * rl.rep) {
           rp = rl;
           lb = true;
@@ -142,7 +158,7 @@ void launch_class(const EriArgs& a0, hipStream_t s, const LaunchTune& t) {
       }
       a.klds = rp.rows;
       dim3 grid(grid_cap(kBlocksPerCuRegs), P::NCH);
-      if constexpr (kLdsBoysDoubles<La, Lb, Lc, Ld> > 0) {
+      if constexpr (kLdsRysDoubles<La, Lb, Lc, Ld> > 0) {
         if (lb) {
           hipLaunchKernelGGL((jk_kernel_regs<La, Lb, Lc, Ld, MIXED, P::CHA, true>), grid, dim3(256 * rp.rep), rp.dyn,
                              s, a);

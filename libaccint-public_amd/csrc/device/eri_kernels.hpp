@@ -268,8 +268,9 @@ __device__ __forceinline__ int orig_offset(int perm, const int (&idx)[4], const 
 // Contracted, normalised block gv[a - A0][b][c][d] of the quartet (bra|ket);
 // all_prims selects every primitive pair (raw ERIs) or only those above the
 // primitive screening bound (J/K).
-// LB: one- and two-root classes read the Boys Taylor rows from the
-// workgroup's copy in LDS (lrows), else from the device table
+// LB: the Boys Taylor rows (one and two roots) or the Chebyshev coefficients
+// of the rule (three and four roots) from the workgroup's copy in LDS
+// (lrows), else from the device table
 template <int La, int Lb, int Lc, int Ld, typename R, int A0, int A1, bool LB = false>
 __device__ __forceinline__ void quartet_block_regs(
     const EriArgs& g, const PairRec& bra, const PairRec& ket, bool all_prims,
@@ -349,6 +350,8 @@ __device__ __forceinline__ void quartet_block_regs(
           rys_roots12<NR>(T, lrows, u, w);
         else
           rys_roots12<NR>(T, grows, u, w);
+      } else if constexpr (LB) {
+        rys_cheb<NR>(T, lrows, kChebStrideLds<NR>, u, w);
       } else {
         rys_roots<NR>(T, g.rys, u, w);
       }
@@ -992,30 +995,52 @@ __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChun
   }
 }
 
-// Boys Taylor rows a one- or two-root class copies to LDS (doubles): rows
-// 0 .. 16 T_cut of F_0..F_7 (one root) or F_0..F_9 (two roots)
+// The Rys tables a class may copy to LDS (doubles): for one and two roots
+// the Boys Taylor rows 0 .. 16 T_cut of F_0..F_7 or F_0..F_9; for three and
+// four roots the Chebyshev coefficients of the rule below its cut (34 and 51
+// KB, interval stride padded to kChebStrideLds); none for more roots.
+constexpr int lds_rys_doubles(int NR) {
+  return NR == 1   ? kBoysRowsN<1> * kBoysRowW<1>
+         : NR == 2 ? kBoysRowsN<2> * kBoysRowW<2>
+         : NR == 3 ? lai_rys_tcut[3] * kChebStrideLds<3>
+         : NR == 4 ? lai_rys_tcut[4] * kChebStrideLds<4>
+                   : 0;
+}
 template <int La, int Lb, int Lc, int Ld>
-inline constexpr int kLdsBoysDoubles = (La + Lb + Lc + Ld) / 2 + 1 <= 2
-    ? kBoysRowsN<(La + Lb + Lc + Ld) / 2 + 1> * kBoysRowW<(La + Lb + Lc + Ld) / 2 + 1>
-    : 0;
+inline constexpr int kLdsRysDoubles = lds_rys_doubles((La + Lb + Lc + Ld) / 2 + 1);
+
+// Copies the class's Rys tables into dst (kLdsRysDoubles doubles) with the
+// workgroup's lanes; ends with a workgroup barrier.
+template <int NR>
+__device__ __forceinline__ void copy_rys_lds(const double* __restrict__ rys, double* dst) {
+  constexpr int NB = lds_rys_doubles(NR);
+  if constexpr (NR <= 2) {
+    const double* src = rys + (NR == 1 ? LAI_B8_OFF : LAI_B10_OFF);
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) dst[i] = src[i];
+  } else {
+    constexpr int W = 2 * NR * LAI_RYS_NTERMS, ST = kChebStrideLds<NR>;
+    const double* src = rys + lai_rys_off[NR];
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) {
+      const int k = i / ST, r = i - k * ST;
+      dst[i] = r < W ? src[k * W + r] : 0.0;
+    }
+  }
+  __syncthreads();
+}
 
 // Kernel A, anchored: 256 or 512 lanes per chunk (the launch picks the
 // workgroup that keeps the most wavefronts resident with the exchange rows in
-// LDS), bra index chunk C = blockIdx.y.  Dynamic LDS: with LB (one- and
-// two-root classes), the Boys Taylor rows, copied once per workgroup, then
-// the exchange rows [CHA + nb][mn].
+// LDS), bra index chunk C = blockIdx.y.  Dynamic LDS: with LB, the class's
+// Rys tables, copied once per workgroup, then the exchange rows [CHA + nb][mn].
 template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, bool LB>
 __global__ __launch_bounds__(512) void jk_kernel_regs(EriArgs g) {
   extern __shared__ double dyn[];
   __shared__ SortLds srt;
-  constexpr int NB = LB ? kLdsBoysDoubles<La, Lb, Lc, Ld> : 0;
-  static_assert(!LB || NB > 0, "LDS Boys rows are for one- and two-root classes");
+  constexpr int NB = LB ? kLdsRysDoubles<La, Lb, Lc, Ld> : 0;
+  static_assert(!LB || NB > 0, "LDS Rys tables are for one- to four-root classes");
   lds_cdptr lrows = nullptr;
   if constexpr (LB) {
-    constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
-    const double* src = g.rys + (NR == 1 ? LAI_B8_OFF : LAI_B10_OFF);
-    for (int i = threadIdx.x; i < NB; i += blockDim.x) dyn[i] = src[i];
-    __syncthreads();
+    copy_rys_lds<(La + Lb + Lc + Ld) / 2 + 1>(g.rys, dyn);
     lrows = (lds_cdptr)(dyn);
   }
   double* krow = dyn + NB;
@@ -1034,9 +1059,9 @@ __global__ __launch_bounds__(512) void jk_kernel_regs(EriArgs g) {
 // and B, the sort and the chunk's loads are shared by the whole workgroup;
 // the slices' partial ket Coulomb blocks Jt[cd] meet in LDS and go to HBM
 // with one atomic per element instead of one per slice.
-template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C, bool LB = false>
 __device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
-                                                double* jpart) {
+                                                double* jpart, lds_cdptr lrows = nullptr) {
   constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   constexpr int NCH = (na + CHA - 1) / CHA;
   constexpr int A0 = C * CHA, A1 = (A0 + CHA < na) ? A0 + CHA : na;
@@ -1088,7 +1113,7 @@ __device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk&
         if constexpr (MIXED)
           quartet_block_regs<La, Lb, Lc, Ld, float, A0, A1>(g, bra, ket, false, gv);
         else
-          quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1>(g, bra, ket, false, gv);
+          quartet_block_regs<La, Lb, Lc, Ld, double, A0, A1, LB>(g, bra, ket, false, gv, lrows);
         fc = ket.fa;
         fd = ket.fb;
         const bool kl = ket.same;
@@ -1205,32 +1230,39 @@ __device__ __forceinline__ void jk_chunk_merged(const EriArgs& g, const JkChunk&
   }
 }
 
-template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C>
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, int C, bool LB = false>
 __device__ __forceinline__ void dispatch_merged(const EriArgs& g, const JkChunk& ch, double* krow, SortLds& srt,
-                                                double* jpart, int w) {
+                                                double* jpart, int w, lds_cdptr lrows = nullptr) {
   constexpr int na = ncart(La);
   if (w == C) {
-    jk_chunk_merged<La, Lb, Lc, Ld, MIXED, CHA, C>(g, ch, krow, srt, jpart);
+    jk_chunk_merged<La, Lb, Lc, Ld, MIXED, CHA, C, LB>(g, ch, krow, srt, jpart, lrows);
   } else if constexpr ((C + 1) * CHA < na) {
-    dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, C + 1>(g, ch, krow, srt, jpart, w);
+    dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, C + 1, LB>(g, ch, krow, srt, jpart, w, lrows);
   }
 }
 
 // Kernel A, merged slices: rep groups of NCH wavefronts per chunk, wavefront
-// w = slice w mod NCH of group w / NCH.  Dynamic LDS: the groups' ket Coulomb
-// parts [rep][NCH][nc*nd][64], then the exchange rows [na + nb][mn]; static:
-// the sort.
-template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA>
+// w = slice w mod NCH of group w / NCH.  Dynamic LDS: with LB the class's Rys
+// tables, then the groups' ket Coulomb parts [rep][NCH][nc*nd][64], then the
+// exchange rows [na + nb][mn]; static: the sort.
+template <int La, int Lb, int Lc, int Ld, bool MIXED, int CHA, bool LB = false>
 __global__ __launch_bounds__(512) void jk_kernel_merged(EriArgs g) {
   constexpr int NCH = (ncart(La) + CHA - 1) / CHA;
+  constexpr int NB = LB ? kLdsRysDoubles<La, Lb, Lc, Ld> : 0;
+  static_assert(!LB || NB > 0, "LDS Rys tables are for one- to four-root classes");
   extern __shared__ double dyn[];
   __shared__ SortLds srt;
-  double* jpart = dyn;
-  double* krow = dyn + (blockDim.x / 64) * (ncart(Lc) * ncart(Ld) * 64);
+  lds_cdptr lrows = nullptr;
+  if constexpr (LB) {
+    copy_rys_lds<(La + Lb + Lc + Ld) / 2 + 1>(g.rys, dyn);
+    lrows = (lds_cdptr)(dyn);
+  }
+  double* jpart = dyn + NB;
+  double* krow = jpart + (blockDim.x / 64) * (ncart(Lc) * ncart(Ld) * 64);
   const int w = (threadIdx.x >> 6) % NCH;  // uniform over the wavefront
   for (int c = blockIdx.x; c < g.nchunks; c += gridDim.x) {
     const JkChunk ch = g.chunks[c];
-    dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, 0>(g, ch, krow, srt, jpart, w);
+    dispatch_merged<La, Lb, Lc, Ld, MIXED, CHA, 0, LB>(g, ch, krow, srt, jpart, w, lrows);
   }
 }
 
