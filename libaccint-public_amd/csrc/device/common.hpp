@@ -3,6 +3,7 @@
 // primitive pairs, Rys root/weight evaluation and atomics.
 #pragma once
 
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -162,6 +163,11 @@ __device__ __forceinline__ double rcp_nr(double x) {
 using lds_cdptr = const __attribute__((address_space(3))) double*;
 template <int N>
 inline constexpr int kBoysRowW = N == 1 ? 8 : 10;
+// Row stride of the rows as read through P: packed in the device table; one
+// double of padding per row in a workgroup's LDS copy (odd strides 9 and 11),
+// so that the rows that neighbouring lanes read start in different banks
+template <typename P, int W>
+inline constexpr int kRowStride = std::is_same_v<P, lds_cdptr> ? W + 1 : W;
 template <int N>
 inline constexpr int kBoysRowsN = lai_rys_tcut[N] * LAI_BT_PER_UNIT + 1;
 
@@ -192,7 +198,7 @@ __device__ __forceinline__ void boys01_at(double T, P rows, double& F0, double& 
     for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
     double rr[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) rr[m] = rows[k * 8 + m];
+    for (int m = 0; m < 8; ++m) rr[m] = rows[k * kRowStride<P, 8> + m];
     F0 = boys_step(rr, 0, e);
     F1 = boys_step(rr, 1, e);
   } else {
@@ -219,7 +225,7 @@ __device__ __forceinline__ void rys_roots12(double T, P rows, double* u, double*
     for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
     double rr[2 * N + 6];  // F_0 .. F_{2N+5}: orders up to 2N-1 plus six Taylor terms
 #pragma unroll
-    for (int m = 0; m < 2 * N + 6; ++m) rr[m] = rows[k * kBoysRowW<N> + m];
+    for (int m = 0; m < 2 * N + 6; ++m) rr[m] = rows[k * kRowStride<P, kBoysRowW<N>> + m];
     const double F0 = boys_step(rr, 0, e), F1 = boys_step(rr, 1, e);
     if constexpr (N == 1) {
       u[0] = F1 * rcp_nr(F0);

@@ -996,12 +996,13 @@ __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChun
 }
 
 // The Rys tables a class may copy to LDS (doubles): for one and two roots
-// the Boys Taylor rows 0 .. 16 T_cut of F_0..F_7 or F_0..F_9; for three and
-// four roots the Chebyshev coefficients of the rule below its cut (34 and 51
-// KB, interval stride padded to kChebStrideLds); none for more roots.
+// the Boys Taylor rows 0 .. 16 T_cut of F_0..F_7 or F_0..F_9 (row stride
+// padded to kRowStride); for three and four roots the Chebyshev coefficients
+// of the rule below its cut (34 and 51 KB, interval stride padded to
+// kChebStrideLds); none for more roots.
 constexpr int lds_rys_doubles(int NR) {
-  return NR == 1   ? kBoysRowsN<1> * kBoysRowW<1>
-         : NR == 2 ? kBoysRowsN<2> * kBoysRowW<2>
+  return NR == 1   ? kBoysRowsN<1> * kRowStride<lds_cdptr, kBoysRowW<1>>
+         : NR == 2 ? kBoysRowsN<2> * kRowStride<lds_cdptr, kBoysRowW<2>>
          : NR == 3 ? lai_rys_tcut[3] * kChebStrideLds<3>
          : NR == 4 ? lai_rys_tcut[4] * kChebStrideLds<4>
                    : 0;
@@ -1015,8 +1016,12 @@ template <int NR>
 __device__ __forceinline__ void copy_rys_lds(const double* __restrict__ rys, double* dst) {
   constexpr int NB = lds_rys_doubles(NR);
   if constexpr (NR <= 2) {
+    constexpr int W = kBoysRowW<NR>, ST = kRowStride<lds_cdptr, W>;
     const double* src = rys + (NR == 1 ? LAI_B8_OFF : LAI_B10_OFF);
-    for (int i = threadIdx.x; i < NB; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) {
+      const int k = i / ST, r = i - k * ST;
+      dst[i] = r < W ? src[k * W + r] : 0.0;
+    }
   } else {
     constexpr int W = 2 * NR * LAI_RYS_NTERMS, ST = kChebStrideLds<NR>;
     const double* src = rys + lai_rys_off[NR];
