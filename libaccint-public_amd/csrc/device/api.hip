@@ -384,7 +384,12 @@ extern "C" int lai_dev_init(int device, lai_ctx** out) {
   }
   std::memcpy(tab.data() + LAI_BT_OFF, lai_boys_taylor_h, sizeof(double) * LAI_BT_ROWS * LAI_BT_M);
   for (int k = 0; k < LAI_BT_ROWS; ++k) {
-    std::memcpy(tab.data() + LAI_B8_OFF + size_t(k) * 8, lai_boys_taylor_h + size_t(k) * LAI_BT_M, sizeof(double) * 8);
+    // one-root rows: F_i / i!, the Taylor coefficients of F_0 (boys01_taylor)
+    double fact = 1.0;
+    for (int i = 0; i < 8; ++i) {
+      if (i > 1) fact *= double(i);
+      tab[LAI_B8_OFF + size_t(k) * 8 + i] = lai_boys_taylor_h[size_t(k) * LAI_BT_M + i] / fact;
+    }
     std::memcpy(tab.data() + LAI_B10_OFF + size_t(k) * 10, lai_boys_taylor_h + size_t(k) * LAI_BT_M, sizeof(double) * 10);
   }
   hipError_t e = ctx->rys.assign(tab.data(), tab.size());

@@ -130,9 +130,9 @@ struct PersistItem {
 // LAI_BT_OFF (128-byte aligned) the Boys Taylor rows (lai_boys_taylor_h).
 // ---------------------------------------------------------------------------
 #define LAI_BT_OFF (((LAI_RYS_TABLE_SIZE + 15) / 16) * 16)
-// From LAI_B8_OFF: F_0..F_7 of every Boys Taylor row again, 8 doubles (64 B) a
-// row, for the one-root evaluations (boys01, rys_roots<1>), which need no
-// higher order: half the cache footprint of the 128-byte rows.
+// From LAI_B8_OFF: the one-root rows, 8 doubles (64 B) a row, half the cache
+// footprint of the 128-byte rows: H_i = F_i(T_k) / i! for i = 0..7, the Taylor
+// coefficients of F_0 about the grid point (boys01_taylor).
 #define LAI_B8_OFF (LAI_BT_OFF + LAI_BT_ROWS * LAI_BT_M)
 // From LAI_B10_OFF: F_0..F_9 of every row, 10 doubles (80 B) a row, for
 // rys_roots<2>.
@@ -186,21 +186,35 @@ __device__ __forceinline__ void rys_hermite(double T, double* u, double* w) {
   }
 }
 
-// F_0(T) and F_1(T): the Boys Taylor rows below the one-root cut, the
+// F_0 and F_1 below the one-root cut from the one-root rows (LAI_B8_OFF),
+// which hold H_i = F_i(T_k) / i! of the grid point T_k = k/16 nearest T.
+// Since dF_m/dT = -F_{m+1}, F_0(T) = P(x) = sum_i H_i x^i with x = T_k - T,
+// and F_1 = -dF_0/dT = P'(x): one Horner pass gives both (13 FMAs).  The
+// error of F_1 is the term F_8 x^7 / 7!, as for the seven-term series of F_1
+// itself; F_0 carries one more term.
+template <typename P>
+__device__ __forceinline__ void boys01_taylor(double T, P rows, double& F0, double& F1) {
+  const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
+  const double x = fma(double(k), 1.0 / LAI_BT_PER_UNIT, -T);
+  double h[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) h[m] = rows[k * kRowStride<P, 8> + m];
+  double p = fma(h[7], x, h[6]), q = h[7];
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    q = fma(q, x, p);
+    p = fma(p, x, h[i]);
+  }
+  F0 = p;
+  F1 = q;
+}
+
+// F_0(T) and F_1(T): the one-root rows below the one-root cut, the
 // Gauss-Hermite limit of the one-root rule above it (F_0 = w, F_1 = u w)
 template <typename P>
 __device__ __forceinline__ void boys01_at(double T, P rows, double& F0, double& F1) {
   if (T < double(lai_rys_tcut[1])) {
-    const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
-    const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
-    double e[7];
-#pragma unroll
-    for (int j = 1; j <= 6; ++j) e[j] = -d * (1.0 / j);
-    double rr[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) rr[m] = rows[k * kRowStride<P, 8> + m];
-    F0 = boys_step(rr, 0, e);
-    F1 = boys_step(rr, 1, e);
+    boys01_taylor(T, rows, F0, F1);
   } else {
     double u, w;
     rys_hermite<1>(T, &u, &w);
@@ -217,7 +231,16 @@ __device__ __forceinline__ void boys01(double T, const double* __restrict__ tab,
 template <int N, typename P>
 __device__ __forceinline__ void rys_roots12(double T, P rows, double* u, double* w) {
   static_assert(N == 1 || N == 2, "one or two roots");
-  if (T < double(lai_rys_tcut[N])) {
+  if constexpr (N == 1) {
+    if (T < double(lai_rys_tcut[1])) {
+      double F0, F1;
+      boys01_taylor(T, rows, F0, F1);
+      u[0] = F1 * rcp_nr(F0);
+      w[0] = F0;
+    } else {
+      rys_hermite<1>(T, u, w);
+    }
+  } else if (T < double(lai_rys_tcut[N])) {
     const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
     const double d = T - double(k) * (1.0 / LAI_BT_PER_UNIT);
     double e[7];
@@ -227,10 +250,7 @@ __device__ __forceinline__ void rys_roots12(double T, P rows, double* u, double*
 #pragma unroll
     for (int m = 0; m < 2 * N + 6; ++m) rr[m] = rows[k * kRowStride<P, kBoysRowW<N>> + m];
     const double F0 = boys_step(rr, 0, e), F1 = boys_step(rr, 1, e);
-    if constexpr (N == 1) {
-      u[0] = F1 * rcp_nr(F0);
-      w[0] = F0;
-    } else {
+    {
       const double F2 = boys_step(rr, 2, e), F3 = boys_step(rr, 3, e);
       // x^2 + p x + q orthogonal to 1 and x under the moments
       const double idet = rcp_nr(fma(F0, F2, -F1 * F1));
