@@ -1294,14 +1294,14 @@ constexpr GatherSplit gather_split(int O, int NU, int NV, int NL) {
 
 // One contraction of a quartet by NL lanes (a wavefront or a workgroup),
 // gathered by its outputs:
-//   out[OO + p*ND + d] += f * sum_{u<NU, v<NV} X[t*ND + d] * Dk[OD + u*DU + v*DV],
-//   t = p*SP + u*SU + v*SV  (the (a, b, c) task of the term),
-// for the NP*ND outputs (p, d).  X holds the quartet's normalised integrals
-// by task and ket function d; Dk the staged density blocks.  An output's u
-// range is split over S lanes (S-way LDS adds); S = 1 writes are exclusive.
-template <int NL, int NP, int ND, int NU, int NV, int SP, int SU, int SV, int OD, int DU, int DV, int OO>
+//   out[OO + p1*N2 + p2] += f * sum_{u<NU, v<NV} X[p1*X1 + p2*X2 + u*XU + v*XV]
+//                                              * Dk[OD + u*DU + v*DV]
+// for the N1*N2 outputs (p1, p2).  X holds the quartet's normalised integrals
+// X[((a*nb + b)*nc + c)*nd + d]; Dk the staged density blocks.  An output's
+// u range is split over S lanes (S-way LDS adds); S = 1 writes are exclusive.
+template <int NL, int N1, int N2, int NU, int NV, int X1, int X2, int XU, int XV, int OD, int DU, int DV, int OO>
 __device__ __forceinline__ void gather_contract(const double* X, const double* Dk, double* out, double f, int lane) {
-  constexpr int O = NP * ND;
+  constexpr int O = N1 * N2;
   constexpr GatherSplit sp = gather_split(O, NU, NV, NL);
   constexpr int S = sp.s, R = sp.r, F = NU / S, REM = NU % S;
 #pragma unroll
@@ -1309,18 +1309,18 @@ __device__ __forceinline__ void gather_contract(const double* X, const double* D
     const int slot = lane + NL * r;
     const int o = slot % O, part = slot / O;
     if (part < S) {
-      const int p = o / ND, d = o - p * ND;
-      const double* x = X + (p * SP + part * SU) * ND + d;
+      const int p1 = o / N2, p2 = o - p1 * N2;
+      const double* x = X + p1 * X1 + p2 * X2 + part * XU;
       const double* dk = Dk + OD + part * DU;
       double acc = 0.0;
 #pragma unroll
       for (int i = 0; i < F; ++i)
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc += x[(S * i * SU + v * SV) * ND] * dk[S * i * DU + v * DV];
+        for (int v = 0; v < NV; ++v) acc += x[S * i * XU + v * XV] * dk[S * i * DU + v * DV];
       if constexpr (REM > 0) {
         if (part < REM) {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) acc += x[(S * F * SU + v * SV) * ND] * dk[S * F * DU + v * DV];
+          for (int v = 0; v < NV; ++v) acc += x[S * F * XU + v * XV] * dk[S * F * DU + v * DV];
         }
       }
       if constexpr (S == 1)
@@ -1331,11 +1331,32 @@ __device__ __forceinline__ void gather_contract(const double* X, const double* D
   }
 }
 
+// The six contractions of a quartet's integrals X with its density blocks
+// (at the offsets Oxy of the J/K blocks of the same shape), each gathered by
+// output over NL lanes: J[ab] = sum_cd X D[cd], J[cd] = sum_ab X D[ab],
+// K[ac] = sum_bd X D[bd], K[ad] = sum_bc X D[bc], K[bc] = sum_ad X D[ad],
+// K[bd] = sum_ac X D[ac]; the same-pair and bra-equals-ket cases drop the
+// duplicates.
+template <int NL, int La, int Lb, int Lc, int Ld>
+__device__ __forceinline__ void gather_all(const double* X, const double* Dk, double* out, double fJ, double fK,
+                                           bool ij, bool kl, bool bk, int lane) {
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
+                OBD = OBC + nb * nc;
+  constexpr int SA = nb * nc * nd, SB = nc * nd, SC = nd;  // strides of a, b, c in X
+  gather_contract<NL, na, nb, nc, nd, SA, SB, SC, 1, OCD, nd, 1, OAB>(X, Dk, out, fJ, lane);
+  gather_contract<NL, na, nc, nb, nd, SA, SC, SB, 1, OBD, nd, 1, OAC>(X, Dk, out, fK, lane);
+  if (!ij) gather_contract<NL, nb, nc, na, nd, SB, SC, SA, 1, OAD, nd, 1, OBC>(X, Dk, out, fK, lane);
+  if (!bk) gather_contract<NL, nc, nd, na, nb, SC, 1, SA, SB, OAB, nb, 1, OCD>(X, Dk, out, fJ, lane);
+  if (!kl) gather_contract<NL, na, nd, nb, nc, SA, 1, SB, SC, OBC, nc, 1, OAD>(X, Dk, out, fK, lane);
+  if (!ij && !kl) gather_contract<NL, nb, nd, na, nc, SB, 1, SA, SC, OAC, nc, 1, OBD>(X, Dk, out, fK, lane);
+}
+
 // Kernel B: one workgroup per chunk, the chunk's quartets one after another.
 // Each quartet's ket Coulomb and exchange blocks go to HBM with one atomic per
 // element; the bra Coulomb block Jt[ab] is summed in LDS over the chunk.
-// Where the quartet's integrals fit in the table storage, the contractions
-// with an output per ket function d are gathered by output (gather_contract).
+// Where the quartet's integrals fit in the table storage, all six
+// contractions are gathered by output (gather_all).
 // Wavefronts per SIMD that bound the registers of the workgroup kernel: three
 // for at most 4 accumulators per lane, two up to the largest classes whose
 // integrals fit in LDS, one for the rest.
@@ -1353,17 +1374,14 @@ __global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk
   constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
   constexpr int NJ = (NTASK + NT - 1) / NT;
   constexpr int NPQ = Sh::template npq<NT>();
-  // FP64 tables (the FP32 recursion uses the same storage as float); where
-  // they are large enough, then the quartet's normalised integrals, so that
-  // the digestion takes one task at a time (fewer registers)
-  // the tables, then X[t*nd + d] by task t = (a*nb + b)*nc + c for the
-  // gathered contractions: the storage grows to hold X up to 48 KiB (these
-  // classes' register demand leaves at most two workgroups per CU)
+  // FP64 tables (the FP32 recursion uses the same storage as float), then
+  // the quartet's normalised integrals X[t*nd + d] by task t = (a*nb + b)*nc
+  // + c for the gathered contractions: the storage grows to hold X up to 48
+  // KiB (these classes' register demand leaves at most two workgroups per CU)
   constexpr int XN = NTASK * nd;
   constexpr int TABN = NPQ * Sh::TSP > XN || XN > 6144 ? NPQ * Sh::TSP : XN;
   __shared__ double tab[TABN];
   constexpr bool XLDS = XN <= TABN;
-  constexpr int XUNROLL = XLDS ? 1 : NJ;
   // the quartet's six J/K blocks, summed in LDS and added to HBM once per
   // quartet (Jt[cd], Kt[ac], Kt[ad], Kt[bc], Kt[bd]) or once per chunk (Jt[ab])
   constexpr int OAB = 0, OCD = na * nb, OAC = OCD + nc * nd, OAD = OAC + na * nc, OBC = OAD + na * nd,
@@ -1443,33 +1461,29 @@ __global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk
             for (int d = 0; d < nd; ++d) tab[(tid + NT * j) * nd + d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
       }
       __syncthreads();
-#pragma unroll XUNROLL
-      for (int j = 0; j < NJ; ++j) {
-        const int t = tid + NT * j;
-        if (t >= NTASK) continue;
-        const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-        double x[nd];
+      if constexpr (XLDS) {
+        gather_all<NT, La, Lb, Lc, Ld>(tab, dblk, blk, fJ, fK, ij, kl, bk, tid);
+      } else {
+        // (ff|ff): the integrals stay in registers, and each task's terms
+        // are scattered with LDS adds (an output shared by up to nb*nc lanes)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) {
-          if constexpr (XLDS)
-            x[d] = tab[t * nd + d];
-          else
-            x[d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
-        }
-        // per-task sums over d: J[ab] (D[cd]), K[ac] (D[bd]), K[bc] (D[ad])
-        double sj = 0.0, s1 = 0.0, s3 = 0.0;
+        for (int j = 0; j < NJ; ++j) {
+          const int t = tid + NT * j;
+          if (t >= NTASK) continue;
+          const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
+          double x[nd];
 #pragma unroll
-        for (int d = 0; d < nd; ++d) {
-          sj += x[d] * dblk[OCD + c * nd + d];
-          s1 += x[d] * dblk[OBD + b * nd + d];
-          s3 += x[d] * dblk[OAD + a * nd + d];
-        }
-        lds_add(&blk[OAB + a * nb + b], fJ * sj);
-        lds_add(&blk[OAC + a * nc + c], fK * s1);
-        if (!ij) lds_add(&blk[OBC + b * nc + c], fK * s3);
-        if constexpr (!XLDS) {
-          // the per-d contractions scattered: an output shared by up to
-          // nb*nc lanes of an LDS add
+          for (int d = 0; d < nd; ++d) x[d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
+          double sj = 0.0, s1 = 0.0, s3 = 0.0;
+#pragma unroll
+          for (int d = 0; d < nd; ++d) {
+            sj += x[d] * dblk[OCD + c * nd + d];
+            s1 += x[d] * dblk[OBD + b * nd + d];
+            s3 += x[d] * dblk[OAD + a * nd + d];
+          }
+          lds_add(&blk[OAB + a * nb + b], fJ * sj);
+          lds_add(&blk[OAC + a * nc + c], fK * s1);
+          if (!ij) lds_add(&blk[OBC + b * nc + c], fK * s3);
           const double dab = dblk[OAB + a * nb + b], dbc = dblk[OBC + b * nc + c], dac = dblk[OAC + a * nc + c];
 #pragma unroll
           for (int d = 0; d < nd; ++d) {
@@ -1478,14 +1492,6 @@ __global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk
             if (!ij && !kl) lds_add(&blk[OBD + b * nd + d], fK * x[d] * dac);
           }
         }
-      }
-      if constexpr (XLDS) {
-        // the per-d contractions gathered by output: J[cd] = sum_ab X D[ab],
-        // K[ad] = sum_bc X D[bc], K[bd] = sum_ac X D[ac]
-        if (!bk) gather_contract<NT, nc, nd, na, nb, 1, nb * nc, nc, OAB, nb, 1, OCD>(tab, dblk, blk, fJ, tid);
-        if (!kl) gather_contract<NT, na, nd, nb, nc, nb * nc, nc, 1, OBC, nc, 1, OAD>(tab, dblk, blk, fK, tid);
-        if (!ij && !kl)
-          gather_contract<NT, nb, nd, na, nc, nc, nb * nc, 1, OAC, nc, 1, OBD>(tab, dblk, blk, fK, tid);
       }
       __syncthreads();
       // the quartet's ket and exchange blocks to HBM, one atomic per element
@@ -1755,31 +1761,7 @@ __global__ __launch_bounds__(64 * NW, (WaveLds<La, Lb, Lc, Ld, NW>::WPE)) void j
 #pragma unroll
           for (int d = 0; d < nd; ++d) xw[(lane + 64 * j) * nd + d] = v[j][d] * sabc[j] * norm_corr(Ld, d);
       wave_sync();
-      // contractions summed over d within a task: J[ab] (D[cd]), K[ac]
-      // (D[bd]), K[bc] (D[ad]); an output is shared by at most nc, nb or na
-      // lanes of an LDS add
-#pragma unroll 1
-      for (int j = 0; j < NJ; ++j) {
-        const int t = lane + 64 * j;
-        if (t >= NTASK) continue;
-        const int c = t % nc, b = (t / nc) % nb, a = t / (nc * nb);
-        double sj = 0.0, s1 = 0.0, s3 = 0.0;
-#pragma unroll
-        for (int d = 0; d < nd; ++d) {
-          const double x = xw[t * nd + d];
-          sj += x * dw[OCD + c * nd + d];
-          s1 += x * dw[OBD + b * nd + d];
-          s3 += x * dw[OAD + a * nd + d];
-        }
-        lds_add(&bw[OAB + a * nb + b], fJ * sj);
-        lds_add(&bw[OAC + a * nc + c], fK * s1);
-        if (!ij) lds_add(&bw[OBC + b * nc + c], fK * s3);
-      }
-      // contractions with an output per ket function d, gathered by output:
-      // J[cd] = sum_ab X D[ab], K[ad] = sum_bc X D[bc], K[bd] = sum_ac X D[ac]
-      if (!bk) gather_contract<64, nc, nd, na, nb, 1, nb * nc, nc, OAB, nb, 1, OCD>(xw, dw, bw, fJ, lane);
-      if (!kl) gather_contract<64, na, nd, nb, nc, nb * nc, nc, 1, OBC, nc, 1, OAD>(xw, dw, bw, fK, lane);
-      if (!ij && !kl) gather_contract<64, nb, nd, na, nc, nc, nb * nc, 1, OAC, nc, 1, OBD>(xw, dw, bw, fK, lane);
+      gather_all<64, La, Lb, Lc, Ld>(xw, dw, bw, fJ, fK, ij, kl, bk, lane);
       wave_sync();  // the wave's quartet block is complete
       // this quartet's ket and exchange blocks to HBM, one atomic per element;
       // Jt[ab] stays in LDS for the rest of the chunk
