@@ -1400,6 +1400,23 @@ __global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk
     sabc[j] = norm_corr(La, a) * norm_corr(Lb, b) * norm_corr(Lc, c);
   }
   for (int e = tid; e < NBLK; e += NT) blk[e] = 0.0;  // ordered by the barriers of lds_quartet_acc
+  // the density element this lane stages in each of its NDS slots o = tid +
+  // NT*i: row fr[rs] + ro, column fr[cs] + co of the shells' first functions
+  // fr = (fa, fb, fc, fd), packed rs | cs << 2 | ro << 4 | co << 8
+  constexpr int NDS = (NBLK + NT - 1) / NT;
+  int dsl[NDS];
+#pragma unroll
+  for (int i = 0; i < NDS; ++i) {
+    const int o = tid + NT * i;
+    int rs, cs, ro, co;
+    if (o < OCD) { rs = 0; cs = 1; ro = (o - OAB) / nb; co = (o - OAB) % nb; }
+    else if (o < OAC) { rs = 2; cs = 3; ro = (o - OCD) / nd; co = (o - OCD) % nd; }
+    else if (o < OAD) { rs = 0; cs = 2; ro = (o - OAC) / nc; co = (o - OAC) % nc; }
+    else if (o < OBC) { rs = 0; cs = 3; ro = (o - OAD) / nd; co = (o - OAD) % nd; }
+    else if (o < OBD) { rs = 1; cs = 2; ro = (o - OBC) / nc; co = (o - OBC) % nc; }
+    else { rs = 1; cs = 3; ro = (o - OBD) / nd; co = (o - OBD) % nd; }
+    dsl[i] = o < NBLK ? (rs | cs << 2 | ro << 4 | co << 8) : -1;
+  }
 
   for (int ci = blockIdx.x; ci < g.nchunks; ci += gridDim.x) {
     const JkChunk ch = g.chunks[ci];
@@ -1417,6 +1434,20 @@ __global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk
       const PairRec ket = load_pair_uniform(g.pairs, kp);
       const int mode = quartet_select(g.sel, ch.bra, bra, kp, ket);
       if (mode == 0) continue;  // uniform over the workgroup
+      const int fc = ket.fa, fd = ket.fb;
+      // the quartet's six density blocks, loaded now and staged after the
+      // products so that the loads' latency is hidden behind them
+      double dpre[NDS];
+#pragma unroll
+      for (int i = 0; i < NDS; ++i) {
+        const int pk = dsl[i];
+        if (pk >= 0) {
+          const int rs = pk & 3, cs = (pk >> 2) & 3;
+          const int r = (rs == 0 ? fa : rs == 1 ? fb : fc) + ((pk >> 4) & 15);
+          const int q = (cs == 1 ? fb : cs == 2 ? fc : fd) + (pk >> 8);
+          dpre[i] = D[size_t(r) * n + q];
+        }
+      }
       double v[NJ][nd];
       if constexpr (MIXED) {
         float acc[NJ][nd];
@@ -1434,23 +1465,15 @@ __global__ __launch_bounds__(NT, (lds_kernel_wpe<La, Lb, Lc, Ld, NT>())) void jk
 #pragma unroll
           for (int d = 0; d < nd; ++d) v[j][d] = acc[j][d];
       }
-      const int fc = ket.fa, fd = ket.fb;
       const bool kl = ket.same, bk = (kp == ch.bra);
       const double fJ = (ij ? 1.0 : 2.0) * (kl ? 1.0 : 2.0);
       const double fK = bk ? 1.0 : 2.0;
       // the quartet's six density blocks D[xy] at the offsets Oxy of the J/K
       // blocks of the same shape (the previous quartet's readers have passed
       // the barriers of lds_quartet_acc)
-      for (int o = tid; o < NBLK; o += NT) {
-        size_t r, q;
-        if (o < OCD) { r = fa + (o - OAB) / nb; q = fb + (o - OAB) % nb; }
-        else if (o < OAC) { r = fc + (o - OCD) / nd; q = fd + (o - OCD) % nd; }
-        else if (o < OAD) { r = fa + (o - OAC) / nc; q = fc + (o - OAC) % nc; }
-        else if (o < OBC) { r = fa + (o - OAD) / nd; q = fd + (o - OAD) % nd; }
-        else if (o < OBD) { r = fb + (o - OBC) / nc; q = fc + (o - OBC) % nc; }
-        else { r = fb + (o - OBD) / nd; q = fd + (o - OBD) % nd; }
-        dblk[o] = D[r * n + q];
-      }
+#pragma unroll
+      for (int i = 0; i < NDS; ++i)
+        if (dsl[i] >= 0) dblk[tid + NT * i] = dpre[i];
       // the normalised integrals of this lane's tasks through the table
       // storage, free after the products
       if constexpr (XLDS) {
