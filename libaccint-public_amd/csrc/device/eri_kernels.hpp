@@ -1821,9 +1821,16 @@ template <int La, int Lb, int Lc, int Ld>
 struct ClassPolicy {
   static constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   static constexpr int rest = nb * nc * nd;
-  static constexpr int CHA = rest >= 27 ? 1 : (27 / rest < na ? 27 / rest : na);
+  // lane-per-quartet classes: at most 27 outputs per lane in registers
+  // decide the shape; a class whose bra that cuts into slices then takes up
+  // to 54 per lane (half the slices, each of which repeats the roots and the
+  // 2D recursion: measured faster despite scratch spills at the 256-register
+  // cap of a 512-lane workgroup)
+  static constexpr int CHA0 = rest >= 27 ? 1 : (27 / rest < na ? 27 / rest : na);
+  static constexpr int NCH0 = (na + CHA0 - 1) / CHA0;
+  static constexpr bool regs = rest <= 27 && NCH0 <= 6;
+  static constexpr int CHA = regs && NCH0 > 1 ? (54 / rest < na ? 54 / rest : na) : CHA0;
   static constexpr int NCH = (na + CHA - 1) / CHA;
-  static constexpr bool regs = rest <= 27 && NCH <= 6;
   static constexpr int NOUT = na * nb * nc * nd;
   static constexpr int NT = 256;  // lanes of the workgroup-per-quartet shape
   // LDS-shape classes of at most 6 (a, b, c) tasks per lane of a wavefront

@@ -11,8 +11,9 @@
 // Covered: the lane-per-quartet classes whose chunk code runs in a 256-lane
 // workgroup -- those with the bra unsplit (256 lanes, one quartet per lane)
 // or split into 2 or 4 slices (groups of 2 or 4 wavefronts, kernel A
-// merged).  One kernel holds all their code, so it runs at the register and
-// LDS footprint of the largest of them.
+// merged) -- at the base register budget of 27 outputs per lane.  One kernel
+// holds all their code, so it runs at the register and LDS footprint of the
+// largest of them.
 #include <algorithm>
 
 #include "eri_kernels.hpp"
@@ -28,7 +29,9 @@ struct PersistClass {
   static constexpr int pb = cls_bra(ID), pk = cls_ket(ID);
   static constexpr int La = pc_La(pb), Lb = pc_Lb(pb), Lc = pc_La(pk), Ld = pc_Lb(pk);
   using P = ClassPolicy<La, Lb, Lc, Ld>;
-  static constexpr bool covered = P::regs && (P::NCH == 1 || P::NCH == 2 || P::NCH == 4);
+  // (classes whose slices take the larger register budget keep their own
+  // launches: their footprint would set the one kernel's)
+  static constexpr bool covered = P::regs && P::CHA == P::CHA0 && (P::NCH == 1 || P::NCH == 2 || P::NCH == 4);
   // dynamic LDS in doubles: the merged slices' ket Coulomb parts, then the
   // exchange rows (na + nb rows of nbf)
   static constexpr int jpart = P::NCH > 1 ? (kPersistNT / 64) * ncart(Lc) * ncart(Ld) * 64 : 0;
