@@ -1822,10 +1822,10 @@ struct ClassPolicy {
   static constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
   static constexpr int rest = nb * nc * nd;
   // lane-per-quartet classes: at most 27 outputs per lane in registers
-  // decide the shape; a class whose bra that cuts into slices then takes up
-  // to 54 per lane (half the slices, each of which repeats the roots and the
-  // 2D recursion: measured faster despite scratch spills at the 256-register
-  // cap of a 512-lane workgroup)
+  // decide the shape; a class whose bra is cut into slices then takes up to
+  // 54 per lane (half the slices, each of which repeats the roots and the 2D
+  // recursion: measured faster despite scratch spills at the 256-register cap
+  // of a 512-lane workgroup)
   static constexpr int CHA0 = rest >= 27 ? 1 : (27 / rest < na ? 27 / rest : na);
   static constexpr int NCH0 = (na + CHA0 - 1) / CHA0;
   static constexpr bool regs = rest <= 27 && NCH0 <= 6;
