@@ -392,6 +392,17 @@ extern "C" int lai_dev_init(int device, lai_ctx** out) {
     }
     std::memcpy(tab.data() + LAI_B10_OFF + size_t(k) * 10, lai_boys_taylor_h + size_t(k) * LAI_BT_M, sizeof(double) * 10);
   }
+  // coarser one-root rows for the LDS copies: T_k = k/8 is row 2k of the
+  // 1/16 grid; H_i = F_i / i! for i = 0..8
+  static_assert(2 * (LAI_B9_ROWS - 1) < LAI_BT_ROWS && LAI_BT_M >= 9, "one-root rows out of the Taylor table");
+  static_assert(LAI_B9_ROWS == 8 * lai_rys_tcut[1] + 1, "one-root rows must cover the one-root cut");
+  for (int k = 0; k < LAI_B9_ROWS; ++k) {
+    double fact = 1.0;
+    for (int i = 0; i < 9; ++i) {
+      if (i > 1) fact *= double(i);
+      tab[LAI_B9_OFF + size_t(k) * 9 + i] = lai_boys_taylor_h[size_t(2 * k) * LAI_BT_M + i] / fact;
+    }
+  }
   hipError_t e = ctx->rys.assign(tab.data(), tab.size());
   if (e != hipSuccess) {
     delete ctx;

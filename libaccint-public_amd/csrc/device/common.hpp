@@ -137,7 +137,13 @@ struct PersistItem {
 // From LAI_B10_OFF: F_0..F_9 of every row, 10 doubles (80 B) a row, for
 // rys_roots<2>.
 #define LAI_B10_OFF (LAI_B8_OFF + LAI_BT_ROWS * 8)
-#define LAI_DEVICE_TABLE_SIZE (LAI_B10_OFF + LAI_BT_ROWS * 10)
+// From LAI_B9_OFF: the one-root rows on the coarser grid T_k = k/8 (k = 0 ..
+// 8 T_cut(1)), 9 doubles a row: H_i = F_i(T_k) / i! for i = 0..8, the source
+// of a workgroup's LDS copy (22.5 KB instead of 40 KB at step 1/16, so that
+// the copy fits beside as many resident waves)
+#define LAI_B9_OFF (LAI_B10_OFF + LAI_BT_ROWS * 10)
+#define LAI_B9_ROWS (8 * 39 + 1)
+#define LAI_DEVICE_TABLE_SIZE (LAI_B9_OFF + LAI_B9_ROWS * 9)
 
 // Taylor step from the grid row r (F_m(T_k)) to F_m(T_k + d):
 // F_m + e1 (F_{m+1} + e2 (F_{m+2} + ... + e6 F_{m+6})) with e_j = -d / j.
@@ -192,16 +198,21 @@ __device__ __forceinline__ void rys_hermite(double T, double* u, double* w) {
 // and F_1 = -dF_0/dT = P'(x): one Horner pass gives both (13 FMAs).  The
 // error of F_1 is the term F_8 x^7 / 7!, as for the seven-term series of F_1
 // itself; F_0 carries one more term.
+// A workgroup's LDS copy holds the coarser rows of LAI_B9_OFF (step 1/8,
+// H_0..H_8, |x| <= 1/16): one more term keeps the same truncation order.
 template <typename P>
 __device__ __forceinline__ void boys01_taylor(double T, P rows, double& F0, double& F1) {
-  const int k = int(fma(T, double(LAI_BT_PER_UNIT), 0.5));
-  const double x = fma(double(k), 1.0 / LAI_BT_PER_UNIT, -T);
-  double h[8];
+  constexpr bool lds = std::is_same_v<P, lds_cdptr>;
+  constexpr int NH = lds ? 9 : 8;                   // coefficients H_0 .. H_{NH-1}
+  constexpr double G = lds ? 8.0 : double(LAI_BT_PER_UNIT);  // grid points per unit of T
+  const int k = int(fma(T, G, 0.5));
+  const double x = fma(double(k), 1.0 / G, -T);
+  double h[NH];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) h[m] = rows[k * kRowStride<P, 8> + m];
-  double p = fma(h[7], x, h[6]), q = h[7];
+  for (int m = 0; m < NH; ++m) h[m] = rows[k * (lds ? 9 : 8) + m];
+  double p = fma(h[NH - 1], x, h[NH - 2]), q = h[NH - 1];
 #pragma unroll
-  for (int i = 5; i >= 0; --i) {
+  for (int i = NH - 3; i >= 0; --i) {
     q = fma(q, x, p);
     p = fma(p, x, h[i]);
   }

@@ -995,13 +995,14 @@ __device__ __forceinline__ void dispatch_jk_chunk(const EriArgs& g, const JkChun
   }
 }
 
-// The Rys tables a class may copy to LDS (doubles): for one and two roots
-// the Boys Taylor rows 0 .. 16 T_cut of F_0..F_7 or F_0..F_9 (row stride
-// padded to kRowStride); for three and four roots the Chebyshev coefficients
+// The Rys tables a class may copy to LDS (doubles): for one root the coarser
+// rows of LAI_B9_OFF (step 1/8, nine Taylor coefficients of F_0); for two
+// roots the Boys Taylor rows 0 .. 16 T_cut of F_0..F_9 (row stride padded to
+// kRowStride); for three and four roots the Chebyshev coefficients
 // of the rule below its cut (34 and 51 KB, interval stride padded to
 // kChebStrideLds); none for more roots.
 constexpr int lds_rys_doubles(int NR) {
-  return NR == 1   ? kBoysRowsN<1> * kRowStride<lds_cdptr, kBoysRowW<1>>
+  return NR == 1   ? LAI_B9_ROWS * 9
          : NR == 2 ? kBoysRowsN<2> * kRowStride<lds_cdptr, kBoysRowW<2>>
          : NR == 3 ? lai_rys_tcut[3] * kChebStrideLds<3>
          : NR == 4 ? lai_rys_tcut[4] * kChebStrideLds<4>
@@ -1015,9 +1016,13 @@ inline constexpr int kLdsRysDoubles = lds_rys_doubles((La + Lb + Lc + Ld) / 2 + 
 template <int NR>
 __device__ __forceinline__ void copy_rys_lds(const double* __restrict__ rys, double* dst) {
   constexpr int NB = lds_rys_doubles(NR);
-  if constexpr (NR <= 2) {
+  if constexpr (NR == 1) {
+    // the coarser one-root rows, packed (odd stride 9)
+    const double* src = rys + LAI_B9_OFF;
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) dst[i] = src[i];
+  } else if constexpr (NR == 2) {
     constexpr int W = kBoysRowW<NR>, ST = kRowStride<lds_cdptr, W>;
-    const double* src = rys + (NR == 1 ? LAI_B8_OFF : LAI_B10_OFF);
+    const double* src = rys + LAI_B10_OFF;
     for (int i = threadIdx.x; i < NB; i += blockDim.x) {
       const int k = i / ST, r = i - k * ST;
       dst[i] = r < W ? src[k * W + r] : 0.0;
