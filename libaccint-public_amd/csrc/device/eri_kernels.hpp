@@ -1834,8 +1834,11 @@ struct ClassPolicy {
   static constexpr int CHA0 = rest >= 27 ? 1 : (27 / rest < na ? 27 / rest : na);
   static constexpr int NCH0 = (na + CHA0 - 1) / CHA0;
   static constexpr bool regs = rest <= 27 && NCH0 <= 6;
-  static constexpr int CHA = regs && NCH0 > 1 ? (54 / rest < na ? 54 / rest : na) : CHA0;
-  static constexpr int NCH = (na + CHA - 1) / CHA;
+  static constexpr int CHA1 = regs && NCH0 > 1 ? (54 / rest < na ? 54 / rest : na) : CHA0;
+  static constexpr int NCH = (na + CHA1 - 1) / CHA1;
+  // the bra components spread evenly over the slices (a group of slice
+  // wavefronts finishes with its largest slice)
+  static constexpr int CHA = (na + NCH - 1) / NCH;
   static constexpr int NOUT = na * nb * nc * nd;
   static constexpr int NT = 256;  // lanes of the workgroup-per-quartet shape
   // LDS-shape classes of at most 6 (a, b, c) tasks per lane of a wavefront
