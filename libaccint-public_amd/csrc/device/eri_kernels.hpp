@@ -1644,17 +1644,45 @@ __device__ __forceinline__ void wave_quartet_acc(const EriArgs& g, const PairRec
 #ifndef LAI_WAVE_TAB_KB
 #define LAI_WAVE_TAB_KB 12
 #endif
+// Tables per wave of the wavefront kernel, in KB: LAI_WAVE_TAB_KB, or 8 where
+// the larger budget would leave the class's LDS holding fewer wavefronts per
+// SIMD than its registers allow and the smaller one holds more (more resident
+// waves for fewer primitive quartets per step; measured 1.08x on (fp|ds),
+// (fs|dp) and (fd|ps), and slower on the classes it does not apply to).  The
+// occupancy arithmetic is that of WaveLds below.
+template <int La, int Lb, int Lc, int Ld>
+constexpr int wave_tab_kb() {
+  constexpr int na = ncart(La), nb = ncart(Lb), nc = ncart(Lc), nd = ncart(Ld);
+  constexpr int NR = (La + Lb + Lc + Ld) / 2 + 1;
+  constexpr int S = (La + 1) * (Lb + 1) * (Lc + 1) * (Ld + 1);
+  constexpr int TS = 3 * NR * (S | 1);
+  constexpr int NJ = (na * nb * nc + 63) / 64;
+  constexpr int NBLK = na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
+  constexpr int wpe_regs = (NJ * nd <= 9 && NJ <= 3) ? 3 : 2;
+  auto wpe_lds = [&](int kb) {
+    int npq = kb * 1024 / (TS * 8);
+    if (npq > 64 / (3 * NR)) npq = 64 / (3 * NR);
+    if (npq < 1) npq = 1;
+    const int nw = npq * TS * 8 <= kb * 1024 ? 4 : 2;
+    const int tabn = npq * TS > NJ * nd * 64 ? npq * TS : NJ * nd * 64;
+    return (160 * 1024 / (nw * (tabn + 2 * NBLK) * 8)) * nw / 4;
+  };
+  const int big = wpe_lds(LAI_WAVE_TAB_KB), small = wpe_lds(8);
+  return (big < wpe_regs && small > big) ? 8 : LAI_WAVE_TAB_KB;
+}
+
 template <int La, int Lb, int Lc, int Ld>
 struct WaveShape {
   static constexpr int NTASK = LdsTasks<La, Lb, Lc, Ld>::NTASK;
   static constexpr int NJ = (NTASK + 63) / 64;
+  static constexpr int KB = wave_tab_kb<La, Lb, Lc, Ld>();
   // table entries of one primitive quartet (odd stride per (root, axis))
   static constexpr int TS = 3 * LdsShape<La, Lb, Lc, Ld, double>::NR * (LdsShape<La, Lb, Lc, Ld, double>::S | 1);
-  // primitive quartets per wave and step, within LAI_WAVE_TAB_KB of tables per wave
+  // primitive quartets per wave and step, within KB of tables per wave
   static constexpr int NPQ = [] {
     using Sh = LdsShape<La, Lb, Lc, Ld, double>;
     constexpr int by_lanes = 64 / (3 * Sh::NR);
-    constexpr int by_lds = int((LAI_WAVE_TAB_KB * 1024) / (3 * Sh::NR * (Sh::S | 1) * sizeof(double)));
+    constexpr int by_lds = int((KB * 1024) / (3 * Sh::NR * (Sh::S | 1) * sizeof(double)));
     constexpr int n = by_lanes < by_lds ? by_lanes : by_lds;
     return n < 1 ? 1 : n;
   }();
@@ -1846,7 +1874,8 @@ struct ClassPolicy {
   static constexpr bool waves = !regs && LdsTasks<La, Lb, Lc, Ld>::NTASK <= 6 * 64;
   // four wavefronts, two where one wavefront's tables exceed the per-wave budget
   static constexpr int NW =
-      WaveShape<La, Lb, Lc, Ld>::NPQ * WaveShape<La, Lb, Lc, Ld>::TS * 8 <= LAI_WAVE_TAB_KB * 1024 ? 4 : 2;
+      WaveShape<La, Lb, Lc, Ld>::NPQ * WaveShape<La, Lb, Lc, Ld>::TS * 8 <= WaveShape<La, Lb, Lc, Ld>::KB * 1024 ? 4
+                                                                                                        : 2;
   // shape codes: 0 lane per quartet, 1 wavefront per quartet, 2 workgroup per quartet
   static constexpr int shape = regs ? 0 : (waves ? 1 : 2);
   // the second shape compiled for the autotuner: register classes that split
